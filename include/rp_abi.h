@@ -1,0 +1,224 @@
+/*
+ * rp_abi.h — C ABI of the MI355X DEM time-step engine (librp_hip.so).
+ *
+ * This is the drop-in boundary for the GPU path of rusty_particles.  The
+ * reference binds its GPU backend through three hooks:
+ *   - GpuSimulation::new(&[Particle], dt, bounds_min, bounds_max, periodic,
+ *     p_mat, w_mat, normal_model, tangential_model)   src/gpu.rs:61-71,
+ *     called from Simulation::enable_gpu               src/simulation.rs:129-146
+ *   - GpuSimulation::step(&self)                       src/gpu.rs:153-166,
+ *     called from Simulation::step                     src/simulation.rs:156-158
+ *   - GpuSimulation::read_particles(&self)             src/gpu.rs:168-213,
+ *     called from Simulation::sync_particles           src/simulation.rs:564-568
+ * Each entry point below names the reference item it replaces.
+ *
+ * Conventions
+ *   - Plain C types only.  No exception or panic crosses the ABI; every call
+ *     returns rp_status (0 = OK, < 0 = error) and rp_last_error() returns the
+ *     message of the last failing call on the calling thread.
+ *   - The library owns device memory; the caller owns every host array.
+ *   - Particle order at the ABI is always the reference's API order: insertion
+ *     order minus removed particles (the reference's retain + re-index,
+ *     src/simulation.rs:200-219).  The device keeps its own cell-sorted order
+ *     internally.
+ *   - One engine = one HIP stream on the current device.  An engine is not
+ *     thread-safe; use one engine per thread.
+ *   - All state is FP64.  The reference computes in f32 (glam Vec3, Cargo.toml:7);
+ *     FP64 is this engine's documented precision.
+ */
+#ifndef RP_ABI_H
+#define RP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rp_status;
+
+enum {
+  RP_OK = 0,
+  RP_ERR_INVALID = -1,  /* bad argument or parameter */
+  RP_ERR_DEVICE = -2,   /* no HIP device, or a HIP runtime error */
+  RP_ERR_NOMEM = -3,    /* device allocation failed */
+  RP_ERR_STATE = -4,    /* call not valid in the engine's current state */
+  RP_ERR_IO = -5        /* file could not be read or parsed */
+};
+
+/* Normal force models.  The values are the reference's enum discriminants,
+ * which its GPU path already treats as an ABI (src/physics.rs:6-17,
+ * src/gpu.rs:108). */
+enum {
+  RP_NORMAL_LINEAR = 0,      /* LinearSpringDashpot  physics.rs:49-70   */
+  RP_NORMAL_HERTZ = 1,       /* Hertzian             physics.rs:86-129  */
+  RP_NORMAL_HYSTERETIC = 2,  /* Hysteretic           physics.rs:334-380 */
+  RP_NORMAL_JKR = 3,         /* JKR                  physics.rs:132-219 */
+  RP_NORMAL_SJKR = 4         /* SimplifiedJKR        physics.rs:234-325 */
+};
+
+/* Tangential force models (src/physics.rs:20-28). */
+enum {
+  RP_TANGENTIAL_COULOMB = 0, /* physics.rs:385-397 */
+  RP_TANGENTIAL_LSC = 1,     /* LinearSpringCoulomb physics.rs:400-434 */
+  RP_TANGENTIAL_MINDLIN = 2  /* physics.rs:443-459 */
+};
+
+/* Semantics mode.  PARITY reproduces the reference's step_cpu exactly,
+ * quirks included (shear modulus placeholder 0, material.rs:49).  */
+enum { RP_MODE_PARITY = 0 };
+
+/* Material, field order as Material::new (src/material.rs:12) and the
+ * Python setter (src/lib.rs:109). */
+typedef struct rp_material {
+  double youngs_modulus;
+  double poissons_ratio;
+  double density;
+  double friction_coefficient;
+  double restitution_coefficient;
+  double surface_energy;
+} rp_material;
+
+/* Everything GpuSimulation::new receives apart from the particles
+ * (src/gpu.rs:61-71), plus the items the reference hard-codes. */
+typedef struct rp_params {
+  double dt;                 /* s */
+  double bounds_min[3];      /* m */
+  double bounds_max[3];      /* m */
+  int32_t periodic[3];       /* 0/1 per axis (Simulation.periodic) */
+  int32_t normal_model;      /* RP_NORMAL_*     */
+  int32_t tangential_model;  /* RP_TANGENTIAL_* */
+  int32_t mode;              /* RP_MODE_*       */
+  rp_material particle;      /* Simulation.particle_material */
+  rp_material wall;          /* Simulation.wall_material     */
+  double gravity[3];         /* (0,-9.81,0) = simulation.rs:267 */
+  double ref_cell_size;      /* 1.0 = simulation.rs:66; decides which particles
+                                the reference grid can see (grid.rs:84-94) */
+  double cell_size;          /* engine's own broad-phase cell, 0 = automatic */
+  double skin;               /* Verlet-list skin distance, 0 = automatic */
+} rp_params;
+
+/* Counters of the most recent rp_engine_step call (integers only). */
+typedef struct rp_stats {
+  uint64_t steps;              /* steps taken since create */
+  uint64_t rebuilds;           /* neighbour-list rebuilds since create */
+  uint64_t contacts;           /* particle-particle directed contacts, last step */
+  uint64_t wall_contacts;      /* particle-triangle contacts, last step */
+  uint64_t removed;            /* particles removed since create */
+  uint64_t jkr_failures;       /* JKR roots rejected by the residual check, last step
+                                  (physics.rs:193-199) */
+  int32_t excessive_overlap;   /* simulation.rs:361-370,469-473,528-530 */
+  int32_t neighbour_capacity;  /* current per-particle list capacity */
+  int32_t max_neighbours;      /* largest list length seen at a rebuild */
+  int32_t reserved;
+} rp_stats;
+
+typedef struct rp_engine rp_engine;
+
+/* Fills *p with the reference defaults of Simulation::new
+ * (src/simulation.rs:62-91): particle material (1e7,0.3,2500,0.5,0.5,0),
+ * wall material (1e9,0.3,7800,0.5,0.5,0), Hertzian + Mindlin, no periodic
+ * axis, gravity (0,-9.81,0), reference cell 1.0 m. */
+void rp_params_default(rp_params* p, double dt, const double bounds_min[3],
+                       const double bounds_max[3]);
+
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int32_t rp_device_count(void);
+
+/* Replaces GpuSimulation::new (src/gpu.rs:61-151) minus the particle upload.
+ * Fails with RP_ERR_DEVICE when no HIP device is present, the condition the
+ * reference maps to RuntimeError in enable_gpu (src/lib.rs:146-148). */
+rp_status rp_engine_create(const rp_params* params, rp_engine** out);
+
+/* Live parameter update.  The reference snapshots parameters at enable_gpu
+ * (src/gpu.rs:100-115); this engine applies later setter calls too. */
+rp_status rp_engine_set_params(rp_engine* e, const rp_params* params);
+
+/* Replaces the particle upload of GpuSimulation::new (src/gpu.rs:92-98).
+ * Arrays hold n particles in API order; x3/v3/w3/a3/tau3 are n*3 doubles,
+ * radius/mass n doubles, fixed n bytes (0/1).  Any of v3, w3, a3, tau3 may
+ * be NULL (zero).  Kahan residuals start at zero, as in Particle::new
+ * (src/particle.rs:22-37).  Replaces all particles held by the engine. */
+rp_status rp_engine_upload(rp_engine* e, size_t n, const double* x3,
+                           const double* v3, const double* w3, const double* a3,
+                           const double* tau3, const double* radius,
+                           const double* mass, const uint8_t* fixed);
+
+/* As rp_engine_upload, but also sets the three Kahan residual vectors
+ * (position, velocity, angular velocity; src/particle.rs:11-17). */
+rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3,
+                                const double* v3, const double* w3,
+                                const double* a3, const double* tau3,
+                                const double* rx3, const double* rv3,
+                                const double* rw3, const double* radius,
+                                const double* mass, const uint8_t* fixed);
+
+/* Appends one STL mesh, as Simulation::add_mesh (src/simulation.rs:98-100)
+ * after Mesh::load_stl (src/mesh.rs:125-143): ntri triangles, v9 = ntri*9
+ * floats (v0,v1,v2), n3 = ntri*3 floats (facet normal), STL float32. */
+rp_status rp_engine_add_mesh(rp_engine* e, size_t ntri, const float* v9,
+                             const float* n3);
+
+/* Reads a binary or ASCII STL file and appends it as rp_engine_add_mesh.
+ * RP_ERR_IO when the file cannot be read (the reference's IOError,
+ * src/lib.rs:87-91). */
+rp_status rp_engine_add_mesh_stl(rp_engine* e, const char* path);
+
+/* Replaces GpuSimulation::step (src/gpu.rs:153-166): advances nsteps time
+ * steps of Simulation::step_cpu semantics (src/simulation.rs:173-562).
+ * Enqueues work on the engine's stream and returns without a host sync per
+ * step. */
+rp_status rp_engine_step(rp_engine* e, uint64_t nsteps);
+
+/* Waits for all queued work of the engine. */
+rp_status rp_engine_synchronize(rp_engine* e);
+
+/* Number of particles after removals (Simulation.particles.len()). */
+rp_status rp_engine_count(rp_engine* e, size_t* n);
+
+/* Replaces read_particles (src/gpu.rs:168-213): synchronises and copies the
+ * state back in API order.  Any pointer may be NULL (skipped).  Unlike the
+ * reference, angular velocity is returned, not zeroed (gpu.rs:205-210). */
+rp_status rp_engine_download(rp_engine* e, double* x3, double* v3, double* w3,
+                             double* a3, double* tau3);
+
+/* Kahan residuals, radius, mass and fixed flags in API order (NULL skips). */
+rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3,
+                                 double* rw3, double* radius, double* mass,
+                                 uint8_t* fixed);
+
+/* Turns recording of the pair list on or off (off by default).  When on,
+ * the last step of each rp_engine_step call records its contact pairs for
+ * rp_engine_pairs. */
+rp_status rp_engine_set_record(rp_engine* e, int on);
+
+/* Reads an STL file (binary or ASCII, as Mesh::load_stl via stl_io,
+ * src/mesh.rs:125-143) without an engine.  Call with v9 = n3 = NULL to get
+ * the triangle count in *ntri, then with arrays of ntri*9 and ntri*3 floats.
+ * RP_ERR_IO when the file cannot be read. */
+rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
+
+/* Particle-particle contact list of the most recent step, in API order and,
+ * per particle, in the reference's neighbour order (grid.rs:102-141 cell
+ * walk, ascending index inside a cell; simulation.rs:317-357).  A pair the
+ * reference's grid walk visits twice is listed twice.  offsets has n+1
+ * entries; js receives up to cap entries; *total receives the full count.
+ * Returns RP_ERR_INVALID (with *total set) when cap is too small. */
+rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
+                          size_t cap, size_t* total);
+
+/* Counters (rp_stats). */
+rp_status rp_engine_stats(rp_engine* e, rp_stats* out);
+
+/* Releases every device resource of the engine. */
+void rp_engine_destroy(rp_engine* e);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RP_ABI_H */

@@ -1,0 +1,1634 @@
+// rp_engine.hip — MI355X (gfx950) DEM time step behind the rp_abi.h C ABI.
+//
+// One engine holds one simulation domain on one GPU.  Per time step it runs
+// the reference's Simulation::step_cpu (src/simulation.rs:173-562) as:
+//
+//   rebuild chain (runs only when a device-side flag asks for it; otherwise
+//   every kernel of the chain returns at once):
+//     k_keys        fine-cell key + per-cell histogram (integer atomics)
+//     k_scan_*      exclusive scan of the histogram -> cell start offsets
+//     k_scatter     particles into their cell's range (order within the cell
+//                   still arbitrary)
+//     k_rank_permute rank inside the cell by stable id (deterministic), move
+//                   the particle state to its new slot, drop removed ones
+//     k_build       Verlet neighbour list per particle (cutoff + skin), the
+//                   last workgroup to finish swaps the state buffers
+//   k_force         gather over the neighbour list: exact pair predicate,
+//                   reference-grid visibility, normal + tangential laws,
+//                   triangles of the particle's cell, then a = g + F/m,
+//                   tau = T and the second half kick; when another step
+//                   follows in the same call, also the next step's periodic
+//                   wrap, removal test, first half kick and drift, so the
+//                   state makes one round trip through HBM per step.
+//
+// Memory layout (HBM): structure of arrays, FP64, each vec3 field stored as
+// three planes of `cap` doubles; two copies of the persistent state (x, v, w
+// and their Kahan residuals, radius, mass, flags, stable id) swap at each
+// rebuild; a, tau and the neighbour lists (column-major: entry k of slot i
+// at nbr[k*cap + i]) are single copies.  No float atomics anywhere: results
+// are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/rp_abi.h"
+#include "rp_physics.hpp"
+#include "rp_stl.hpp"
+
+using namespace rp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+rp_status fail(rp_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess)                                                                     \
+      return fail(RP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));          \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr uint8_t FLAG_FIXED = 1;
+constexpr uint8_t FLAG_DEAD = 2;
+constexpr int kScanItems = 16;                   // items per thread in the scan
+constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
+constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
+constexpr double kPi = 3.14159265358979323846;
+
+// Per-step particle state of one buffer (device pointers, `cap` slots).
+// The force kernel reads one buffer and writes the other, so no thread can
+// see a neighbour's next-step values while gathering this step's forces.
+struct Dyn {
+  double* x;   // 3 planes
+  double* v;
+  double* w;
+  double* rx;  // Kahan residuals of x, v, w
+  double* rv;
+  double* rw;
+  double* a;   // acceleration and torque of the last step; written only by
+  double* tau; // a step that ends at rest, read by the next call's first half
+};
+
+// Per-particle constants of one buffer; swapped only when a rebuild
+// re-orders the particles.
+struct Stat {
+  double* r;
+  double* m;
+  uint8_t* flags;
+  uint32_t* sid;  // stable id = insertion index (API order key)
+};
+
+// Device-resident control block.
+struct Ctl {
+  Dyn dyn[2];
+  Stat stat[2];
+  int cur_dyn;             // current Dyn buffer
+  int cur_stat;            // current Stat buffer
+  int n;                   // slots in use (alive + removed-not-yet-compacted)
+  int alive;               // alive particles
+  int rebuild[2];          // rebuild requests, indexed by step parity
+  unsigned int done;       // finished workgroups of k_build
+  unsigned int done_force; // finished workgroups of k_force
+  int overflow;            // neighbour list overflow seen
+  int max_count;           // largest candidate count seen by k_build
+  int overlap;             // excessive-overlap flag of the last step
+  unsigned long long contacts;       // directed P-P contacts (with multiplicity), last step
+  unsigned long long wall_contacts;  // particle-triangle contacts, last step
+  unsigned long long rebuilds;       // rebuilds done
+  unsigned long long jkr_failures;   // JKR residual rejections, last step
+  int pair_overflow;       // pair record overflow
+};
+
+// Everything a kernel needs that is fixed between parameter changes.
+struct KParams {
+  double dt;
+  double bmin[3], bmax[3], bsize[3];
+  int periodic[3];
+  int any_periodic;
+  int normal, tangential;
+  double g[3];
+  MaterialConsts mc;
+  double margin_factor;  // 1.5 for JKR/sJKR (r_sum + 0.5 r_sum), else 1.0
+  // reference grid (grid.rs:12-22, cell 1.0 m at simulation.rs:66)
+  double ref_origin[3];
+  double ref_cs;
+  int ref_dims[3];
+  // fine grid
+  double go[3];      // origin
+  double ginv[3];    // 1 / cell size
+  int gn[3];         // cells per axis
+  int gstride[3];    // linear index strides
+  unsigned int ncells;
+  double skin;
+  double half_skin;
+  double r_max;
+  int cap;           // slot capacity (plane stride)
+  int nbr_cap;       // neighbour list capacity per particle
+  int pair_cap;      // recorded pair entries per particle
+};
+
+// ---- small device helpers ---------------------------------------------------
+
+__device__ inline dv3 ld3(const double* p, int i, int cap) { return mk(p[i], p[i + cap], p[i + 2 * cap]); }
+__device__ inline void st3(double* p, int i, int cap, dv3 v) {
+  p[i] = v.x;
+  p[i + cap] = v.y;
+  p[i + 2 * cap] = v.z;
+}
+__host__ __device__ inline double comp(dv3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Fine-cell coordinate along one axis: periodic axes wrap, others clamp.
+__device__ inline int fine_coord(double x, int k, const KParams& P, bool clamp_only) {
+  double q = floor((x - P.go[k]) * P.ginv[k]);
+  int n = P.gn[k];
+  if (!(q == q)) return 0;
+  if (q < -1.0e9) q = -1.0e9;
+  if (q > 1.0e9) q = 1.0e9;
+  long long c = (long long)q;
+  if (P.periodic[k] && !clamp_only) {
+    c %= n;
+    if (c < 0) c += n;
+    return (int)c;
+  }
+  if (c < 0) return 0;
+  if (c >= n) return n - 1;
+  return (int)c;
+}
+
+// Reference-grid cell coordinate (grid.rs:84-86): floor((x - origin)/cs) as i32.
+__device__ inline int ref_coord(double x, int k, const KParams& P) {
+  return sat_i32(floor((x - P.ref_origin[k]) / P.ref_cs));
+}
+
+// Number of offsets o in {-1,0,1} whose wrapped reference neighbour cell of
+// ci equals cj (grid.rs:108-129).  Also returns a bit mask of matching offsets
+// (bit o+1).
+__device__ inline int ref_matches(int ci, int cj, int k, const KParams& P, int* mask) {
+  int cnt = 0, msk = 0;
+  int d = P.ref_dims[k];
+  for (int o = -1; o <= 1; ++o) {
+    int nn = wrap_add_i32(ci, o);
+    if (P.periodic[k]) {
+      if (nn < 0)
+        nn += d;
+      else if (nn >= d)
+        nn -= d;
+    }
+    if (nn == cj) {
+      ++cnt;
+      msk |= 1 << (o + 1);
+    }
+  }
+  *mask = msk;
+  return cnt;
+}
+
+// XCD-aware block index: consecutive logical blocks land on the same XCD
+// (hardware dispatches block b to XCD b % 8), so each XCD's L2 sees a
+// contiguous range of cell-sorted particles and their neighbours.
+__device__ inline int xcd_block(int b, int nb) {
+  int q = nb / 8, r = nb % 8;
+  int x = b % 8, k = b / 8;
+  return x * q + min(x, r) + k;
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// ---- rebuild chain ----------------------------------------------------------
+
+// Keys + histogram.  Also clears the other parity's rebuild request (it was
+// last read one step ago and is next written by this step's force kernel).
+__global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, uint32_t* key,
+                                                 uint32_t* cell_count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rebuild[par ^ 1] = 0;
+  if (!ctl->rebuild[par]) return;
+  const Dyn S = ctl->dyn[ctl->cur_dyn];
+  const Stat T = ctl->stat[ctl->cur_stat];
+  const int n = ctl->n;
+  const int cap = P.cap;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t c;
+    if (T.flags[i] & FLAG_DEAD) {
+      c = P.ncells;
+    } else {
+      int cx = fine_coord(S.x[i], 0, P, false);
+      int cy = fine_coord(S.x[i + cap], 1, P, false);
+      int cz = fine_coord(S.x[i + 2 * cap], 2, P, false);
+      c = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+    }
+    key[i] = c;
+    atomicAdd(&cell_count[c], 1u);
+  }
+}
+
+// Exclusive scan of cell_count[0..m) into cell_start[0..m], cell_start[m] = total.
+__global__ void __launch_bounds__(kBlock) k_scan_sums(const Ctl* ctl, int par, const uint32_t* in, int m,
+                                                      uint32_t* block_sums) {
+  if (!ctl->rebuild[par]) return;
+  __shared__ uint32_t red[kBlock];
+  for (int b = blockIdx.x; b * kScanChunk < m; b += gridDim.x) {
+    uint32_t s = 0;
+    int base = b * kScanChunk + threadIdx.x;
+    for (int k = 0; k < kScanItems; ++k) {
+      int idx = base + k * kBlock;
+      if (idx < m) s += in[idx];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) block_sums[b] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(const Ctl* ctl, int par, uint32_t* block_sums, int nb) {
+  if (!ctl->rebuild[par]) return;
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int idx = base + threadIdx.x;
+    uint32_t v = idx < nb ? block_sums[idx] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      uint32_t t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    uint32_t incl = buf[threadIdx.x];
+    if (idx < nb) block_sums[idx] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const Ctl* ctl, int par, const uint32_t* in, int m,
+                                                       const uint32_t* block_offsets, uint32_t* out) {
+  if (!ctl->rebuild[par]) return;
+  __shared__ uint32_t tsum[kBlock];
+  for (int b = blockIdx.x; b * kScanChunk < m; b += gridDim.x) {
+    // thread t owns items [b*chunk + t*items, +items): contiguous per thread
+    int base = b * kScanChunk + threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+      int idx = base + k;
+      vals[k] = idx < m ? in[idx] : 0;
+      s += vals[k];
+    }
+    tsum[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      uint32_t t = threadIdx.x >= off ? tsum[threadIdx.x - off] : 0;
+      __syncthreads();
+      tsum[threadIdx.x] += t;
+      __syncthreads();
+    }
+    uint32_t run = block_offsets[b] + tsum[threadIdx.x] - s;
+    for (int k = 0; k < kScanItems; ++k) {
+      int idx = base + k;
+      if (idx < m) out[idx] = run;
+      if (idx == m - 1) out[m] = run + vals[k];
+      run += vals[k];
+    }
+    __syncthreads();
+  }
+}
+
+// Scatter into cell ranges; the decrementing cursor leaves cell_count zeroed
+// for the next rebuild.
+__global__ void __launch_bounds__(kBlock) k_scatter(const Ctl* ctl, int par, const uint32_t* key,
+                                                    const uint32_t* cell_start, uint32_t* cell_count,
+                                                    uint32_t* tmp) {
+  if (!ctl->rebuild[par]) return;
+  const int n = ctl->n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t c = key[i];
+    uint32_t pos = cell_start[c] + atomicSub(&cell_count[c], 1u) - 1u;
+    tmp[pos] = (uint32_t)i;
+  }
+}
+
+// Deterministic slot = cell start + rank by stable id among the cell's
+// particles; copies the persistent state into the other buffer.  Removed
+// particles (sentinel cell) are dropped.
+__global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KParams P, const uint32_t* key,
+                                                         const uint32_t* cell_start, const uint32_t* tmp) {
+  if (!ctl->rebuild[par]) return;
+  const Dyn S = ctl->dyn[ctl->cur_dyn];
+  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
+  const Stat T = ctl->stat[ctl->cur_stat];
+  const Stat U = ctl->stat[ctl->cur_stat ^ 1];
+  const int n = ctl->n;
+  const int cap = P.cap;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t c = key[i];
+    if (c >= P.ncells) continue;
+    uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
+    uint32_t my = T.sid[i];
+    uint32_t rank = 0;
+    for (uint32_t p = s0; p < s1; ++p) rank += (T.sid[tmp[p]] < my) ? 1u : 0u;
+    int d = (int)(s0 + rank);
+    for (int k = 0; k < 3; ++k) {
+      int o = k * cap;
+      D.x[d + o] = S.x[i + o];
+      D.v[d + o] = S.v[i + o];
+      D.w[d + o] = S.w[i + o];
+      D.rx[d + o] = S.rx[i + o];
+      D.rv[d + o] = S.rv[i + o];
+      D.rw[d + o] = S.rw[i + o];
+      D.a[d + o] = S.a[i + o];
+      D.tau[d + o] = S.tau[i + o];
+    }
+    U.r[d] = T.r[i];
+    U.m[d] = T.m[i];
+    U.flags[d] = T.flags[i];
+    U.sid[d] = my;
+  }
+}
+
+// Verlet list of slot i (in the new buffer): every j with
+// |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin.  The last workgroup to
+// finish makes the new buffer current and records the new slot count.
+__global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, const uint32_t* cell_start,
+                                                  int* nbr_count, int* nbr, float* xb) {
+  if (!ctl->rebuild[par]) return;
+  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
+  const Stat U = ctl->stat[ctl->cur_stat ^ 1];
+  const int n = (int)cell_start[P.ncells];  // alive particles after compaction
+  const int cap = P.cap;
+  int local_max = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    dv3 xi = ld3(D.x, i, cap);
+    double ri = U.r[i];
+    int ci[3] = {fine_coord(xi.x, 0, P, false), fine_coord(xi.y, 1, P, false), fine_coord(xi.z, 2, P, false)};
+    double lim_max = P.margin_factor * (ri + P.r_max) + P.skin;
+    int lo[3], hi[3];
+    bool all[3];
+    for (int k = 0; k < 3; ++k) {
+      int kk = (int)ceil(lim_max * P.ginv[k]);
+      if (kk < 1) kk = 1;
+      if (P.periodic[k]) {
+        all[k] = (2 * kk + 1 >= P.gn[k]);
+        lo[k] = all[k] ? 0 : ci[k] - kk;
+        hi[k] = all[k] ? P.gn[k] - 1 : ci[k] + kk;
+      } else {
+        all[k] = false;
+        lo[k] = max(0, ci[k] - kk);
+        hi[k] = min(P.gn[k] - 1, ci[k] + kk);
+      }
+    }
+    int cnt = 0;
+    for (int cz = lo[2]; cz <= hi[2]; ++cz) {
+      int wz = cz;
+      if (P.periodic[2] && !all[2]) wz = (cz % P.gn[2] + P.gn[2]) % P.gn[2];
+      for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+        int wy = cy;
+        if (P.periodic[1] && !all[1]) wy = (cy % P.gn[1] + P.gn[1]) % P.gn[1];
+        for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+          int wx = cx;
+          if (P.periodic[0] && !all[0]) wx = (cx % P.gn[0] + P.gn[0]) % P.gn[0];
+          uint32_t c = (uint32_t)(wx * P.gstride[0] + wy * P.gstride[1] + wz * P.gstride[2]);
+          uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
+          for (uint32_t j = s0; j < s1; ++j) {
+            if ((int)j == i) continue;
+            dv3 xj = ld3(D.x, (int)j, cap);
+            dv3 d = sub(xi, xj);
+            if (P.periodic[0]) {
+              if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
+              else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
+            }
+            if (P.periodic[1]) {
+              if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
+              else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
+            }
+            if (P.periodic[2]) {
+              if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
+              else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
+            }
+            double lim = P.margin_factor * (ri + U.r[j]) + P.skin;
+            if (dot(d, d) < lim * lim) {
+              if (cnt < P.nbr_cap) nbr[cnt * cap + i] = (int)j;
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    nbr_count[i] = min(cnt, P.nbr_cap);
+    local_max = max(local_max, cnt);
+    xb[i] = (float)xi.x;
+    xb[i + cap] = (float)xi.y;
+    xb[i + 2 * cap] = (float)xi.z;
+  }
+  if (local_max > 0) {
+    atomicMax(&ctl->max_count, local_max);
+    if (local_max > P.nbr_cap) atomicOr(&ctl->overflow, 1);
+  }
+  // last workgroup: swap buffers
+  __syncthreads();
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(&ctl->done, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    ctl->done = 0;
+    ctl->cur_dyn ^= 1;
+    ctl->cur_stat ^= 1;
+    ctl->n = n;
+    ctl->rebuilds += 1;
+  }
+}
+
+// ---- per-step kernels ---------------------------------------------------------
+
+// Start of step s+1 for one particle, from its at-rest state after step s:
+// periodic wrap (simulation.rs:182-197), removal test (:200-214), first half
+// kick, drift and angular half kick (:235-263).  Returns false when removed.
+__device__ inline bool begin_step(const KParams& P, dv3& x, dv3& v, dv3& w, dv3& rx, dv3& rv, dv3& rw, dv3 a,
+                                  dv3 tau, double r, double m, uint8_t flags) {
+  if (P.any_periodic) {
+    if (P.periodic[0]) {
+      if (x.x < P.bmin[0]) x.x += P.bsize[0];
+      else if (x.x >= P.bmax[0]) x.x -= P.bsize[0];
+    }
+    if (P.periodic[1]) {
+      if (x.y < P.bmin[1]) x.y += P.bsize[1];
+      else if (x.y >= P.bmax[1]) x.y -= P.bsize[1];
+    }
+    if (P.periodic[2]) {
+      if (x.z < P.bmin[2]) x.z += P.bsize[2];
+      else if (x.z >= P.bmax[2]) x.z -= P.bsize[2];
+    }
+  }
+  bool keep = (P.periodic[0] || (x.x >= P.bmin[0] - 1.0 && x.x <= P.bmax[0] + 1.0)) &&
+              (P.periodic[1] || (x.y >= P.bmin[1] - 1.0)) &&
+              (P.periodic[2] || (x.z >= P.bmin[2] - 1.0 && x.z <= P.bmax[2] + 1.0));
+  if (!keep) return false;
+  if (!(flags & FLAG_FIXED)) {
+    const double dt = P.dt;
+    kahan1(v.x, rv.x, (0.5 * a.x) * dt);
+    kahan1(v.y, rv.y, (0.5 * a.y) * dt);
+    kahan1(v.z, rv.z, (0.5 * a.z) * dt);
+    kahan1(x.x, rx.x, v.x * dt);
+    kahan1(x.y, rx.y, v.y * dt);
+    kahan1(x.z, rx.z, v.z * dt);
+    double inertia = 0.4 * m * r * r;
+    kahan1(w.x, rw.x, (0.5 * (tau.x / inertia)) * dt);
+    kahan1(w.y, rw.y, (0.5 * (tau.y / inertia)) * dt);
+    kahan1(w.z, rw.z, (0.5 * (tau.z / inertia)) * dt);
+  }
+  return true;
+}
+
+// Displacement since the last list build, minimum image on periodic axes.
+__device__ inline bool moved_too_far(const KParams& P, dv3 x, const float* xb, int i) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double d = comp(x, k) - (double)xb[i + k * P.cap];
+    if (P.periodic[k]) {
+      if (d > P.bsize[k] * 0.5) d -= P.bsize[k];
+      else if (d < -P.bsize[k] * 0.5) d += P.bsize[k];
+    }
+    d2 += d * d;
+  }
+  return !(d2 < P.half_skin * P.half_skin);
+}
+
+// A removed particle keeps slot i until the next rebuild drops it; its
+// position becomes NaN in the buffer written now, so every later pair test
+// against it is false.
+__device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i, int cap, uint8_t flags) {
+  T.flags[i] = flags | FLAG_DEAD;
+  const double qnan = __builtin_nan("");
+  D.x[i] = qnan;
+  D.x[i + cap] = qnan;
+  D.x[i + 2 * cap] = qnan;
+  atomicSub(&ctl->alive, 1);
+}
+
+// First half of a step from the at-rest state (used once at the start of
+// each rp_engine_step call).
+__global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, const float* xb) {
+  const Dyn S = ctl->dyn[ctl->cur_dyn];
+  const Stat T = ctl->stat[ctl->cur_stat];
+  const int n = ctl->n;
+  const int cap = P.cap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool far = false;
+  if (i < n) {
+    uint8_t flags = T.flags[i];
+    if (!(flags & FLAG_DEAD)) {
+      dv3 x = ld3(S.x, i, cap), v = ld3(S.v, i, cap), w = ld3(S.w, i, cap);
+      dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
+      dv3 a = ld3(S.a, i, cap), tau = ld3(S.tau, i, cap);
+      if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, T.r[i], T.m[i], flags)) {
+        mark_removed(ctl, T, S, i, cap, flags);
+      } else {
+        st3(S.x, i, cap, x);
+        st3(S.v, i, cap, v);
+        st3(S.w, i, cap, w);
+        st3(S.rx, i, cap, rx);
+        st3(S.rv, i, cap, rv);
+        st3(S.rw, i, cap, rw);
+        far = moved_too_far(P, x, xb, i);
+      }
+    }
+  }
+  if (__any(far) && (threadIdx.x & 63) == 0) atomicOr(&ctl->rebuild[par], 1);
+}
+
+// Gather force kernel + integration epilogue.
+// ADVANCE: also performs the next step's first half (begin_step) and leaves
+// the state drifted; otherwise stores a, tau and leaves the at-rest state.
+// RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
+template <bool ADVANCE, bool RECORD>
+__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, const int* nbr_count,
+                                                  const int* nbr, const Tri* tris, const uint32_t* tri_start,
+                                                  const uint32_t* tri_idx, const float* xb,
+                                                  unsigned long long* pairbuf,
+                                                  int* paircount) {
+  const Dyn S = ctl->dyn[ctl->cur_dyn];
+  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
+  const Stat T = ctl->stat[ctl->cur_stat];
+  const int n = ctl->n;
+  const int cap = P.cap;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  unsigned long long ncont = 0, nwall = 0, njkr = 0;
+  bool flag_overlap = false, far = false;
+  if (i < n) {
+    const uint8_t flags = T.flags[i];
+    if (flags & FLAG_DEAD) {
+      const double qnan = __builtin_nan("");
+      D.x[i] = qnan;
+      D.x[i + cap] = qnan;
+      D.x[i + 2 * cap] = qnan;
+    } else {
+      const dv3 xi = ld3(S.x, i, cap);
+      const dv3 vi = ld3(S.v, i, cap);
+      const dv3 wi = ld3(S.w, i, cap);
+      const double ri = T.r[i];
+      const double mi = T.m[i];
+      const bool adhesive = P.normal == RP_NORMAL_JKR || P.normal == RP_NORMAL_SJKR;
+      int rci[3] = {ref_coord(xi.x, 0, P), ref_coord(xi.y, 1, P), ref_coord(xi.z, 2, P)};
+      dv3 f = zero3(), t = zero3();
+      int npair = 0;
+      const int cnt = nbr_count[i];
+      for (int k = 0; k < cnt; ++k) {
+        const int j = nbr[k * cap + i];
+        const dv3 xj = ld3(S.x, j, cap);
+        const double rj = T.r[j];
+        dv3 d = sub(xi, xj);
+        if (P.periodic[0]) {
+          if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
+          else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
+        }
+        if (P.periodic[1]) {
+          if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
+          else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
+        }
+        if (P.periodic[2]) {
+          if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
+          else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
+        }
+        const double dsq = dot(d, d);
+        const double rsum = ri + rj;
+        const double margin = adhesive ? rsum * 0.5 : 0.0;
+        const double cutoff = rsum + margin;
+        if (!(dsq < cutoff * cutoff)) continue;
+        // Visible to the reference grid, and reached by its 3x3x3 walk how many times?
+        int mult = 1, masks[3] = {0, 0, 0};
+        for (int kk = 0; kk < 3; ++kk) {
+          int cj = ref_coord(comp(xj, kk), kk, P);
+          if (cj < 0 || cj >= P.ref_dims[kk]) {
+            mult = 0;
+            break;
+          }
+          mult *= ref_matches(rci[kk], cj, kk, P, &masks[kk]);
+        }
+        if (mult == 0) continue;
+        const double dist = sqrt(dsq);
+        const double overlap = rsum - dist;
+        if (overlap > 0.2 * fmin(ri, rj)) flag_overlap = true;
+        const dv3 nrm = dist > 1e-6 ? divs(d, dist) : mk(0.0, 1.0, 0.0);
+        const dv3 ra = mul(neg(nrm), ri);
+        const dv3 rb = mul(nrm, rj);
+        const dv3 vj = ld3(S.v, j, cap);
+        const dv3 wj = ld3(S.w, j, cap);
+        const double mj = T.m[j];
+        const dv3 vrel = sub(add(vi, cross(wi, ra)), add(vj, cross(wj, rb)));
+        const double rstar = (ri * rj) / (ri + rj);
+        const double mstar = (mi * mj) / (mi + mj);
+        dv3 fn;
+        int jfail = 0;
+        switch (P.normal) {
+          case RP_NORMAL_HERTZ: fn = hertz(overlap, nrm, vrel, P.mc.pp, rstar, mstar, P.mc.beta_abs); break;
+          case RP_NORMAL_LINEAR: {
+            double kn = P.mc.pp.estar * rstar;
+            double gn = 0.5 * sqrt(mstar * kn);
+            fn = linear_sd(overlap, nrm, vrel, kn, gn);
+            break;
+          }
+          case RP_NORMAL_HYSTERETIC: fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0); break;
+          case RP_NORMAL_JKR: fn = jkr(overlap, nrm, P.mc.pp, rstar, &jfail); break;
+          default: fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar); break;
+        }
+        njkr += (unsigned long long)(jfail * mult);
+        const double fnmag = len(fn);
+        dv3 ft;
+        if (P.tangential == RP_TANGENTIAL_COULOMB) {
+          ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+        } else {
+          // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
+          double kt = P.tangential == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
+          ft = spring_coulomb_g0(fnmag, kt, P.mc.mu);
+        }
+        const dv3 fc = add(fn, ft);
+        const dv3 tc = cross(ra, fc);
+        for (int rep = 0; rep < mult; ++rep) {
+          f = add(f, fc);
+          t = add(t, tc);
+        }
+        ncont += (unsigned long long)mult;
+        if (RECORD) {
+          const unsigned long long sj = T.sid[j];
+          for (int oz = 0; oz < 3; ++oz) {
+            if (!(masks[2] >> oz & 1)) continue;
+            for (int oy = 0; oy < 3; ++oy) {
+              if (!(masks[1] >> oy & 1)) continue;
+              for (int ox = 0; ox < 3; ++ox) {
+                if (!(masks[0] >> ox & 1)) continue;
+                unsigned long long rank = (unsigned long long)(oz * 9 + oy * 3 + ox);
+                if (npair < P.pair_cap) pairbuf[(size_t)i * P.pair_cap + npair] = (rank << 32) | sj;
+                else atomicOr(&ctl->pair_overflow, 1);
+                ++npair;
+              }
+            }
+          }
+        }
+      }
+      if (RECORD) paircount[i] = min(npair, P.pair_cap);
+      // walls: triangles binned into this particle's (clamped) fine cell
+      if (P.ncells > 0) {
+        int cx = fine_coord(xi.x, 0, P, true), cy = fine_coord(xi.y, 1, P, true), cz = fine_coord(xi.z, 2, P, true);
+        uint32_t c = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+        uint32_t t0 = tri_start[c], t1 = tri_start[c + 1];
+        for (uint32_t q = t0; q < t1; ++q) {
+          const Tri tr = tris[tri_idx[q]];
+          const dv3 cp = closest_point(tr, xi);
+          const dv3 dv = sub(xi, cp);
+          const double dsq = dot(dv, dv);
+          if (!(dsq < ri * ri)) continue;
+          const double dist = sqrt(dsq);
+          const double pen = ri - dist;
+          const dv3 nrm = dist > 1e-6 ? divs(dv, dist) : tr.n;
+          if (pen > 0.2 * ri) flag_overlap = true;
+          const dv3 ra = mul(neg(nrm), ri);
+          const dv3 vrel = add(vi, cross(wi, ra));
+          dv3 fn;
+          int jfail = 0;
+          switch (P.normal) {
+            case RP_NORMAL_HERTZ: fn = hertz(pen, nrm, vrel, P.mc.pw, ri, mi, P.mc.beta_abs); break;
+            case RP_NORMAL_LINEAR: {
+              double kn = P.mc.pw.estar * ri;
+              double gn = 0.5 * sqrt(mi * kn);
+              fn = linear_sd(pen, nrm, vrel, kn, gn);
+              break;
+            }
+            case RP_NORMAL_HYSTERETIC: fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * ri, 0.0); break;
+            case RP_NORMAL_JKR: fn = jkr(pen, nrm, P.mc.pw, ri, &jfail); break;
+            default: fn = sjkr(pen, nrm, vrel, P.mc.pw, ri); break;
+          }
+          njkr += (unsigned long long)jfail;
+          const double fnmag = len(fn);
+          const dv3 ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+          const dv3 fc = add(fn, ft);
+          f = add(f, fc);
+          t = add(t, cross(ra, fc));
+          ++nwall;
+        }
+      }
+      // apply (simulation.rs:533-536) and second half kick (:543-561)
+      const dv3 a = mk(P.g[0] + f.x / mi, P.g[1] + f.y / mi, P.g[2] + f.z / mi);
+      const dv3 tau = mk(0.0 + t.x, 0.0 + t.y, 0.0 + t.z);
+      dv3 v = vi, w = wi;
+      dv3 rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
+      if (!(flags & FLAG_FIXED)) {
+        const double dt = P.dt;
+        kahan1(v.x, rv.x, (0.5 * a.x) * dt);
+        kahan1(v.y, rv.y, (0.5 * a.y) * dt);
+        kahan1(v.z, rv.z, (0.5 * a.z) * dt);
+        const double inertia = 0.4 * mi * ri * ri;
+        kahan1(w.x, rw.x, (0.5 * (tau.x / inertia)) * dt);
+        kahan1(w.y, rw.y, (0.5 * (tau.y / inertia)) * dt);
+        kahan1(w.z, rw.z, (0.5 * (tau.z / inertia)) * dt);
+      }
+      dv3 x = xi, rx = ld3(S.rx, i, cap);
+      bool alive = true;
+      if (ADVANCE) {
+        if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, ri, mi, flags)) {
+          mark_removed(ctl, T, D, i, cap, flags);
+          alive = false;
+        } else {
+          far = moved_too_far(P, x, xb, i);
+        }
+      } else {
+        st3(D.a, i, cap, a);
+        st3(D.tau, i, cap, tau);
+      }
+      if (alive) {
+        st3(D.x, i, cap, x);
+        st3(D.v, i, cap, v);
+        st3(D.w, i, cap, w);
+        st3(D.rx, i, cap, rx);
+        st3(D.rv, i, cap, rv);
+        st3(D.rw, i, cap, rw);
+      }
+    }
+  }
+  // counters: one atomic per wave
+  unsigned long long c1 = wave_sum_u64(ncont), c2 = wave_sum_u64(nwall), c3 = wave_sum_u64(njkr);
+  bool anyo = __any(flag_overlap), anyf = __any(far);
+  if ((threadIdx.x & 63) == 0) {
+    if (c1) atomicAdd(&ctl->contacts, c1);
+    if (c2) atomicAdd(&ctl->wall_contacts, c2);
+    if (c3) atomicAdd(&ctl->jkr_failures, c3);
+    if (anyo) atomicOr(&ctl->overlap, 1);
+    if (ADVANCE && anyf) atomicOr(&ctl->rebuild[par ^ 1], 1);
+  }
+  // last workgroup: the buffer just written becomes current
+  __syncthreads();
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(&ctl->done_force, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    ctl->done_force = 0;
+    ctl->cur_dyn ^= 1;
+  }
+}
+
+__global__ void k_reset_step_counters(Ctl* ctl) {
+  ctl->contacts = 0;
+  ctl->wall_contacts = 0;
+  ctl->jkr_failures = 0;
+  ctl->overlap = 0;
+}
+
+__global__ void k_request_rebuild(Ctl* ctl, int par) {
+  ctl->rebuild[par] = 1;
+  ctl->rebuild[par ^ 1] = 0;
+}
+
+// ---- host side ----------------------------------------------------------------
+
+struct HostMesh {
+  std::vector<float> v9, n3;
+};
+
+size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+struct rp_engine {
+  rp_params prm{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cap = 0;         // slot capacity
+  int n_uploaded = 0;
+  double r_max = 0.0, r_mean = 0.0;
+  uint64_t step_count = 0;
+  bool have_particles = false;
+  bool geometry_dirty = true;
+  bool record = false;
+  KParams kp{};
+  Ctl host_ctl{};
+  Ctl* d_ctl = nullptr;
+  // buffers
+  void* d_dyn_mem[2] = {nullptr, nullptr};
+  void* d_stat_mem[2] = {nullptr, nullptr};
+  float* d_xb = nullptr;
+  uint32_t* d_key = nullptr;
+  uint32_t* d_tmp = nullptr;
+  uint32_t* d_cell_count = nullptr;
+  uint32_t* d_cell_start = nullptr;
+  uint32_t* d_block_sums = nullptr;
+  size_t cells_alloc = 0;
+  int* d_nbr_count = nullptr;
+  int* d_nbr = nullptr;
+  int nbr_cap = 0;
+  Tri* d_tris = nullptr;
+  uint32_t* d_tri_start = nullptr;
+  uint32_t* d_tri_idx = nullptr;
+  std::vector<HostMesh> meshes;
+  unsigned long long* d_pairbuf = nullptr;
+  int* d_paircount = nullptr;
+  int pair_cap = 0;
+  bool pairs_valid = false;
+};
+
+namespace {
+
+void free_ptr(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+MaterialConsts material_consts(const rp_params& p) {
+  auto pair = [](double e1, double nu1, double e2, double nu2, double g1, double g2) {
+    PairConsts c{};
+    double inv = (1.0 - nu1 * nu1) / e1 + (1.0 - nu2 * nu2) / e2;
+    c.estar = 1.0 / inv;
+    c.k43e = 4.0 / 3.0 * c.estar;
+    c.twoe = 2.0 * c.estar;
+    c.gamma = (g1 + g2) * 0.5;
+    double e8pg = 8.0 * kPi * c.gamma;
+    c.jc = e8pg / c.estar;
+    c.jsqrtc = std::sqrt(c.jc);
+    c.j8pg_e = e8pg * c.estar;
+    c.jpull = -2.0 / 3.0 * kPi * c.gamma;
+    c.spull = 1.5 * kPi * c.gamma;
+    c.sterm = 3.0 * (kPi * kPi) * (c.gamma * c.gamma);
+    c.estar_sq = c.estar * c.estar;
+    return c;
+  };
+  const rp_material& pm = p.particle;
+  const rp_material& wm = p.wall;
+  MaterialConsts m{};
+  m.pp = pair(pm.youngs_modulus, pm.poissons_ratio, pm.youngs_modulus, pm.poissons_ratio, pm.surface_energy,
+              pm.surface_energy);
+  m.pw = pair(pm.youngs_modulus, pm.poissons_ratio, wm.youngs_modulus, wm.poissons_ratio, pm.surface_energy,
+              wm.surface_energy);
+  m.mu = pm.friction_coefficient;
+  double lne = std::log(pm.restitution_coefficient);
+  double beta = lne / std::sqrt(lne * lne + kPi * kPi);
+  m.beta_abs = std::fabs(beta);
+  return m;
+}
+
+rp_status validate(const rp_params* p) {
+  if (!p) return fail(RP_ERR_INVALID, "params is NULL");
+  if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(RP_ERR_INVALID, "dt must be positive and finite");
+  for (int k = 0; k < 3; ++k) {
+    if (!std::isfinite(p->bounds_min[k]) || !std::isfinite(p->bounds_max[k]))
+      return fail(RP_ERR_INVALID, "bounds must be finite");
+    if (p->periodic[k] && !(p->bounds_max[k] > p->bounds_min[k]))
+      return fail(RP_ERR_INVALID, "periodic axis needs bounds_max > bounds_min");
+  }
+  if (p->normal_model < 0 || p->normal_model > 4) return fail(RP_ERR_INVALID, "invalid normal model");
+  if (p->tangential_model < 0 || p->tangential_model > 2) return fail(RP_ERR_INVALID, "invalid tangential model");
+  if (p->mode != RP_MODE_PARITY) return fail(RP_ERR_INVALID, "unsupported mode");
+  if (!(p->ref_cell_size > 0.0)) return fail(RP_ERR_INVALID, "ref_cell_size must be positive");
+  if (p->cell_size < 0.0 || p->skin < 0.0) return fail(RP_ERR_INVALID, "cell_size and skin must be >= 0");
+  return RP_OK;
+}
+
+// Host-side kernel parameters from rp_params + particle radii + meshes.
+rp_status compute_geometry(rp_engine* e) {
+  const rp_params& p = e->prm;
+  KParams& K = e->kp;
+  K.dt = p.dt;
+  K.any_periodic = 0;
+  for (int k = 0; k < 3; ++k) {
+    K.bmin[k] = p.bounds_min[k];
+    K.bmax[k] = p.bounds_max[k];
+    K.bsize[k] = p.bounds_max[k] - p.bounds_min[k];
+    K.periodic[k] = p.periodic[k] ? 1 : 0;
+    K.any_periodic |= K.periodic[k];
+    K.g[k] = p.gravity[k];
+    K.ref_origin[k] = p.bounds_min[k];
+  }
+  K.normal = p.normal_model;
+  K.tangential = p.tangential_model;
+  K.mc = material_consts(p);
+  K.margin_factor = (p.normal_model == RP_NORMAL_JKR || p.normal_model == RP_NORMAL_SJKR) ? 1.5 : 1.0;
+  K.ref_cs = p.ref_cell_size;
+  for (int k = 0; k < 3; ++k) K.ref_dims[k] = sat_i32(std::ceil(K.bsize[k] / K.ref_cs)) + 1;
+
+  // fine grid
+  double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
+  double rmean = e->r_mean > 0.0 ? e->r_mean : rmax;
+  K.r_max = rmax;
+  K.skin = p.skin > 0.0 ? p.skin : 0.1 * rmax;
+  K.half_skin = 0.5 * K.skin;
+  double cut_max = K.margin_factor * 2.0 * rmax + K.skin;
+  double cs = p.cell_size > 0.0 ? p.cell_size : (rmax > 2.0 * rmean ? 0.5 * cut_max : cut_max);
+  // non-periodic extent: domain bounds plus mesh bounding box, one cell of margin
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = p.bounds_min[k];
+    hi[k] = p.bounds_max[k];
+  }
+  for (const HostMesh& m : e->meshes)
+    for (size_t q = 0; q < m.v9.size(); ++q) {
+      int k = (int)(q % 3);
+      double v = (double)m.v9[q];
+      if (!std::isfinite(v)) continue;
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+    }
+  const double kMaxCells = 48.0e6;
+  for (int attempt = 0; attempt < 64; ++attempt) {
+    double total = 1.0;
+    for (int k = 0; k < 3; ++k) {
+      if (K.periodic[k]) {
+        int n = (int)std::floor(K.bsize[k] / cs);
+        if (n < 1) n = 1;
+        K.gn[k] = n;
+        K.go[k] = p.bounds_min[k];
+        K.ginv[k] = (double)n / K.bsize[k];
+      } else {
+        double l0 = lo[k] - cs, l1 = hi[k] + cs;
+        int n = (int)std::ceil((l1 - l0) / cs);
+        if (n < 1) n = 1;
+        K.gn[k] = n;
+        K.go[k] = l0;
+        K.ginv[k] = 1.0 / cs;
+      }
+      total *= (double)K.gn[k];
+    }
+    if (total <= kMaxCells) break;
+    cs *= 1.25;
+  }
+  // slowest-varying axis = the one with most cells
+  int order[3] = {0, 1, 2};
+  std::sort(order, order + 3, [&](int a, int b) { return K.gn[a] < K.gn[b]; });
+  int stride = 1;
+  for (int q = 0; q < 3; ++q) {
+    K.gstride[order[q]] = stride;
+    stride *= K.gn[order[q]];
+  }
+  K.ncells = (unsigned int)stride;
+  K.cap = e->cap;
+  K.nbr_cap = e->nbr_cap;
+  K.pair_cap = e->pair_cap;
+  return RP_OK;
+}
+
+rp_status upload_meshes_binned(rp_engine* e) {
+  KParams& K = e->kp;
+  std::vector<Tri> tris;
+  for (const HostMesh& m : e->meshes) {
+    size_t nt = m.n3.size() / 3;
+    for (size_t t = 0; t < nt; ++t) {
+      Tri tr;
+      const float* v = &m.v9[9 * t];
+      tr.v0 = mk(v[0], v[1], v[2]);
+      tr.v1 = mk(v[3], v[4], v[5]);
+      tr.v2 = mk(v[6], v[7], v[8]);
+      tr.n = mk(m.n3[3 * t], m.n3[3 * t + 1], m.n3[3 * t + 2]);
+      tris.push_back(tr);
+    }
+  }
+  // bin: every cell whose (clamped) box meets the triangle's AABB grown by r_max
+  std::vector<uint32_t> counts((size_t)K.ncells + 1, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> entries;  // (cell, tri)
+  const double grow = K.r_max * 1.0001 + 1e-12;
+  auto clampc = [&](double x, int k) {
+    double q = std::floor((x - K.go[k]) * K.ginv[k]);
+    if (!(q == q)) return 0;
+    if (q < 0) return 0;
+    if (q >= K.gn[k]) return K.gn[k] - 1;
+    return (int)q;
+  };
+  for (size_t t = 0; t < tris.size(); ++t) {
+    const Tri& tr = tris[t];
+    int c0[3], c1[3];
+    for (int k = 0; k < 3; ++k) {
+      double a = comp(tr.v0, k), b = comp(tr.v1, k), c = comp(tr.v2, k);
+      double mn = std::min(a, std::min(b, c)) - grow, mx = std::max(a, std::max(b, c)) + grow;
+      c0[k] = clampc(mn, k);
+      c1[k] = clampc(mx, k);
+    }
+    double ncov = (double)(c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1);
+    if (ncov > 2.0e8) return fail(RP_ERR_NOMEM, "triangle covers too many grid cells");
+    for (int z = c0[2]; z <= c1[2]; ++z)
+      for (int y = c0[1]; y <= c1[1]; ++y)
+        for (int x = c0[0]; x <= c1[0]; ++x) {
+          uint32_t c = (uint32_t)(x * K.gstride[0] + y * K.gstride[1] + z * K.gstride[2]);
+          entries.emplace_back(c, (uint32_t)t);
+        }
+  }
+  std::stable_sort(entries.begin(), entries.end(),
+                   [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                     return a.first < b.first;
+                   });
+  std::vector<uint32_t> start((size_t)K.ncells + 1, 0), idx(entries.size());
+  for (auto& en : entries) counts[en.first]++;
+  uint32_t run = 0;
+  for (size_t c = 0; c < K.ncells; ++c) {
+    start[c] = run;
+    run += counts[c];
+  }
+  start[K.ncells] = run;
+  for (size_t q = 0; q < entries.size(); ++q) idx[q] = entries[q].second;
+  free_ptr(e->d_tris);
+  free_ptr(e->d_tri_start);
+  free_ptr(e->d_tri_idx);
+  e->d_tris = nullptr;
+  e->d_tri_start = nullptr;
+  e->d_tri_idx = nullptr;
+  HIP_TRY(hipMalloc(&e->d_tris, std::max<size_t>(1, tris.size()) * sizeof(Tri)));
+  HIP_TRY(hipMalloc(&e->d_tri_start, start.size() * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&e->d_tri_idx, std::max<size_t>(1, idx.size()) * sizeof(uint32_t)));
+  if (!tris.empty())
+    HIP_TRY(hipMemcpyAsync(e->d_tris, tris.data(), tris.size() * sizeof(Tri), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_tri_start, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         e->stream));
+  if (!idx.empty())
+    HIP_TRY(hipMemcpyAsync(e->d_tri_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return RP_OK;
+}
+
+// Buffer carving: Dyn = x v w rx rv rw a tau (24 planes of doubles);
+// Stat = r m (2 planes of doubles), flags (bytes), sid (u32).
+size_t dyn_bytes(int cap) { return round_up((size_t)cap * 24 * sizeof(double), 256); }
+size_t stat_bytes(int cap) {
+  size_t c = (size_t)cap;
+  return round_up(c * 2 * sizeof(double), 256) + round_up(c, 256) + round_up(c * sizeof(uint32_t), 256);
+}
+
+Dyn carve_dyn(void* mem, int cap) {
+  Dyn s{};
+  size_t c = (size_t)cap;
+  double* d = (double*)mem;
+  s.x = d;
+  s.v = d + 3 * c;
+  s.w = d + 6 * c;
+  s.rx = d + 9 * c;
+  s.rv = d + 12 * c;
+  s.rw = d + 15 * c;
+  s.a = d + 18 * c;
+  s.tau = d + 21 * c;
+  return s;
+}
+
+Stat carve_stat(void* mem, int cap) {
+  Stat s{};
+  char* p = (char*)mem;
+  size_t c = (size_t)cap;
+  s.r = (double*)p;
+  s.m = (double*)p + c;
+  p += round_up(c * 2 * sizeof(double), 256);
+  s.flags = (uint8_t*)p;
+  p += round_up(c, 256);
+  s.sid = (uint32_t*)p;
+  return s;
+}
+
+void free_particles(rp_engine* e) {
+  for (int b = 0; b < 2; ++b) {
+    free_ptr(e->d_dyn_mem[b]);
+    free_ptr(e->d_stat_mem[b]);
+    e->d_dyn_mem[b] = nullptr;
+    e->d_stat_mem[b] = nullptr;
+  }
+  free_ptr(e->d_xb);
+  free_ptr(e->d_key);
+  free_ptr(e->d_tmp);
+  free_ptr(e->d_nbr_count);
+  free_ptr(e->d_nbr);
+  free_ptr(e->d_pairbuf);
+  free_ptr(e->d_paircount);
+  e->d_xb = nullptr;
+  e->d_key = e->d_tmp = nullptr;
+  e->d_nbr_count = e->d_nbr = nullptr;
+  e->d_pairbuf = nullptr;
+  e->d_paircount = nullptr;
+  e->cap = 0;
+}
+
+rp_status alloc_cells(rp_engine* e) {
+  size_t need = (size_t)e->kp.ncells + 2;
+  if (need <= e->cells_alloc) return RP_OK;
+  free_ptr(e->d_cell_count);
+  free_ptr(e->d_cell_start);
+  free_ptr(e->d_block_sums);
+  e->d_cell_count = e->d_cell_start = e->d_block_sums = nullptr;
+  HIP_TRY(hipMalloc(&e->d_cell_count, need * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&e->d_cell_start, need * sizeof(uint32_t)));
+  size_t nb = (need + kScanChunk - 1) / kScanChunk + 1;
+  HIP_TRY(hipMalloc(&e->d_block_sums, nb * sizeof(uint32_t)));
+  HIP_TRY(hipMemsetAsync(e->d_cell_count, 0, need * sizeof(uint32_t), e->stream));
+  e->cells_alloc = need;
+  return RP_OK;
+}
+
+rp_status alloc_nbr(rp_engine* e, int nbr_cap) {
+  free_ptr(e->d_nbr);
+  e->d_nbr = nullptr;
+  HIP_TRY(hipMalloc(&e->d_nbr, (size_t)nbr_cap * (size_t)e->cap * sizeof(int)));
+  e->nbr_cap = nbr_cap;
+  e->kp.nbr_cap = nbr_cap;
+  return RP_OK;
+}
+
+int grid_for(size_t n) {
+  size_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > (size_t)kMaxGrid) g = kMaxGrid;
+  return (int)g;
+}
+
+rp_status launch_rebuild(rp_engine* e, int par) {
+  const KParams& K = e->kp;
+  int g = grid_for((size_t)e->cap);
+  int m = (int)K.ncells + 1;  // buckets incl. removed sentinel
+  int nb = (m + kScanChunk - 1) / kScanChunk;
+  int gs = std::min(nb, kMaxGrid);
+  hipLaunchKernelGGL(k_keys, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_key, e->d_cell_count);
+  hipLaunchKernelGGL(k_scan_sums, dim3(gs), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_cell_count, m,
+                     e->d_block_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, e->stream, e->d_ctl, par, e->d_block_sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(gs), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_cell_count, m,
+                     e->d_block_sums, e->d_cell_start);
+  hipLaunchKernelGGL(k_scatter, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_key, e->d_cell_start,
+                     e->d_cell_count, e->d_tmp);
+  hipLaunchKernelGGL(k_rank_permute, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_key,
+                     e->d_cell_start, e->d_tmp);
+  hipLaunchKernelGGL(k_build, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_cell_start,
+                     e->d_nbr_count, e->d_nbr, e->d_xb);
+  HIP_TRY(hipGetLastError());
+  return RP_OK;
+}
+
+rp_status read_ctl(rp_engine* e) {
+  HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return RP_OK;
+}
+
+// Recompute geometry, rebin meshes and make sure the neighbour list is large
+// enough; ends with a synchronous rebuild so capacity is known.
+rp_status prepare(rp_engine* e) {
+  if (!e->have_particles) return fail(RP_ERR_STATE, "no particles uploaded");
+  if (!e->geometry_dirty) return RP_OK;
+  rp_status st = compute_geometry(e);
+  if (st) return st;
+  if ((st = alloc_cells(e))) return st;
+  if ((st = upload_meshes_binned(e))) return st;
+  int par = (int)(e->step_count & 1);
+  // trial rebuild to size the neighbour lists; repeat while it overflows
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, par);
+    if ((st = launch_rebuild(e, par))) return st;
+    if ((st = read_ctl(e))) return st;
+    if (!e->host_ctl.overflow) break;
+    int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 8), 8);
+    int zero = 0;
+    HIP_TRY(hipMemcpyAsync(&e->d_ctl->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if ((st = alloc_nbr(e, want))) return st;
+  }
+  if (e->host_ctl.overflow) return fail(RP_ERR_NOMEM, "neighbour list capacity could not be sized");
+  // headroom for later rebuilds
+  int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 16), 8);
+  if (want > e->nbr_cap) {
+    if ((st = alloc_nbr(e, want))) return st;
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, par);
+    if ((st = launch_rebuild(e, par))) return st;
+  }
+  // the list is valid now; a later k_begin decides about the next rebuild
+  int clear[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(&e->d_ctl->rebuild[0], clear, sizeof(clear), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->geometry_dirty = false;
+  return RP_OK;
+}
+
+rp_status ensure_pairs(rp_engine* e) {
+  if (e->d_pairbuf && e->pair_cap >= e->nbr_cap + 16) return RP_OK;
+  free_ptr(e->d_pairbuf);
+  free_ptr(e->d_paircount);
+  e->d_pairbuf = nullptr;
+  e->d_paircount = nullptr;
+  e->pair_cap = e->nbr_cap + 16;
+  e->kp.pair_cap = e->pair_cap;
+  HIP_TRY(hipMalloc(&e->d_pairbuf, (size_t)e->pair_cap * e->cap * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&e->d_paircount, (size_t)e->cap * sizeof(int)));
+  HIP_TRY(hipMemsetAsync(e->d_paircount, 0, (size_t)e->cap * sizeof(int), e->stream));
+  return RP_OK;
+}
+
+template <bool ADV, bool REC>
+void launch_force(rp_engine* e, int par) {
+  int g = (int)((e->cap + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_force<ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp,
+                     e->d_nbr_count, e->d_nbr, e->d_tris, e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_pairbuf,
+                     e->d_paircount);
+}
+
+}  // namespace
+
+// ---- C ABI ----------------------------------------------------------------------
+
+extern "C" {
+
+void rp_params_default(rp_params* p, double dt, const double bmin[3], const double bmax[3]) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->dt = dt;
+  for (int k = 0; k < 3; ++k) {
+    p->bounds_min[k] = bmin ? bmin[k] : 0.0;
+    p->bounds_max[k] = bmax ? bmax[k] : 0.0;
+  }
+  p->normal_model = RP_NORMAL_HERTZ;
+  p->tangential_model = RP_TANGENTIAL_MINDLIN;
+  p->mode = RP_MODE_PARITY;
+  p->particle = rp_material{1e7, 0.3, 2500.0, 0.5, 0.5, 0.0};
+  p->wall = rp_material{1e9, 0.3, 7800.0, 0.5, 0.5, 0.0};
+  p->gravity[0] = 0.0;
+  p->gravity[1] = -9.81;
+  p->gravity[2] = 0.0;
+  p->ref_cell_size = 1.0;
+}
+
+int32_t rp_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* rp_last_error(void) { return g_last_error.c_str(); }
+
+rp_status rp_engine_create(const rp_params* params, rp_engine** out) {
+  if (!out) return fail(RP_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  rp_status st = validate(params);
+  if (st) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RP_ERR_DEVICE, "no HIP device available");
+  rp_engine* e = new rp_engine();
+  e->prm = *params;
+  HIP_TRY(hipGetDevice(&e->device));
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete e;
+    return fail(RP_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  he = hipMalloc(&e->d_ctl, sizeof(Ctl));
+  if (he != hipSuccess) {
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return fail(RP_ERR_NOMEM, "hipMalloc(ctl) failed");
+  }
+  Ctl c{};
+  (void)hipMemcpy(e->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice);
+  *out = e;
+  g_last_error.clear();
+  return RP_OK;
+}
+
+rp_status rp_engine_set_params(rp_engine* e, const rp_params* params) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  rp_status st = validate(params);
+  if (st) return st;
+  e->prm = *params;
+  e->geometry_dirty = true;
+  return RP_OK;
+}
+
+rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const double* v3, const double* w3,
+                                const double* a3, const double* tau3, const double* rx3, const double* rv3,
+                                const double* rw3, const double* radius, const double* mass,
+                                const uint8_t* fixed) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (n > 0 && (!x3 || !radius || !mass)) return fail(RP_ERR_INVALID, "x3, radius and mass are required");
+  if (n > (size_t)1 << 30) return fail(RP_ERR_INVALID, "too many particles");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  free_particles(e);
+  int cap = (int)std::max<size_t>(round_up(n, 256), 256);
+  e->cap = cap;
+  for (int b = 0; b < 2; ++b) {
+    HIP_TRY(hipMalloc(&e->d_dyn_mem[b], dyn_bytes(cap)));
+    HIP_TRY(hipMalloc(&e->d_stat_mem[b], stat_bytes(cap)));
+  }
+  HIP_TRY(hipMalloc(&e->d_xb, (size_t)cap * 3 * sizeof(float)));
+  HIP_TRY(hipMalloc(&e->d_key, (size_t)cap * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&e->d_tmp, (size_t)cap * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * sizeof(int)));
+  rp_status st = alloc_nbr(e, 32);
+  if (st) return st;
+  // host staging in plane layout
+  std::vector<double> planes((size_t)cap * 26, 0.0);
+  auto put3 = [&](int field, const double* src) {
+    if (!src) return;
+    for (size_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) planes[(size_t)field * 3 * cap + (size_t)k * cap + i] = src[3 * i + k];
+  };
+  put3(0, x3);
+  put3(1, v3);
+  put3(2, w3);
+  put3(3, rx3);
+  put3(4, rv3);
+  put3(5, rw3);
+  put3(6, a3);
+  put3(7, tau3);
+  double rsum = 0.0, rmax = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    planes[(size_t)24 * cap + i] = radius[i];
+    planes[(size_t)25 * cap + i] = mass[i];
+    rsum += radius[i];
+    if (radius[i] > rmax) rmax = radius[i];
+  }
+  std::vector<uint8_t> fl((size_t)cap, 0);
+  std::vector<uint32_t> sid((size_t)cap, 0);
+  for (size_t i = 0; i < n; ++i) {
+    fl[i] = (fixed && fixed[i]) ? FLAG_FIXED : 0;
+    sid[i] = (uint32_t)i;
+  }
+  Dyn d0 = carve_dyn(e->d_dyn_mem[0], cap);
+  Stat s0 = carve_stat(e->d_stat_mem[0], cap);
+  HIP_TRY(hipMemcpy(d0.x, planes.data(), (size_t)cap * 24 * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s0.r, planes.data() + (size_t)24 * cap, (size_t)cap * 2 * sizeof(double),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s0.flags, fl.data(), (size_t)cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s0.sid, sid.data(), (size_t)cap * sizeof(uint32_t), hipMemcpyHostToDevice));
+  Ctl c{};
+  c.dyn[0] = d0;
+  c.dyn[1] = carve_dyn(e->d_dyn_mem[1], cap);
+  c.stat[0] = s0;
+  c.stat[1] = carve_stat(e->d_stat_mem[1], cap);
+  c.cur_dyn = 0;
+  c.cur_stat = 0;
+  c.n = (int)n;
+  c.alive = (int)n;
+  HIP_TRY(hipMemcpy(e->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+  e->n_uploaded = (int)n;
+  e->r_max = rmax;
+  e->r_mean = n ? rsum / (double)n : 0.0;
+  e->have_particles = true;
+  e->geometry_dirty = true;
+  e->pairs_valid = false;
+  return RP_OK;
+}
+
+rp_status rp_engine_upload(rp_engine* e, size_t n, const double* x3, const double* v3, const double* w3,
+                           const double* a3, const double* tau3, const double* radius, const double* mass,
+                           const uint8_t* fixed) {
+  return rp_engine_upload_full(e, n, x3, v3, w3, a3, tau3, nullptr, nullptr, nullptr, radius, mass, fixed);
+}
+
+rp_status rp_engine_add_mesh(rp_engine* e, size_t ntri, const float* v9, const float* n3) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (ntri > 0 && (!v9 || !n3)) return fail(RP_ERR_INVALID, "v9 and n3 are required");
+  HostMesh m;
+  m.v9.assign(v9, v9 + ntri * 9);
+  m.n3.assign(n3, n3 + ntri * 3);
+  e->meshes.push_back(std::move(m));
+  e->geometry_dirty = true;
+  return RP_OK;
+}
+
+rp_status rp_engine_add_mesh_stl(rp_engine* e, const char* path) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  std::vector<float> v9, n3;
+  std::string err;
+  if (!rp_stl_load(path ? path : "", v9, n3, err)) return fail(RP_ERR_IO, err);
+  return rp_engine_add_mesh(e, n3.size() / 3, v9.data(), n3.data());
+}
+
+rp_status rp_engine_set_record(rp_engine* e, int on) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  e->record = on != 0;
+  return RP_OK;
+}
+
+rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (nsteps == 0) return RP_OK;
+  rp_status st = prepare(e);
+  if (st) return st;
+  if (e->record && (st = ensure_pairs(e))) return st;
+  int gcap = (int)((e->cap + kBlock - 1) / kBlock);
+  int par = (int)(e->step_count & 1);
+  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp, e->d_xb);
+  for (uint64_t s = 0; s < nsteps; ++s) {
+    par = (int)(e->step_count & 1);
+    if ((st = launch_rebuild(e, par))) return st;
+    bool last = (s + 1 == nsteps);
+    if (last) hipLaunchKernelGGL(k_reset_step_counters, dim3(1), dim3(1), 0, e->stream, e->d_ctl);
+    if (last) {
+      if (e->record) launch_force<false, true>(e, par);
+      else launch_force<false, false>(e, par);
+    } else {
+      launch_force<true, false>(e, par);
+    }
+    e->step_count++;
+  }
+  HIP_TRY(hipGetLastError());
+  e->pairs_valid = e->record;
+  return RP_OK;
+}
+
+rp_status rp_engine_synchronize(rp_engine* e) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return RP_OK;
+}
+
+rp_status rp_engine_count(rp_engine* e, size_t* n) {
+  if (!e || !n) return fail(RP_ERR_INVALID, "NULL argument");
+  if (!e->have_particles) {
+    *n = 0;
+    return RP_OK;
+  }
+  rp_status st = read_ctl(e);
+  if (st) return st;
+  *n = (size_t)e->host_ctl.alive;
+  return RP_OK;
+}
+
+namespace {
+// Slot order -> API order (ascending stable id among alive particles).
+rp_status slot_order(rp_engine* e, std::vector<int>& order, Dyn& cur, Stat& cst, int& nslots) {
+  rp_status st = read_ctl(e);
+  if (st) return st;
+  cur = e->host_ctl.dyn[e->host_ctl.cur_dyn];
+  cst = e->host_ctl.stat[e->host_ctl.cur_stat];
+  nslots = e->host_ctl.n;
+  std::vector<uint32_t> sid((size_t)nslots);
+  std::vector<uint8_t> fl((size_t)nslots);
+  if (nslots > 0) {
+    HIP_TRY(hipMemcpy(sid.data(), cst.sid, (size_t)nslots * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(fl.data(), cst.flags, (size_t)nslots, hipMemcpyDeviceToHost));
+  }
+  order.clear();
+  for (int i = 0; i < nslots; ++i)
+    if (!(fl[i] & FLAG_DEAD)) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return sid[a] < sid[b]; });
+  return RP_OK;
+}
+
+rp_status fetch3(const double* dev, int cap, int nslots, const std::vector<int>& order, double* out) {
+  if (!out) return RP_OK;
+  std::vector<double> tmp((size_t)cap * 3);
+  if (nslots > 0) {
+    for (int k = 0; k < 3; ++k)
+      HIP_TRY(hipMemcpy(tmp.data() + (size_t)k * cap, dev + (size_t)k * cap, (size_t)nslots * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  }
+  for (size_t q = 0; q < order.size(); ++q)
+    for (int k = 0; k < 3; ++k) out[3 * q + k] = tmp[(size_t)k * cap + order[q]];
+  return RP_OK;
+}
+}  // namespace
+
+rp_status rp_engine_download(rp_engine* e, double* x3, double* v3, double* w3, double* a3, double* tau3) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (!e->have_particles) return RP_OK;
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  rp_status st = slot_order(e, order, cur, cst, ns);
+  if (st) return st;
+  if ((st = fetch3(cur.x, e->cap, ns, order, x3))) return st;
+  if ((st = fetch3(cur.v, e->cap, ns, order, v3))) return st;
+  if ((st = fetch3(cur.w, e->cap, ns, order, w3))) return st;
+  if ((st = fetch3(cur.a, e->cap, ns, order, a3))) return st;
+  if ((st = fetch3(cur.tau, e->cap, ns, order, tau3))) return st;
+  return RP_OK;
+}
+
+rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3, double* rw3, double* radius,
+                                 double* mass, uint8_t* fixed) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (!e->have_particles) return RP_OK;
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  rp_status st = slot_order(e, order, cur, cst, ns);
+  if (st) return st;
+  if ((st = fetch3(cur.rx, e->cap, ns, order, rx3))) return st;
+  if ((st = fetch3(cur.rv, e->cap, ns, order, rv3))) return st;
+  if ((st = fetch3(cur.rw, e->cap, ns, order, rw3))) return st;
+  std::vector<double> tmp((size_t)std::max(ns, 1));
+  std::vector<uint8_t> fl((size_t)std::max(ns, 1));
+  if (radius && ns > 0) {
+    HIP_TRY(hipMemcpy(tmp.data(), cst.r, (size_t)ns * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < order.size(); ++q) radius[q] = tmp[order[q]];
+  }
+  if (mass && ns > 0) {
+    HIP_TRY(hipMemcpy(tmp.data(), cst.m, (size_t)ns * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < order.size(); ++q) mass[q] = tmp[order[q]];
+  }
+  if (fixed && ns > 0) {
+    HIP_TRY(hipMemcpy(fl.data(), cst.flags, (size_t)ns, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < order.size(); ++q) fixed[q] = (fl[order[q]] & FLAG_FIXED) ? 1 : 0;
+  }
+  return RP_OK;
+}
+
+rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js, size_t cap, size_t* total) {
+  if (!e || !total) return fail(RP_ERR_INVALID, "NULL argument");
+  if (!e->pairs_valid) return fail(RP_ERR_STATE, "no recorded step: call rp_engine_set_record(e,1) and step");
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  rp_status st = slot_order(e, order, cur, cst, ns);
+  if (st) return st;
+  if (e->host_ctl.pair_overflow) return fail(RP_ERR_NOMEM, "pair record overflow");
+  std::vector<uint32_t> sid((size_t)std::max(ns, 1));
+  std::vector<int> pc((size_t)std::max(ns, 1));
+  std::vector<unsigned long long> pb((size_t)std::max(ns, 1) * e->pair_cap);
+  if (ns > 0) {
+    HIP_TRY(hipMemcpy(sid.data(), cst.sid, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pc.data(), e->d_paircount, (size_t)ns * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pb.data(), e->d_pairbuf, (size_t)ns * e->pair_cap * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  }
+  // stable id -> API index
+  std::vector<std::pair<uint32_t, int64_t>> api;
+  api.reserve(order.size());
+  for (size_t q = 0; q < order.size(); ++q) api.emplace_back(sid[order[q]], (int64_t)q);
+  auto to_api = [&](uint32_t s) -> int64_t {
+    auto it = std::lower_bound(api.begin(), api.end(), std::make_pair(s, (int64_t)-1));
+    return (it != api.end() && it->first == s) ? it->second : -1;
+  };
+  size_t tot = 0;
+  for (size_t q = 0; q < order.size(); ++q) tot += (size_t)pc[order[q]];
+  *total = tot;
+  if (offsets == nullptr || js == nullptr) return RP_OK;
+  if (cap < tot) return fail(RP_ERR_INVALID, "pair buffer too small");
+  size_t o = 0;
+  for (size_t q = 0; q < order.size(); ++q) {
+    int slot = order[q];
+    offsets[q] = (int64_t)o;
+    std::vector<unsigned long long> ent(pb.begin() + (size_t)slot * e->pair_cap,
+                                        pb.begin() + (size_t)slot * e->pair_cap + pc[slot]);
+    std::sort(ent.begin(), ent.end());  // (rank of the 27-cell walk, stable id) = reference order
+    for (unsigned long long v : ent) js[o++] = to_api((uint32_t)(v & 0xffffffffu));
+  }
+  offsets[order.size()] = (int64_t)o;
+  return RP_OK;
+}
+
+rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
+  if (!e || !out) return fail(RP_ERR_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof(*out));
+  if (!e->have_particles) return RP_OK;
+  rp_status st = read_ctl(e);
+  if (st) return st;
+  const Ctl& c = e->host_ctl;
+  out->steps = e->step_count;
+  out->rebuilds = c.rebuilds;
+  out->contacts = c.contacts;
+  out->wall_contacts = c.wall_contacts;
+  out->removed = (uint64_t)(e->n_uploaded - c.alive);
+  out->excessive_overlap = c.overlap;
+  out->neighbour_capacity = e->nbr_cap;
+  out->jkr_failures = c.jkr_failures;
+  out->max_neighbours = c.max_count;
+  return RP_OK;
+}
+
+void rp_engine_destroy(rp_engine* e) {
+  if (!e) return;
+  (void)hipStreamSynchronize(e->stream);
+  free_particles(e);
+  free_ptr(e->d_cell_count);
+  free_ptr(e->d_cell_start);
+  free_ptr(e->d_block_sums);
+  free_ptr(e->d_tris);
+  free_ptr(e->d_tri_start);
+  free_ptr(e->d_tri_idx);
+  free_ptr(e->d_ctl);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""Loader and thin object wrapper for librp_hip.so (the C ABI in include/rp_abi.h).
+
+The library is the product: there is no fallback.  If it is missing or a
+call fails, a Python exception says so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librp_hip.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
+_eng = C.c_void_p
+
+
+def lib() -> C.CDLL:
+    """Load librp_hip.so (once).  Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(
+            f"{_LIB_PATH} is missing: build it with `python -m rusty_particles_amd.build` "
+            "(the HIP engine has no CPU fallback)")
+    L = C.CDLL(_LIB_PATH)
+    L.rp_params_default.argtypes = [C.POINTER(abi.RpParams), C.c_double, _dp, _dp]
+    L.rp_params_default.restype = None
+    L.rp_device_count.argtypes = []
+    L.rp_device_count.restype = C.c_int32
+    L.rp_engine_create.argtypes = [C.POINTER(abi.RpParams), C.POINTER(_eng)]
+    L.rp_engine_set_params.argtypes = [_eng, C.POINTER(abi.RpParams)]
+    L.rp_engine_upload.argtypes = [_eng, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _u8p]
+    L.rp_engine_upload_full.argtypes = [_eng, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _u8p]
+    L.rp_engine_add_mesh.argtypes = [_eng, C.c_size_t, _fp, _fp]
+    L.rp_engine_add_mesh_stl.argtypes = [_eng, C.c_char_p]
+    L.rp_engine_step.argtypes = [_eng, C.c_uint64]
+    L.rp_engine_synchronize.argtypes = [_eng]
+    L.rp_engine_count.argtypes = [_eng, C.POINTER(C.c_size_t)]
+    L.rp_engine_download.argtypes = [_eng, _dp, _dp, _dp, _dp, _dp]
+    L.rp_engine_download_aux.argtypes = [_eng, _dp, _dp, _dp, _dp, _dp, _u8p]
+    L.rp_engine_set_record.argtypes = [_eng, C.c_int]
+    L.rp_stl_read.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), _fp, _fp]
+    L.rp_engine_pairs.argtypes = [_eng, _i64p, _i64p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
+    L.rp_engine_destroy.argtypes = [_eng]
+    L.rp_engine_destroy.restype = None
+    L.rp_last_error.argtypes = []
+    L.rp_last_error.restype = C.c_char_p
+    for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
+                 "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
+                 "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
+                 "rp_stl_read", "rp_engine_pairs", "rp_engine_stats"):
+        getattr(L, name).restype = C.c_int32
+    _lib = L
+    return L
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rp error {code}: {msg}")
+        self.code = code
+
+
+def _check(code: int) -> None:
+    if code != abi.RP_OK:
+        msg = lib().rp_last_error()
+        raise EngineError(code, msg.decode() if msg else "")
+
+
+def _ptr(a, ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _vec(a, n: int) -> np.ndarray | None:
+    if a is None:
+        return None
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(n, 3))
+
+
+def device_count() -> int:
+    return int(lib().rp_device_count())
+
+
+def read_stl(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Triangles of an STL file: (ntri, 9) vertices and (ntri, 3) normals, float32.
+
+    Raises OSError like the reference's add_mesh (src/lib.rs:87-91)."""
+    L = lib()
+    n = C.c_size_t(0)
+    code = L.rp_stl_read(path.encode(), C.byref(n), None, None)
+    if code == abi.RP_ERR_IO:
+        raise OSError(f"cannot read STL file: {path}")
+    _check(code)
+    v9 = np.zeros((n.value, 9), dtype=np.float32)
+    n3 = np.zeros((n.value, 3), dtype=np.float32)
+    if n.value:
+        _check(L.rp_stl_read(path.encode(), C.byref(n), _ptr(v9, C.c_float), _ptr(n3, C.c_float)))
+    return v9, n3
+
+
+class Engine:
+    """One rp_engine (one domain on the current HIP device)."""
+
+    def __init__(self, params: abi.RpParams):
+        self._L = lib()
+        h = _eng()
+        _check(self._L.rp_engine_create(C.byref(params), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.rp_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: abi.RpParams) -> None:
+        _check(self._L.rp_engine_set_params(self._h, C.byref(params)))
+
+    def upload(self, x, radius, mass, fixed=None, v=None, w=None, a=None, tau=None, rx=None, rv=None,
+               rw=None) -> None:
+        x = np.asarray(x, dtype=np.float64)
+        n = int(x.reshape(-1, 3).shape[0])
+        xs = _vec(x, n)
+        r = np.ascontiguousarray(np.asarray(radius, dtype=np.float64).reshape(n))
+        m = np.ascontiguousarray(np.asarray(mass, dtype=np.float64).reshape(n))
+        f = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, dtype=np.uint8).reshape(n))
+        arrs = [_vec(q, n) for q in (v, w, a, tau, rx, rv, rw)]
+        _check(self._L.rp_engine_upload_full(self._h, n, _ptr(xs), *[_ptr(q) for q in arrs], _ptr(r), _ptr(m),
+                                             _ptr(f, C.c_uint8)))
+
+    def add_mesh(self, v9, n3) -> None:
+        v9 = np.ascontiguousarray(np.asarray(v9, dtype=np.float32).reshape(-1, 9))
+        n3 = np.ascontiguousarray(np.asarray(n3, dtype=np.float32).reshape(-1, 3))
+        _check(self._L.rp_engine_add_mesh(self._h, v9.shape[0], _ptr(v9, C.c_float), _ptr(n3, C.c_float)))
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(self._L.rp_engine_step(self._h, int(nsteps)))
+
+    def synchronize(self) -> None:
+        _check(self._L.rp_engine_synchronize(self._h))
+
+    def count(self) -> int:
+        n = C.c_size_t(0)
+        _check(self._L.rp_engine_count(self._h, C.byref(n)))
+        return int(n.value)
+
+    def download(self) -> dict:
+        n = self.count()
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "w", "a", "tau", "rx", "rv", "rw")}
+        out["radius"] = np.zeros(n)
+        out["mass"] = np.zeros(n)
+        out["fixed"] = np.zeros(n, dtype=np.uint8)
+        _check(self._L.rp_engine_download(self._h, *[_ptr(out[k]) for k in ("x", "v", "w", "a", "tau")]))
+        _check(self._L.rp_engine_download_aux(self._h, _ptr(out["rx"]), _ptr(out["rv"]), _ptr(out["rw"]),
+                                              _ptr(out["radius"]), _ptr(out["mass"]),
+                                              _ptr(out["fixed"], C.c_uint8)))
+        return out
+
+    def set_record(self, on: bool) -> None:
+        _check(self._L.rp_engine_set_record(self._h, 1 if on else 0))
+
+    def pairs(self) -> tuple[np.ndarray, np.ndarray]:
+        """(offsets[n+1], js[total]) of the last recorded step, in the reference's neighbour order."""
+        tot = C.c_size_t(0)
+        _check(self._L.rp_engine_pairs(self._h, None, None, 0, C.byref(tot)))
+        n = self.count()
+        offs = np.zeros(n + 1, dtype=np.int64)
+        js = np.zeros(max(tot.value, 1), dtype=np.int64)
+        _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
+                                       C.byref(tot)))
+        return offs, js[: tot.value]
+
+    def stats(self) -> dict:
+        s = abi.RpStats()
+        _check(self._L.rp_engine_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in abi.RpStats._fields_ if name != "reserved"}

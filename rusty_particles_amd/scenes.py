@@ -1,0 +1,190 @@
+"""Seeded synthetic workloads and procedural STL meshes.
+
+Coordinates are in the reference's frame: gravity along -Y
+(src/simulation.rs:267).  A workload described z-up maps (x, y, z) -> (x, z, -y).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import struct
+
+import numpy as np
+
+from . import abi
+
+_SCENE_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librp_scene.so")
+_scene = None
+
+
+def _lib():
+    global _scene
+    if _scene is None:
+        if not os.path.exists(_SCENE_LIB):
+            raise ImportError(f"{_SCENE_LIB} missing: run `python -m rusty_particles_amd.build scene`")
+        L = C.CDLL(_SCENE_LIB)
+        dp = C.POINTER(C.c_double)
+        L.rps_rsa.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, C.c_ulonglong, C.c_longlong, dp, dp]
+        L.rps_rsa.restype = C.c_longlong
+        _scene = L
+    return _scene
+
+
+def rsa(n: int, lo, hi, rmin: float, rmax: float, seed: int, max_attempts: int = 100000):
+    """Random sequential addition of n non-overlapping spheres, radii U[rmin, rmax]."""
+    x = np.zeros((n, 3))
+    r = np.zeros(n)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    placed = _lib().rps_rsa(n, lo.ctypes.data_as(dp), hi.ctypes.data_as(dp), rmin, rmax, seed, max_attempts,
+                            x.ctypes.data_as(dp), r.ctypes.data_as(dp))
+    if placed < n:
+        raise RuntimeError(f"RSA placed only {placed} of {n} spheres")
+    return x, r
+
+
+def jittered_lattice(counts, spacing: float, origin, jitter: float, seed: int, n: int | None = None):
+    """Cubic lattice (x fastest), uniform jitter in [-jitter, jitter] per axis."""
+    nx, ny, nz = counts
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    g = g.transpose(2, 1, 0, 3).reshape(-1, 3).astype(np.float64)
+    pts = np.asarray(origin, dtype=np.float64) + g * spacing
+    rng = np.random.default_rng(seed)
+    pts = pts + rng.uniform(-jitter, jitter, size=pts.shape)
+    if n is not None:
+        pts = pts[:n]
+    return pts
+
+
+# ---- meshes -------------------------------------------------------------------
+
+def _tri_normals(v9: np.ndarray) -> np.ndarray:
+    a = v9[:, 0:3].astype(np.float64)
+    b = v9[:, 3:6].astype(np.float64)
+    c = v9[:, 6:9].astype(np.float64)
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return n.astype(np.float32)
+
+
+def box_mesh(lo, hi):
+    """12 triangles closing the box [lo, hi], normals pointing inward."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    c = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], dtype=np.float64)
+    centre = c.mean(0)
+    quads = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
+    tris = []
+    for a, b, cc, d in quads:
+        for t in ((a, b, cc), (a, cc, d)):
+            p = c[list(t)]
+            nrm = np.cross(p[1] - p[0], p[2] - p[0])
+            if np.dot(nrm, centre - p[0]) < 0:  # make the normal point inward
+                p = p[[0, 2, 1]]
+            tris.append(p.reshape(9))
+    v9 = np.array(tris, dtype=np.float32)
+    return v9, _tri_normals(v9)
+
+
+def hopper_mesh(r_out: float, half_angle_deg: float, r_top: float, y0: float = 0.0, nseg: int = 64,
+                nring: int = 32):
+    """Conical hopper around +Y: radius r_out at y0 growing to r_top; 2*nseg*nring triangles."""
+    h = (r_top - r_out) / math.tan(math.radians(half_angle_deg))
+    ys = y0 + h * np.arange(nring + 1) / nring
+    rs = r_out + (ys - y0) * math.tan(math.radians(half_angle_deg))
+    th = 2 * np.pi * np.arange(nseg + 1) / nseg
+    tris = []
+    for i in range(nring):
+        for j in range(nseg):
+            p00 = (rs[i] * np.cos(th[j]), ys[i], rs[i] * np.sin(th[j]))
+            p01 = (rs[i] * np.cos(th[j + 1]), ys[i], rs[i] * np.sin(th[j + 1]))
+            p10 = (rs[i + 1] * np.cos(th[j]), ys[i + 1], rs[i + 1] * np.sin(th[j]))
+            p11 = (rs[i + 1] * np.cos(th[j + 1]), ys[i + 1], rs[i + 1] * np.sin(th[j + 1]))
+            tris.append(p00 + p10 + p11)
+            tris.append(p00 + p11 + p01)
+    v9 = np.array(tris, dtype=np.float32)
+    n3 = _tri_normals(v9)
+    # inward (towards the axis)
+    cen = (v9[:, 0:3] + v9[:, 3:6] + v9[:, 6:9]) / 3.0
+    outward = np.stack([cen[:, 0], np.zeros(len(cen)), cen[:, 2]], 1)
+    flip = (n3 * outward).sum(1) > 0
+    n3[flip] *= -1
+    return v9, n3
+
+
+def write_stl_binary(path, v9, n3, header=b"rusty_particles_amd"):
+    v9 = np.asarray(v9, dtype="<f4").reshape(-1, 9)
+    n3 = np.asarray(n3, dtype="<f4").reshape(-1, 3)
+    with open(path, "wb") as f:
+        f.write(header[:80].ljust(80, b"\0"))
+        f.write(struct.pack("<I", v9.shape[0]))
+        for t in range(v9.shape[0]):
+            f.write(n3[t].tobytes())
+            f.write(v9[t].tobytes())
+            f.write(b"\0\0")
+
+
+def write_stl_ascii(path, v9, n3, name="mesh"):
+    v9 = np.asarray(v9, dtype=np.float32).reshape(-1, 9)
+    n3 = np.asarray(n3, dtype=np.float32).reshape(-1, 3)
+    with open(path, "w") as f:
+        f.write(f"solid {name}\n")
+        for t in range(v9.shape[0]):
+            f.write(f"  facet normal {n3[t,0]!r} {n3[t,1]!r} {n3[t,2]!r}\n    outer loop\n")
+            for k in range(3):
+                f.write(f"      vertex {v9[t,3*k]!r} {v9[t,3*k+1]!r} {v9[t,3*k+2]!r}\n")
+            f.write("    endloop\n  endfacet\n")
+        f.write(f"endsolid {name}\n")
+
+
+# ---- workloads -------------------------------------------------------------------
+
+def sphere_mass(r, rho):
+    return rho * 4.0 / 3.0 * np.pi * np.asarray(r) ** 3
+
+
+def hopper_workload(n: int = 1_000_000, seed: int = 0xD3E50002, phi: float = 0.25):
+    """1M polydisperse spheres (r ~ U[0.5, 1.5] mm, rho 2500) above a conical hopper
+    (half-angle 30 deg, outlet radius 8 mm, 4096 triangles), Hertz normal +
+    Mindlin tangential, E=5e7, nu=0.25, e=0.6, mu=0.4, dt=2e-6.
+
+    Seeded sequential rejection fills a 0.1 x 0.1 m column above the hopper to
+    solid fraction `phi` (the column height grows with n; 0.1 x 0.1 x 0.2 m
+    cannot hold 1M such spheres).  Returns (params, x, r, m, meshes)."""
+    rmin, rmax = 0.5e-3, 1.5e-3
+    mean_r3 = (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin))
+    vol = n * 4.0 / 3.0 * np.pi * mean_r3
+    height = vol / phi / (0.1 * 0.1)
+    hv9, hn3 = hopper_mesh(0.008, 30.0, 0.075, y0=0.0, nseg=64, nring=32)
+    y_top = float(hv9[:, 1::3].max())
+    lo = (-0.05, y_top + 0.005, -0.05)
+    hi = (0.05, y_top + 0.005 + height, 0.05)
+    x, r = rsa(n, lo, hi, rmin, rmax, seed)
+    m = sphere_mass(r, 2500.0)
+    p = abi.default_params(2e-6, (-0.1, -0.05, -0.1), (0.1, hi[1] + 0.01, 0.1))
+    p.normal_model = abi.RP_NORMAL_HERTZ
+    p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
+    p.particle = abi.RpMaterial(5e7, 0.25, 2500.0, 0.4, 0.6, 0.0)
+    return p, x, r, m, [(hv9, hn3)]
+
+
+def box_settle_workload(n: int = 10_000, seed: int = 0xD3E50001):
+    """10,000 spheres r=5 mm, rho 2500, jittered lattice (spacing 11 mm, +-0.4 mm)
+    in the box [-0.15, 0.15]^3 closed by a 12-triangle STL box, Hertz + Coulomb,
+    E=1e7, nu=0.3, e=0.5, mu=0.5, dt=1e-5."""
+    counts = (22, 22, 21)
+    spacing = 0.011
+    ext = np.array(counts) - 1
+    origin = -0.5 * spacing * ext
+    x = jittered_lattice(counts, spacing, origin, 0.0004, seed, n)
+    r = np.full(x.shape[0], 0.005)
+    m = sphere_mass(r, 2500.0)
+    p = abi.default_params(1e-5, (-0.15, -0.15, -0.15), (0.15, 0.15, 0.15))
+    p.normal_model = abi.RP_NORMAL_HERTZ
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(1e7, 0.3, 2500.0, 0.5, 0.5, 0.0)
+    v9, n3 = box_mesh((-0.15, -0.15, -0.15), (0.15, 0.15, 0.15))
+    return p, x, r, m, [(v9, n3)]

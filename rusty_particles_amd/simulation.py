@@ -1,0 +1,284 @@
+"""Python surface of rusty_particles (src/lib.rs:15-277) over the HIP engine.
+
+Method names, argument order, defaults and exception types follow the pyo3
+class `Simulation` of the reference.  Differences, all on purpose:
+
+* The time step runs on the GPU engine (librp_hip.so).  The first step()
+  creates the engine if enable_gpu() was not called; with no HIP device it
+  raises RuntimeError, the error enable_gpu raises in the reference
+  (src/lib.rs:146-148).  There is no CPU time step in this package.
+* State is FP64.  Setter values are kept as given (the reference rounds them
+  to f32); run() still computes its step count in f32 as lib.rs:183 does.
+* Setters and add_particle after the engine exists take effect (the
+  reference's GPU path snapshots them at enable_gpu, src/gpu.rs:92-115), and
+  getters return the engine's current state.
+"""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+
+from . import abi
+from . import native
+
+_threads_set = False
+
+
+def set_num_threads(num_threads: int) -> None:
+    """src/lib.rs:15-22.  The reference sizes rayon's global pool, which can be
+    built once; a second call raises RuntimeError.  The GPU step has no host
+    thread pool, so only that contract is kept."""
+    global _threads_set
+    if _threads_set:
+        raise RuntimeError("Failed to set num threads: The global thread pool has already been initialized.")
+    if int(num_threads) < 0:
+        raise OverflowError("can't convert negative int to unsigned")
+    _threads_set = True
+
+
+def hello() -> str:
+    """src/lib.rs:28-31."""
+    return "Hello from Rust!"
+
+
+_FIELDS3 = ("x", "v", "w", "a", "tau", "rx", "rv", "rw")
+
+
+class Simulation:
+    """The reference's `Simulation` class (src/lib.rs:41-277)."""
+
+    def __init__(self, dt: float, bmin, bmax, particle_count: int = 0):
+        self._params = abi.default_params(float(dt), bmin, bmax)
+        self._dt32 = np.float32(dt)
+        n0 = 0
+        self._h = {k: np.zeros((n0, 3)) for k in _FIELDS3}
+        self._h["radius"] = np.zeros(n0)
+        self._h["mass"] = np.zeros(n0)
+        self._h["fixed"] = np.zeros(n0, dtype=np.uint8)
+        self._host_valid = True      # host arrays hold the latest state
+        self._engine_valid = False   # engine holds the latest state
+        self._params_dirty = True
+        self._meshes: list[tuple[np.ndarray, np.ndarray]] = []
+        self._engine: native.Engine | None = None
+        self._record = False
+
+    # ---- construction (src/lib.rs:57-80) ---------------------------------------
+    @staticmethod
+    def create(dt, min_x, min_y, min_z, max_x, max_y, max_z, particle_count):
+        return Simulation(dt, (min_x, min_y, min_z), (max_x, max_y, max_z), particle_count)
+
+    def add_particle(self, x, y, z, radius, mass, fixed=False):
+        self._sync_host()
+        h = self._h
+        h["x"] = np.vstack([h["x"], [[float(x), float(y), float(z)]]])
+        for k in ("v", "w", "a", "tau", "rx", "rv", "rw"):
+            h[k] = np.vstack([h[k], np.zeros((1, 3))])
+        h["radius"] = np.append(h["radius"], float(radius))
+        h["mass"] = np.append(h["mass"], float(mass))
+        h["fixed"] = np.append(h["fixed"], np.uint8(1 if fixed else 0))
+        self._engine_valid = False
+
+    def add_particles(self, positions, radii, masses, fixed=None):
+        """Bulk form of add_particle (same semantics, one call for many particles)."""
+        self._sync_host()
+        pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        h = self._h
+        h["x"] = np.vstack([h["x"], pos])
+        for k in ("v", "w", "a", "tau", "rx", "rv", "rw"):
+            h[k] = np.vstack([h[k], np.zeros((n, 3))])
+        h["radius"] = np.concatenate([h["radius"], np.broadcast_to(np.asarray(radii, dtype=np.float64), (n,))])
+        h["mass"] = np.concatenate([h["mass"], np.broadcast_to(np.asarray(masses, dtype=np.float64), (n,))])
+        f = np.zeros(n, dtype=np.uint8) if fixed is None else np.asarray(fixed, dtype=np.uint8).reshape(n)
+        h["fixed"] = np.concatenate([h["fixed"], f])
+        self._engine_valid = False
+
+    def add_mesh(self, path):
+        """src/lib.rs:87-91 (IOError when the file cannot be read)."""
+        v9, n3 = native.read_stl(str(path))
+        self.add_mesh_arrays(v9, n3)
+
+    def add_mesh_arrays(self, v9, n3):
+        """Appends triangles given as (ntri, 9) vertices and (ntri, 3) normals."""
+        v9 = np.asarray(v9, dtype=np.float32).reshape(-1, 9)
+        n3 = np.asarray(n3, dtype=np.float32).reshape(-1, 3)
+        self._meshes.append((v9, n3))
+        if self._engine is not None:
+            self._engine.add_mesh(v9, n3)
+
+    # ---- stepping (src/lib.rs:94-96, 182-195) --------------------------------------
+    def step(self):
+        self._steps(1)
+
+    def run(self, duration):
+        steps = int(math.ceil(float(np.float32(duration) / self._dt32)))
+        if steps > 0:
+            self._steps(steps)
+
+    def _steps(self, n: int) -> None:
+        eng = self._ensure_engine()
+        if self._params_dirty:
+            eng.set_params(self._params)
+            self._params_dirty = False
+        if not self._engine_valid:
+            h = self._h
+            eng.upload(h["x"], h["radius"], h["mass"], h["fixed"], v=h["v"], w=h["w"], a=h["a"], tau=h["tau"],
+                       rx=h["rx"], rv=h["rv"], rw=h["rw"])
+            self._engine_valid = True
+        eng.step(n)
+        self._host_valid = False
+
+    # ---- materials, models, boundaries (src/lib.rs:98-180) --------------------------
+    def set_particle_material(self, youngs_modulus, poissons_ratio, density, friction_coefficient,
+                              restitution_coefficient, surface_energy=0.0):
+        self._params.particle = abi.RpMaterial(youngs_modulus, poissons_ratio, density, friction_coefficient,
+                                               restitution_coefficient, surface_energy)
+        self._params_dirty = True
+
+    def set_wall_material(self, youngs_modulus, poissons_ratio, density, friction_coefficient,
+                          restitution_coefficient, surface_energy=0.0):
+        self._params.wall = abi.RpMaterial(youngs_modulus, poissons_ratio, density, friction_coefficient,
+                                           restitution_coefficient, surface_energy)
+        self._params_dirty = True
+
+    def set_periodic_boundary(self, x, y, z):
+        for k, flag in enumerate((x, y, z)):
+            self._params.periodic[k] = 1 if flag else 0
+        self._params_dirty = True
+
+    def set_contact_models(self, normal, tangential):
+        nm = abi.NORMAL_MODELS.get(str(normal).lower())
+        if nm is None:
+            raise ValueError("Invalid normal force model")
+        tm = abi.TANGENTIAL_MODELS.get(str(tangential).lower())
+        if tm is None:
+            raise ValueError("Invalid tangential force model")
+        self._params.normal_model = nm
+        self._params.tangential_model = tm
+        self._params_dirty = True
+
+    def set_gravity(self, gx, gy, gz):
+        """Extension: the reference hard-codes (0, -9.81, 0) (src/simulation.rs:267)."""
+        self._params.gravity[0], self._params.gravity[1], self._params.gravity[2] = gx, gy, gz
+        self._params_dirty = True
+
+    def set_engine_tuning(self, cell_size=0.0, skin=0.0):
+        """Extension: broad-phase cell size and Verlet skin (0 = automatic)."""
+        self._params.cell_size = float(cell_size)
+        self._params.skin = float(skin)
+        self._params_dirty = True
+
+    def enable_gpu(self):
+        """src/lib.rs:146-148: RuntimeError when no device can be used."""
+        self._ensure_engine()
+
+    def init_mpi(self):
+        """src/lib.rs:150-152.  One GPU holds the whole domain: no slab decomposition."""
+        print("init_mpi: single-GPU engine, the whole domain stays on one device", file=sys.stderr)
+
+    # ---- output and getters (src/lib.rs:202-276) --------------------------------------
+    def write_vtk(self, filename):
+        """Legacy ASCII VTK POLYDATA, layout of src/vtk.rs:5-44."""
+        self._sync_host()
+        h = self._h
+        n = h["x"].shape[0]
+        f = lambda v: np.format_float_positional(np.float32(v), unique=True, trim="-")
+        try:
+            with open(filename, "w") as fh:
+                fh.write("# vtk DataFile Version 3.0\nDEM Simulation Data\nASCII\nDATASET POLYDATA\n")
+                fh.write(f"POINTS {n} float\n")
+                for p in h["x"]:
+                    fh.write(f"{f(p[0])} {f(p[1])} {f(p[2])}\n")
+                fh.write(f"POINT_DATA {n}\nVECTORS velocity float\n")
+                for p in h["v"]:
+                    fh.write(f"{f(p[0])} {f(p[1])} {f(p[2])}\n")
+                fh.write("SCALARS radius float 1\nLOOKUP_TABLE default\n")
+                for r in h["radius"]:
+                    fh.write(f"{f(r)}\n")
+                fh.write("SCALARS id int 1\nLOOKUP_TABLE default\n")
+                for i in range(n):
+                    fh.write(f"{i}\n")
+        except OSError as e:
+            raise OSError(str(e)) from e
+
+    def get_particle_count(self):
+        if not self._host_valid and self._engine is not None:
+            return self._engine.count()
+        return int(self._h["x"].shape[0])
+
+    def _get3(self, key, idx):
+        self._sync_host()
+        idx = int(idx)
+        if idx < 0 or idx >= self._h["x"].shape[0]:
+            raise IndexError("Particle index out of bounds")
+        p = self._h[key][idx]
+        return (float(p[0]), float(p[1]), float(p[2]))
+
+    def get_particle_position(self, id):
+        return self._get3("x", id)
+
+    def get_particle_velocity(self, id):
+        return self._get3("v", id)
+
+    def get_particle_angular_velocity(self, index):
+        return self._get3("w", index)
+
+    def _set3(self, key, idx, a, b, c):
+        self._sync_host()
+        idx = int(idx)
+        if idx < 0 or idx >= self._h["x"].shape[0]:
+            raise IndexError("Particle index out of bounds")
+        self._h[key][idx] = (float(a), float(b), float(c))
+        self._engine_valid = False
+
+    def set_particle_velocity(self, id, vx, vy, vz):
+        self._set3("v", id, vx, vy, vz)
+
+    def set_particle_angular_velocity(self, index, x, y, z):
+        self._set3("w", index, x, y, z)
+
+    # ---- extensions for bulk access and diagnostics --------------------------------------
+    def state(self) -> dict:
+        """Copy of the full FP64 state in API order (x, v, w, a, tau, residuals, radius, mass, fixed)."""
+        self._sync_host()
+        return {k: np.array(v) for k, v in self._h.items()}
+
+    def set_record_pairs(self, on=True):
+        self._record = bool(on)
+        if self._engine is not None:
+            self._engine.set_record(self._record)
+
+    def contact_pairs(self):
+        """(offsets, js) pair list of the last step (needs set_record_pairs(True) before stepping)."""
+        if self._engine is None:
+            raise RuntimeError("no step has run on the engine")
+        return self._engine.pairs()
+
+    def stats(self) -> dict:
+        if self._engine is None:
+            return {}
+        return self._engine.stats()
+
+    # ---- internals ---------------------------------------------------------------------
+    def _ensure_engine(self) -> native.Engine:
+        if self._engine is None:
+            try:
+                eng = native.Engine(self._params)
+            except native.EngineError as e:
+                raise RuntimeError(str(e)) from e
+            self._engine = eng
+            self._params_dirty = False
+            self._engine_valid = False
+            for v9, n3 in self._meshes:
+                eng.add_mesh(v9, n3)
+            eng.set_record(self._record)
+        return self._engine
+
+    def _sync_host(self) -> None:
+        if self._host_valid:
+            return
+        st = self._engine.download()
+        self._h = {k: st[k] for k in (*_FIELDS3, "radius", "mass", "fixed")}
+        self._host_valid = True

@@ -1,0 +1,129 @@
+"""HIP engine vs the CPU oracle (FP64) on the same seeded inputs.
+
+Bars (the reference's step_cpu restated in rp_oracle.cpp is the oracle):
+  * contact pair lists: bit-exact, same set, same reference neighbour order;
+  * a and tau after one step: within 1e-10 relative to the per-particle sum of
+    contact-force magnitudes (forces are summed in a different order);
+  * 100-step trajectories: within the tolerances stated per test.
+"""
+import numpy as np
+import pytest
+
+from rusty_particles_amd import abi, scenes
+
+from helpers import (assert_forces_close, assert_pairs_equal, assert_state_close, dense_scene, params,
+                     run_engine, run_oracle)
+
+pytestmark = pytest.mark.gpu
+
+NORMALS = [abi.RP_NORMAL_LINEAR, abi.RP_NORMAL_HERTZ, abi.RP_NORMAL_HYSTERETIC, abi.RP_NORMAL_JKR,
+           abi.RP_NORMAL_SJKR]
+TANGENTIALS = [abi.RP_TANGENTIAL_COULOMB, abi.RP_TANGENTIAL_LSC, abi.RP_TANGENTIAL_MINDLIN]
+
+
+def _box_case(normal, tangential, gamma=0.0, seed=1):
+    sc = dense_scene(counts=(9, 9, 9), seed=seed)
+    half = 0.5 * 8 * 0.0098 + 0.0045  # walls cut into the outer layer
+    lo, hi = (-half,) * 3, (half,) * 3
+    p = params(1e-6, lo, hi, normal=normal, tangential=tangential, gamma=gamma,
+               wall=(1e8, 0.3, 7800.0, 0.5, 0.5, gamma))
+    mesh = scenes.box_mesh(lo, hi)
+    return p, sc, [mesh]
+
+
+@pytest.mark.parametrize("normal", NORMALS)
+@pytest.mark.parametrize("tangential", TANGENTIALS)
+def test_one_step_models(normal, tangential):
+    gamma = 0.05 if normal in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 0.0
+    print(f"case normal={normal} tangential={tangential}", flush=True)
+    p, sc, meshes = _box_case(normal, tangential, gamma=gamma)
+    o = run_oracle(p, sc, meshes, steps=1)
+    e = run_engine(p, sc, meshes, steps=1)
+    npairs = assert_pairs_equal(o, e)
+    assert npairs > 1000
+    if normal == abi.RP_NORMAL_JKR and tangential == abi.RP_TANGENTIAL_MINDLIN:
+        # sqrt(R* delta) of a separated JKR pair is NaN, so kt = 0*NaN = NaN in the
+        # reference (physics.rs:456): both sides must produce the same NaN pattern.
+        so, se = o.state(), e.download()
+        assert np.array_equal(np.isnan(so["a"]), np.isnan(se["a"]))
+        return
+    assert_forces_close(o, e, p)
+
+
+def test_walls_and_fixed():
+    sc = dense_scene(counts=(8, 8, 8), seed=3, fixed_frac=0.1)
+    half = 0.5 * 7 * 0.0098 + 0.004
+    p = params(1e-6, (-half,) * 3, (half,) * 3)
+    meshes = [scenes.box_mesh((-half,) * 3, (half,) * 3)]
+    o = run_oracle(p, sc, meshes, steps=1)
+    e = run_engine(p, sc, meshes, steps=1)
+    assert_pairs_equal(o, e)
+    assert_forces_close(o, e, p)
+    st = e.download()
+    assert e.stats()["wall_contacts"] > 0
+    fx = sc["fixed"].astype(bool)
+    assert np.array_equal(st["x"][fx], sc["x"][fx])
+
+
+@pytest.mark.parametrize("normal", [abi.RP_NORMAL_HERTZ, abi.RP_NORMAL_JKR])
+def test_periodic_box(normal):
+    n, s = 8, 0.0098
+    L = n * s
+    sc = dense_scene(counts=(n, n, n), spacing=s, seed=5, centre=(0.5 * s - 0.5 * s, 0, 0))
+    sc["x"] = sc["x"] - sc["x"].min(0) + 0.5 * s  # fill [0, L) so contacts wrap
+    gamma = 0.05 if normal == abi.RP_NORMAL_JKR else 0.0
+    p = params(1e-6, (0.0, 0.0, 0.0), (L, L, L), normal=normal, gamma=gamma, periodic=(1, 1, 1))
+    o = run_oracle(p, sc, (), steps=1)
+    e = run_engine(p, sc, (), steps=1)
+    assert_pairs_equal(o, e)
+    assert_forces_close(o, e, p)
+
+
+def test_reference_grid_cells_and_visibility():
+    """Bounds of 3 m make a 4x4x4 reference grid (cell 1 m): the lattice straddles
+    the cell planes at -0.5, and a block sits below bounds_min.x, where the
+    reference grid cannot see it (grid.rs:30-40) although it still gathers."""
+    a = dense_scene(counts=(8, 8, 8), seed=7, centre=(-0.5, -0.5, -0.5))
+    b = dense_scene(counts=(4, 4, 4), seed=8, centre=(-1.5, 0.0, 0.0))
+    sc = {k: np.concatenate([a[k], b[k]]) for k in a}
+    p = params(1e-6, (-1.5, -1.5, -1.5), (1.5, 1.5, 1.5))
+    o = run_oracle(p, sc, (), steps=1)
+    e = run_engine(p, sc, (), steps=1)
+    assert_pairs_equal(o, e)
+    assert_forces_close(o, e, p)
+
+
+def test_removal_and_reindex():
+    sc = dense_scene(counts=(6, 6, 6), seed=9)
+    n = sc["x"].shape[0]
+    # a few particles just above the removal plane y = ymin - 1, moving down fast
+    idx = np.arange(0, n, 17)
+    sc["x"][idx, 1] = -0.2 - 1.0 + 1e-4
+    sc["v"][idx] = (0.0, -50.0, 0.0)
+    p = params(1e-5, (-0.2, -0.2, -0.2), (0.2, 0.2, 0.2))
+    o = run_oracle(p, sc, (), steps=3)
+    e = run_engine(p, sc, (), steps=3)
+    assert o.get_particle_count() == e.count() < n
+    assert_pairs_equal(o, e)
+    assert_state_close(o, e, 1e-15, 1e-12, 1e-9)
+
+
+def test_trajectory_100_steps():
+    """Hertz + Coulomb in a walled box with gravity, 100 steps of 1e-6 s, issued
+    as several rp_engine_step calls.  FP64 summation-order rounding (~1e-16
+    relative per step) is the only difference, so positions agree to 1e-12 m
+    and velocities to 1e-9 m/s over 100 steps."""
+    p, sc, meshes = _box_case(abi.RP_NORMAL_HERTZ, abi.RP_TANGENTIAL_COULOMB)
+    o = run_oracle(p, sc, meshes, steps=100)
+    e = run_engine(p, sc, meshes, chunks=[1, 9, 40, 50])
+    assert_pairs_equal(o, e)
+    assert_state_close(o, e, 1e-12, 1e-9, 1e-6)
+
+
+def test_deterministic_rerun():
+    p, sc, meshes = _box_case(abi.RP_NORMAL_HERTZ, abi.RP_TANGENTIAL_COULOMB, seed=11)
+    e1 = run_engine(p, sc, meshes, steps=20)
+    e2 = run_engine(p, sc, meshes, chunks=[5, 15])
+    s1, s2 = e1.download(), e2.download()
+    for k in ("x", "v", "w", "a", "tau"):
+        assert np.array_equal(s1[k], s2[k]), k
