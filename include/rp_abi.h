@@ -208,6 +208,16 @@ rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
 rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
+/* Turns per-kernel timing on or off (off by default).  When on, every
+ * force-kernel launch, every rebuild chain and every first-half kernel is
+ * bracketed by HIP events on the engine's stream. */
+rp_status rp_engine_set_timing(rp_engine* e, int on);
+
+/* Synchronises, then returns the summed event times in milliseconds and the
+ * number of timed launches since the last call: index 0 = force kernel,
+ * 1 = rebuild chain (including its no-op launches), 2 = first half kernel. */
+rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3);
+
 /* Counters (rp_stats). */
 rp_status rp_engine_stats(rp_engine* e, rp_stats* out);
 

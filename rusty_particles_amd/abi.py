@@ -123,6 +123,8 @@ EXPORTS = (
     "rp_stl_read",
     "rp_engine_pairs",
     "rp_engine_stats",
+    "rp_engine_set_timing",
+    "rp_engine_timing",
     "rp_engine_destroy",
     "rp_last_error",
 )

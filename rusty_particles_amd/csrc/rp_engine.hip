@@ -437,7 +437,22 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
         }
       }
     }
-    nbr_count[i] = min(cnt, P.nbr_cap);
+    const int kept = min(cnt, P.nbr_cap);
+    // Ascending stable id = ascending API index: the reference's summation
+    // order inside one reference cell (grid.rs:42-46 inserts in index order).
+    for (int a = 1; a < kept; ++a) {
+      int ja = nbr[a * cap + i];
+      uint32_t ka = U.sid[ja];
+      int b = a - 1;
+      while (b >= 0) {
+        int jb = nbr[b * cap + i];
+        if (U.sid[jb] <= ka) break;
+        nbr[(b + 1) * cap + i] = jb;
+        --b;
+      }
+      nbr[(b + 1) * cap + i] = ja;
+    }
+    nbr_count[i] = kept;
     local_max = max(local_max, cnt);
     xb[i] = (float)xi.x;
     xb[i + cap] = (float)xi.y;
@@ -855,6 +870,11 @@ struct rp_engine {
   int* d_paircount = nullptr;
   int pair_cap = 0;
   bool pairs_valid = false;
+  // optional per-kernel timing with HIP events on the engine stream
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event); end = start + 1
 };
 
 namespace {
@@ -1241,6 +1261,31 @@ rp_status ensure_pairs(rp_engine* e) {
   return RP_OK;
 }
 
+// kind: 0 = force kernel, 1 = rebuild chain, 2 = first half (k_begin)
+hipEvent_t next_event(rp_engine* e) {
+  if (e->ev_used == e->ev_pool.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    e->ev_pool.push_back(ev);
+  }
+  return e->ev_pool[e->ev_used++];
+}
+
+void mark_begin(rp_engine* e, int kind) {
+  if (!e->timing) return;
+  size_t idx = e->ev_used;
+  hipEvent_t a = next_event(e);
+  hipEvent_t b = next_event(e);
+  if (!a || !b) return;
+  (void)hipEventRecord(a, e->stream);
+  e->ev_marks.emplace_back(kind, idx);
+}
+
+void mark_end(rp_engine* e) {
+  if (!e->timing || e->ev_marks.empty()) return;
+  (void)hipEventRecord(e->ev_pool[e->ev_marks.back().second + 1], e->stream);
+}
+
 template <bool ADV, bool REC>
 void launch_force(rp_engine* e, int par) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
@@ -1433,18 +1478,24 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (e->record && (st = ensure_pairs(e))) return st;
   int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   int par = (int)(e->step_count & 1);
+  mark_begin(e, 2);
   hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp, e->d_xb);
+  mark_end(e);
   for (uint64_t s = 0; s < nsteps; ++s) {
     par = (int)(e->step_count & 1);
+    mark_begin(e, 1);
     if ((st = launch_rebuild(e, par))) return st;
+    mark_end(e);
     bool last = (s + 1 == nsteps);
     if (last) hipLaunchKernelGGL(k_reset_step_counters, dim3(1), dim3(1), 0, e->stream, e->d_ctl);
+    mark_begin(e, 0);
     if (last) {
       if (e->record) launch_force<false, true>(e, par);
       else launch_force<false, false>(e, par);
     } else {
       launch_force<true, false>(e, par);
     }
+    mark_end(e);
     e->step_count++;
   }
   HIP_TRY(hipGetLastError());
@@ -1616,9 +1667,37 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
   return RP_OK;
 }
 
+rp_status rp_engine_set_timing(rp_engine* e, int on) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->timing = on != 0;
+  e->ev_used = 0;
+  e->ev_marks.clear();
+  return RP_OK;
+}
+
+rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3) {
+  if (!e || !ms3 || !count3) return fail(RP_ERR_INVALID, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int k = 0; k < 3; ++k) {
+    ms3[k] = 0.0;
+    count3[k] = 0;
+  }
+  for (auto& mk : e->ev_marks) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev_pool[mk.second], e->ev_pool[mk.second + 1]));
+    ms3[mk.first] += (double)ms;
+    count3[mk.first] += 1;
+  }
+  e->ev_used = 0;
+  e->ev_marks.clear();
+  return RP_OK;
+}
+
 void rp_engine_destroy(rp_engine* e) {
   if (!e) return;
   (void)hipStreamSynchronize(e->stream);
+  for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
   free_particles(e);
   free_ptr(e->d_cell_count);
   free_ptr(e->d_cell_start);

@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
     L.rp_stl_read.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), _fp, _fp]
     L.rp_engine_pairs.argtypes = [_eng, _i64p, _i64p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
+    L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
+    L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -59,7 +61,8 @@ def lib() -> C.CDLL:
     for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
-                 "rp_stl_read", "rp_engine_pairs", "rp_engine_stats"):
+                 "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
+                 "rp_engine_timing"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -186,6 +189,17 @@ class Engine:
         _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
                                        C.byref(tot)))
         return offs, js[: tot.value]
+
+    def set_timing(self, on: bool) -> None:
+        _check(self._L.rp_engine_set_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> dict:
+        """Summed HIP-event times (ms) and launch counts since the last call."""
+        ms = np.zeros(3)
+        cnt = (C.c_uint64 * 3)()
+        _check(self._L.rp_engine_timing(self._h, _ptr(ms), cnt))
+        names = ("force", "rebuild", "begin")
+        return {names[k]: (float(ms[k]), int(cnt[k])) for k in range(3)}
 
     def stats(self) -> dict:
         s = abi.RpStats()
