@@ -86,6 +86,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (torch) so a counter profile of this "
+                         "run can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not part of "
@@ -120,7 +123,8 @@ def main() -> int:
     wall = time.perf_counter() - t0
     st = eng.stats()
     value = n * args.steps / wall
-    log(f"timed {args.steps} steps: {wall:.3f}s, {value:.4g} particle-steps/s, rebuilds {st['rebuilds']}")
+    log(f"timed {args.steps} steps: {wall:.3f}s, {value:.4g} particle-steps/s, rebuilds {st['rebuilds']}, "
+        f"max list length {st['max_neighbours']}, list capacity {st['neighbour_capacity']}")
 
     # kernel roofline: HIP events around each launch on the engine stream
     eng.set_timing(True)
@@ -133,6 +137,15 @@ def main() -> int:
     achieved = bytes_per_launch / avg_s / 1e9
     log(f"force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
         f"rebuild chain avg {tm['rebuild'][0] / max(tm['rebuild'][1], 1) * 1e3:.1f} us")
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 268435456 bytes read + 268435456 bytes written")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{args.workload}_force_traffic.json")
     if os.path.exists(tfile):

@@ -90,25 +90,25 @@ struct Stat {
   uint32_t* sid;  // stable id = insertion index (API order key)
 };
 
-// Device-resident control block.
+// Device-resident control block: slot count and rare flags.  Per-step
+// counters live in per-workgroup arrays written with plain stores, so no
+// kernel performs same-address atomics on its common path.
 struct Ctl {
-  Dyn dyn[2];
-  Stat stat[2];
-  int cur_dyn;             // current Dyn buffer
-  int cur_stat;            // current Stat buffer
   int n;                   // slots in use (alive + removed-not-yet-compacted)
   int alive;               // alive particles
   int rebuild[2];          // rebuild requests, indexed by step parity
-  unsigned int done;       // finished workgroups of k_build
-  unsigned int done_force; // finished workgroups of k_force
   int overflow;            // neighbour list overflow seen
   int max_count;           // largest candidate count seen by k_build
-  int overlap;             // excessive-overlap flag of the last step
-  unsigned long long contacts;       // directed P-P contacts (with multiplicity), last step
-  unsigned long long wall_contacts;  // particle-triangle contacts, last step
-  unsigned long long rebuilds;       // rebuilds done
-  unsigned long long jkr_failures;   // JKR residual rejections, last step
+  unsigned long long rebuilds;  // rebuilds done
   int pair_overflow;       // pair record overflow
+};
+
+// Per-workgroup counters of one force launch.
+struct BlockStats {
+  unsigned int contacts;       // directed P-P contacts (with multiplicity)
+  unsigned int wall_contacts;  // particle-triangle contacts
+  unsigned int jkr_failures;   // JKR residual rejections
+  unsigned int overlap;        // excessive-overlap flag
 };
 
 // Everything a kernel needs that is fixed between parameter changes.
@@ -168,31 +168,32 @@ __device__ inline int fine_coord(double x, int k, const KParams& P, bool clamp_o
 }
 
 // Reference-grid cell coordinate (grid.rs:84-86): floor((x - origin)/cs) as i32.
-__device__ inline int ref_coord(double x, int k, const KParams& P) {
-  return sat_i32(floor((x - P.ref_origin[k]) / P.ref_cs));
+// The axis is a template argument so every parameter access has a constant
+// index (a runtime index would copy the parameter block to scratch memory).
+template <int K>
+__device__ inline int ref_coord(double x, const KParams& P) {
+  return sat_i32(floor((x - P.ref_origin[K]) / P.ref_cs));
 }
 
-// Number of offsets o in {-1,0,1} whose wrapped reference neighbour cell of
-// ci equals cj (grid.rs:108-129).  Also returns a bit mask of matching offsets
-// (bit o+1).
-__device__ inline int ref_matches(int ci, int cj, int k, const KParams& P, int* mask) {
-  int cnt = 0, msk = 0;
-  int d = P.ref_dims[k];
+// Bit mask of the offsets o in {-1,0,1} (bit o+1) whose wrapped reference
+// neighbour cell of ci equals cj (grid.rs:108-129); popcount = how often the
+// reference's 27-cell walk reaches cj along this axis.
+template <int K>
+__device__ inline int ref_mask(int ci, int cj, const KParams& P) {
+  const int d = P.ref_dims[K];
+  int msk = 0;
+#pragma unroll
   for (int o = -1; o <= 1; ++o) {
     int nn = wrap_add_i32(ci, o);
-    if (P.periodic[k]) {
+    if (P.periodic[K]) {
       if (nn < 0)
         nn += d;
       else if (nn >= d)
         nn -= d;
     }
-    if (nn == cj) {
-      ++cnt;
-      msk |= 1 << (o + 1);
-    }
+    if (nn == cj) msk |= 1 << (o + 1);
   }
-  *mask = msk;
-  return cnt;
+  return msk;
 }
 
 // XCD-aware block index: consecutive logical blocks land on the same XCD
@@ -204,8 +205,13 @@ __device__ inline int xcd_block(int b, int nb) {
   return x * q + min(x, r) + k;
 }
 
-__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+__device__ inline unsigned int wave_sum_u32(unsigned int v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ inline int wave_max_i32(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off, 64));
   return v;
 }
 
@@ -213,12 +219,10 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 
 // Keys + histogram.  Also clears the other parity's rebuild request (it was
 // last read one step ago and is next written by this step's force kernel).
-__global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, uint32_t* key,
+__global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, Dyn S, Stat T, uint32_t* key,
                                                  uint32_t* cell_count) {
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rebuild[par ^ 1] = 0;
   if (!ctl->rebuild[par]) return;
-  const Dyn S = ctl->dyn[ctl->cur_dyn];
-  const Stat T = ctl->stat[ctl->cur_stat];
   const int n = ctl->n;
   const int cap = P.cap;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -332,15 +336,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const Ctl* ctl, int par, con
 }
 
 // Deterministic slot = cell start + rank by stable id among the cell's
-// particles; copies the persistent state into the other buffer.  Removed
+// particles; copies the state into the scratch buffers (D, U).  Removed
 // particles (sentinel cell) are dropped.
-__global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KParams P, const uint32_t* key,
-                                                         const uint32_t* cell_start, const uint32_t* tmp) {
+__global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KParams P, Dyn S, Stat T, Dyn D,
+                                                         Stat U, const uint32_t* key, const uint32_t* cell_start,
+                                                         const uint32_t* tmp) {
   if (!ctl->rebuild[par]) return;
-  const Dyn S = ctl->dyn[ctl->cur_dyn];
-  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
-  const Stat T = ctl->stat[ctl->cur_stat];
-  const Stat U = ctl->stat[ctl->cur_stat ^ 1];
   const int n = ctl->n;
   const int cap = P.cap;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -369,14 +370,13 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   }
 }
 
-// Verlet list of slot i (in the new buffer): every j with
-// |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin.  The last workgroup to
-// finish makes the new buffer current and records the new slot count.
-__global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, const uint32_t* cell_start,
-                                                  int* nbr_count, int* nbr, float* xb) {
+// Verlet list of slot i (from the re-ordered scratch buffers D, U): every j
+// with |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin; then copies slot
+// i back into the working buffers (W, V) and records the new slot count.
+__global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, Dyn D, Stat U, Dyn W, Stat V,
+                                                  const uint32_t* cell_start, int* nbr_count, int* nbr,
+                                                  float* xb) {
   if (!ctl->rebuild[par]) return;
-  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
-  const Stat U = ctl->stat[ctl->cur_stat ^ 1];
   const int n = (int)cell_start[P.ncells];  // alive particles after compaction
   const int cap = P.cap;
   int local_max = 0;
@@ -457,25 +457,28 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
     xb[i] = (float)xi.x;
     xb[i + cap] = (float)xi.y;
     xb[i + 2 * cap] = (float)xi.z;
+    for (int k = 0; k < 3; ++k) {
+      const int o = i + k * cap;
+      W.x[o] = D.x[o];
+      W.v[o] = D.v[o];
+      W.w[o] = D.w[o];
+      W.rx[o] = D.rx[o];
+      W.rv[o] = D.rv[o];
+      W.rw[o] = D.rw[o];
+      W.a[o] = D.a[o];
+      W.tau[o] = D.tau[o];
+    }
+    V.r[i] = U.r[i];
+    V.m[i] = U.m[i];
+    V.flags[i] = U.flags[i];
+    V.sid[i] = U.sid[i];
   }
-  if (local_max > 0) {
-    atomicMax(&ctl->max_count, local_max);
-    if (local_max > P.nbr_cap) atomicOr(&ctl->overflow, 1);
+  const int wmax = wave_max_i32(local_max);
+  if ((threadIdx.x & 63) == 0 && wmax > 0) {
+    atomicMax(&ctl->max_count, wmax);
+    if (wmax > P.nbr_cap) atomicOr(&ctl->overflow, 1);
   }
-  // last workgroup: swap buffers
-  __syncthreads();
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(&ctl->done, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    ctl->done = 0;
-    ctl->cur_dyn ^= 1;
-    ctl->cur_stat ^= 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->n = n;
     ctl->rebuilds += 1;
   }
@@ -523,17 +526,20 @@ __device__ inline bool begin_step(const KParams& P, dv3& x, dv3& v, dv3& w, dv3&
 }
 
 // Displacement since the last list build, minimum image on periodic axes.
-__device__ inline bool moved_too_far(const KParams& P, dv3 x, const float* xb, int i) {
-  double d2 = 0.0;
-  for (int k = 0; k < 3; ++k) {
-    double d = comp(x, k) - (double)xb[i + k * P.cap];
-    if (P.periodic[k]) {
-      if (d > P.bsize[k] * 0.5) d -= P.bsize[k];
-      else if (d < -P.bsize[k] * 0.5) d += P.bsize[k];
-    }
-    d2 += d * d;
+template <int K>
+__device__ inline double mic_disp(double d, const KParams& P) {
+  if (P.periodic[K]) {
+    if (d > P.bsize[K] * 0.5) d -= P.bsize[K];
+    else if (d < -P.bsize[K] * 0.5) d += P.bsize[K];
   }
-  return !(d2 < P.half_skin * P.half_skin);
+  return d;
+}
+
+__device__ inline bool moved_too_far(const KParams& P, dv3 x, const float* xb, int i) {
+  const double d0 = mic_disp<0>(x.x - (double)xb[i], P);
+  const double d1 = mic_disp<1>(x.y - (double)xb[i + P.cap], P);
+  const double d2 = mic_disp<2>(x.z - (double)xb[i + 2 * P.cap], P);
+  return !(d0 * d0 + d1 * d1 + d2 * d2 < P.half_skin * P.half_skin);
 }
 
 // A removed particle keeps slot i until the next rebuild drops it; its
@@ -550,9 +556,7 @@ __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i
 
 // First half of a step from the at-rest state (used once at the start of
 // each rp_engine_step call).
-__global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, const float* xb) {
-  const Dyn S = ctl->dyn[ctl->cur_dyn];
-  const Stat T = ctl->stat[ctl->cur_stat];
+__global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, Dyn S, Stat T, const float* xb) {
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -579,195 +583,259 @@ __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, 
   if (__any(far) && (threadIdx.x & 63) == 0) atomicOr(&ctl->rebuild[par], 1);
 }
 
-// Gather force kernel + integration epilogue.
+// Pair test of slot i against slot j (simulation.rs:323-357): minimum image
+// per periodic axis, then dist^2 < (r_sum + margin)^2.
+__device__ inline dv3 pair_delta(const KParams& P, dv3 xi, dv3 xj) {
+  dv3 d = sub(xi, xj);
+  if (P.periodic[0]) {
+    if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
+    else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
+  }
+  if (P.periodic[1]) {
+    if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
+    else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
+  }
+  if (P.periodic[2]) {
+    if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
+    else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
+  }
+  return d;
+}
+
+__device__ inline bool in_contact(const KParams& P, double dsq, double rsum) {
+  const bool adhesive = P.normal == RP_NORMAL_JKR || P.normal == RP_NORMAL_SJKR;
+  const double margin = adhesive ? rsum * 0.5 : 0.0;
+  const double cutoff = rsum + margin;
+  return dsq < cutoff * cutoff;
+}
+
+struct Own {
+  dv3 x, v, w;
+  double r, m;
+  int rc0, rc1, rc2;  // reference-grid cell of this particle
+};
+
+struct Acc {
+  dv3 f, t;
+  unsigned int ncont, njkr;
+  int npair;
+  bool overlap;
+};
+
+// One particle-particle contact of the gather (simulation.rs:357-460), added
+// `mult` times when the reference's grid walk reaches j more than once.
+template <int NORMAL, int TANG, bool RECORD>
+__device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T, const Own& o, int j, Acc& acc,
+                                  Ctl* ctl, unsigned long long* prow) {
+  const dv3 xj = ld3(S.x, j, P.cap);
+  const double rj = T.r[j];
+  const dv3 d = pair_delta(P, o.x, xj);
+  const double dsq = dot(d, d);
+  const double rsum = o.r + rj;
+  // visible to the reference grid, and reached how often by its 27-cell walk
+  const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+  if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
+    return;
+  const int mask0 = ref_mask<0>(o.rc0, cj0, P), mask1 = ref_mask<1>(o.rc1, cj1, P),
+            mask2 = ref_mask<2>(o.rc2, cj2, P);
+  const int mult = __popc(mask0) * __popc(mask1) * __popc(mask2);
+  if (mult == 0) return;
+  const double dist = sqrt(dsq);
+  const double overlap = rsum - dist;
+  if (overlap > 0.2 * fmin(o.r, rj)) acc.overlap = true;
+  const dv3 nrm = dist > 1e-6 ? divs(d, dist) : mk(0.0, 1.0, 0.0);
+  const dv3 ra = mul(neg(nrm), o.r);
+  const dv3 rb = mul(nrm, rj);
+  const dv3 vj = ld3(S.v, j, P.cap);
+  const dv3 wj = ld3(S.w, j, P.cap);
+  const double mj = T.m[j];
+  const dv3 vrel = sub(add(o.v, cross(o.w, ra)), add(vj, cross(wj, rb)));
+  const double rstar = (o.r * rj) / (o.r + rj);
+  const double mstar = (o.m * mj) / (o.m + mj);
+  dv3 fn;
+  int jfail = 0;
+  if (NORMAL == RP_NORMAL_HERTZ) {
+    fn = hertz(overlap, nrm, vrel, P.mc.pp, rstar, mstar, P.mc.beta_abs);
+  } else if (NORMAL == RP_NORMAL_LINEAR) {
+    const double kn = P.mc.pp.estar * rstar;
+    const double gn = 0.5 * sqrt(mstar * kn);
+    fn = linear_sd(overlap, nrm, vrel, kn, gn);
+  } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
+    fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0);
+  } else if (NORMAL == RP_NORMAL_JKR) {
+    fn = jkr(overlap, nrm, P.mc.pp, rstar, &jfail);
+  } else {
+    fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar);
+  }
+  acc.njkr += (unsigned int)(jfail * mult);
+  const double fnmag = len(fn);
+  dv3 ft;
+  if (TANG == RP_TANGENTIAL_COULOMB) {
+    ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+  } else {
+    // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
+    const double kt = TANG == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
+    ft = spring_coulomb_g0(fnmag, kt, P.mc.mu);
+  }
+  const dv3 fc = add(fn, ft);
+  const dv3 tc = cross(ra, fc);
+  for (int rep = 0; rep < mult; ++rep) {
+    acc.f = add(acc.f, fc);
+    acc.t = add(acc.t, tc);
+  }
+  acc.ncont += (unsigned int)mult;
+  if (RECORD) {
+    const unsigned long long sj = T.sid[j];
+    for (int oz = 0; oz < 3; ++oz) {
+      if (!(mask2 >> oz & 1)) continue;
+      for (int oy = 0; oy < 3; ++oy) {
+        if (!(mask1 >> oy & 1)) continue;
+        for (int ox = 0; ox < 3; ++ox) {
+          if (!(mask0 >> ox & 1)) continue;
+          const unsigned long long rank = (unsigned long long)(oz * 9 + oy * 3 + ox);
+          if (acc.npair < P.pair_cap) prow[acc.npair] = (rank << 32) | sj;
+          else atomicOr(&ctl->pair_overflow, 1);
+          ++acc.npair;
+        }
+      }
+    }
+  }
+}
+
+// One particle-triangle contact (simulation.rs:465-523).
+template <int NORMAL>
+__device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& tr, Acc& acc, unsigned int& nwall) {
+  const dv3 cp = closest_point(tr, o.x);
+  const dv3 dv = sub(o.x, cp);
+  const double dsq = dot(dv, dv);
+  if (!(dsq < o.r * o.r)) return;
+  const double dist = sqrt(dsq);
+  const double pen = o.r - dist;
+  const dv3 nrm = dist > 1e-6 ? divs(dv, dist) : tr.n;
+  if (pen > 0.2 * o.r) acc.overlap = true;
+  const dv3 ra = mul(neg(nrm), o.r);
+  const dv3 vrel = add(o.v, cross(o.w, ra));
+  dv3 fn;
+  int jfail = 0;
+  if (NORMAL == RP_NORMAL_HERTZ) {
+    fn = hertz(pen, nrm, vrel, P.mc.pw, o.r, o.m, P.mc.beta_abs);
+  } else if (NORMAL == RP_NORMAL_LINEAR) {
+    const double kn = P.mc.pw.estar * o.r;
+    const double gn = 0.5 * sqrt(o.m * kn);
+    fn = linear_sd(pen, nrm, vrel, kn, gn);
+  } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
+    fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * o.r, 0.0);
+  } else if (NORMAL == RP_NORMAL_JKR) {
+    fn = jkr(pen, nrm, P.mc.pw, o.r, &jfail);
+  } else {
+    fn = sjkr(pen, nrm, vrel, P.mc.pw, o.r);
+  }
+  acc.njkr += (unsigned int)jfail;
+  const double fnmag = len(fn);
+  const dv3 ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+  const dv3 fc = add(fn, ft);
+  acc.f = add(acc.f, fc);
+  acc.t = add(acc.t, cross(ra, fc));
+  ++nwall;
+}
+
+__device__ inline int sel4(int q, int a, int b, int c, int d) { return q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d)); }
+
+// Gather force kernel + integration epilogue, one thread per particle slot.
+// Reads Dyn buffer S and writes buffer D (so no thread can read a neighbour's
+// next-step state); the host swaps the two between steps.  All own-state loads are issued before the neighbour loop, and
+// neighbours are tested four at a time, so a wave waits for memory a few
+// times per particle instead of once per neighbour.
 // ADVANCE: also performs the next step's first half (begin_step) and leaves
 // the state drifted; otherwise stores a, tau and leaves the at-rest state.
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
-template <bool ADVANCE, bool RECORD>
-__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, const int* nbr_count,
-                                                  const int* nbr, const Tri* tris, const uint32_t* tri_start,
-                                                  const uint32_t* tri_idx, const float* xb,
-                                                  unsigned long long* pairbuf,
-                                                  int* paircount) {
-  const Dyn S = ctl->dyn[ctl->cur_dyn];
-  const Dyn D = ctl->dyn[ctl->cur_dyn ^ 1];
-  const Stat T = ctl->stat[ctl->cur_stat];
+template <int NORMAL, int TANG, bool ADVANCE, bool RECORD>
+__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, Dyn S, Dyn D, Stat T,
+                                                  const int* nbr_count, const int* nbr, const Tri* tris,
+                                                  const uint32_t* tri_start, const uint32_t* tri_idx,
+                                                  const float* xb, BlockStats* bstats,
+                                                  unsigned long long* pairbuf, int* paircount) {
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  unsigned long long ncont = 0, nwall = 0, njkr = 0;
-  bool flag_overlap = false, far = false;
+  unsigned int nwall = 0;
+  Acc acc;
+  acc.f = zero3();
+  acc.t = zero3();
+  acc.ncont = 0;
+  acc.njkr = 0;
+  acc.npair = 0;
+  acc.overlap = false;
+  bool far = false;
   if (i < n) {
     const uint8_t flags = T.flags[i];
+    Own o;
+    o.x = ld3(S.x, i, cap);
+    o.v = ld3(S.v, i, cap);
+    o.w = ld3(S.w, i, cap);
+    dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
+    o.r = T.r[i];
+    o.m = T.m[i];
+    const int cnt = nbr_count[i];
     if (flags & FLAG_DEAD) {
       const double qnan = __builtin_nan("");
       D.x[i] = qnan;
       D.x[i + cap] = qnan;
       D.x[i + 2 * cap] = qnan;
     } else {
-      const dv3 xi = ld3(S.x, i, cap);
-      const dv3 vi = ld3(S.v, i, cap);
-      const dv3 wi = ld3(S.w, i, cap);
-      const double ri = T.r[i];
-      const double mi = T.m[i];
-      const bool adhesive = P.normal == RP_NORMAL_JKR || P.normal == RP_NORMAL_SJKR;
-      int rci[3] = {ref_coord(xi.x, 0, P), ref_coord(xi.y, 1, P), ref_coord(xi.z, 2, P)};
-      dv3 f = zero3(), t = zero3();
-      int npair = 0;
-      const int cnt = nbr_count[i];
-      for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[k * cap + i];
-        const dv3 xj = ld3(S.x, j, cap);
-        const double rj = T.r[j];
-        dv3 d = sub(xi, xj);
-        if (P.periodic[0]) {
-          if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
-          else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
+      const int cx = fine_coord(o.x.x, 0, P, true), cy = fine_coord(o.x.y, 1, P, true),
+                cz = fine_coord(o.x.z, 2, P, true);
+      const uint32_t cell = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+      const uint32_t t0 = tri_start[cell], t1 = tri_start[cell + 1];
+      o.rc0 = ref_coord<0>(o.x.x, P);
+      o.rc1 = ref_coord<1>(o.x.y, P);
+      o.rc2 = ref_coord<2>(o.x.z, P);
+      unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
+      for (int base = 0; base < cnt; base += 4) {
+        int jj[4];
+        dv3 xq[4];
+        double rq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = jj[q] < 0 ? i : jj[q];
+          xq[q] = ld3(S.x, j, cap);
+          rq[q] = T.r[j];
         }
-        if (P.periodic[1]) {
-          if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
-          else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
+        int hit = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const dv3 d = pair_delta(P, o.x, xq[q]);
+          if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
         }
-        if (P.periodic[2]) {
-          if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
-          else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
-        }
-        const double dsq = dot(d, d);
-        const double rsum = ri + rj;
-        const double margin = adhesive ? rsum * 0.5 : 0.0;
-        const double cutoff = rsum + margin;
-        if (!(dsq < cutoff * cutoff)) continue;
-        // Visible to the reference grid, and reached by its 3x3x3 walk how many times?
-        int mult = 1, masks[3] = {0, 0, 0};
-        for (int kk = 0; kk < 3; ++kk) {
-          int cj = ref_coord(comp(xj, kk), kk, P);
-          if (cj < 0 || cj >= P.ref_dims[kk]) {
-            mult = 0;
-            break;
-          }
-          mult *= ref_matches(rci[kk], cj, kk, P, &masks[kk]);
-        }
-        if (mult == 0) continue;
-        const double dist = sqrt(dsq);
-        const double overlap = rsum - dist;
-        if (overlap > 0.2 * fmin(ri, rj)) flag_overlap = true;
-        const dv3 nrm = dist > 1e-6 ? divs(d, dist) : mk(0.0, 1.0, 0.0);
-        const dv3 ra = mul(neg(nrm), ri);
-        const dv3 rb = mul(nrm, rj);
-        const dv3 vj = ld3(S.v, j, cap);
-        const dv3 wj = ld3(S.w, j, cap);
-        const double mj = T.m[j];
-        const dv3 vrel = sub(add(vi, cross(wi, ra)), add(vj, cross(wj, rb)));
-        const double rstar = (ri * rj) / (ri + rj);
-        const double mstar = (mi * mj) / (mi + mj);
-        dv3 fn;
-        int jfail = 0;
-        switch (P.normal) {
-          case RP_NORMAL_HERTZ: fn = hertz(overlap, nrm, vrel, P.mc.pp, rstar, mstar, P.mc.beta_abs); break;
-          case RP_NORMAL_LINEAR: {
-            double kn = P.mc.pp.estar * rstar;
-            double gn = 0.5 * sqrt(mstar * kn);
-            fn = linear_sd(overlap, nrm, vrel, kn, gn);
-            break;
-          }
-          case RP_NORMAL_HYSTERETIC: fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0); break;
-          case RP_NORMAL_JKR: fn = jkr(overlap, nrm, P.mc.pp, rstar, &jfail); break;
-          default: fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar); break;
-        }
-        njkr += (unsigned long long)(jfail * mult);
-        const double fnmag = len(fn);
-        dv3 ft;
-        if (P.tangential == RP_TANGENTIAL_COULOMB) {
-          ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
-        } else {
-          // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
-          double kt = P.tangential == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
-          ft = spring_coulomb_g0(fnmag, kt, P.mc.mu);
-        }
-        const dv3 fc = add(fn, ft);
-        const dv3 tc = cross(ra, fc);
-        for (int rep = 0; rep < mult; ++rep) {
-          f = add(f, fc);
-          t = add(t, tc);
-        }
-        ncont += (unsigned long long)mult;
-        if (RECORD) {
-          const unsigned long long sj = T.sid[j];
-          for (int oz = 0; oz < 3; ++oz) {
-            if (!(masks[2] >> oz & 1)) continue;
-            for (int oy = 0; oy < 3; ++oy) {
-              if (!(masks[1] >> oy & 1)) continue;
-              for (int ox = 0; ox < 3; ++ox) {
-                if (!(masks[0] >> ox & 1)) continue;
-                unsigned long long rank = (unsigned long long)(oz * 9 + oy * 3 + ox);
-                if (npair < P.pair_cap) pairbuf[(size_t)i * P.pair_cap + npair] = (rank << 32) | sj;
-                else atomicOr(&ctl->pair_overflow, 1);
-                ++npair;
-              }
-            }
-          }
+        while (hit) {
+          const int q = __ffs(hit) - 1;
+          hit &= hit - 1;
+          contact_pp<NORMAL, TANG, RECORD>(P, S, T, o, sel4(q, jj[0], jj[1], jj[2], jj[3]), acc, ctl, prow);
         }
       }
-      if (RECORD) paircount[i] = min(npair, P.pair_cap);
-      // walls: triangles binned into this particle's (clamped) fine cell
-      if (P.ncells > 0) {
-        int cx = fine_coord(xi.x, 0, P, true), cy = fine_coord(xi.y, 1, P, true), cz = fine_coord(xi.z, 2, P, true);
-        uint32_t c = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
-        uint32_t t0 = tri_start[c], t1 = tri_start[c + 1];
-        for (uint32_t q = t0; q < t1; ++q) {
-          const Tri tr = tris[tri_idx[q]];
-          const dv3 cp = closest_point(tr, xi);
-          const dv3 dv = sub(xi, cp);
-          const double dsq = dot(dv, dv);
-          if (!(dsq < ri * ri)) continue;
-          const double dist = sqrt(dsq);
-          const double pen = ri - dist;
-          const dv3 nrm = dist > 1e-6 ? divs(dv, dist) : tr.n;
-          if (pen > 0.2 * ri) flag_overlap = true;
-          const dv3 ra = mul(neg(nrm), ri);
-          const dv3 vrel = add(vi, cross(wi, ra));
-          dv3 fn;
-          int jfail = 0;
-          switch (P.normal) {
-            case RP_NORMAL_HERTZ: fn = hertz(pen, nrm, vrel, P.mc.pw, ri, mi, P.mc.beta_abs); break;
-            case RP_NORMAL_LINEAR: {
-              double kn = P.mc.pw.estar * ri;
-              double gn = 0.5 * sqrt(mi * kn);
-              fn = linear_sd(pen, nrm, vrel, kn, gn);
-              break;
-            }
-            case RP_NORMAL_HYSTERETIC: fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * ri, 0.0); break;
-            case RP_NORMAL_JKR: fn = jkr(pen, nrm, P.mc.pw, ri, &jfail); break;
-            default: fn = sjkr(pen, nrm, vrel, P.mc.pw, ri); break;
-          }
-          njkr += (unsigned long long)jfail;
-          const double fnmag = len(fn);
-          const dv3 ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
-          const dv3 fc = add(fn, ft);
-          f = add(f, fc);
-          t = add(t, cross(ra, fc));
-          ++nwall;
-        }
-      }
+      if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
+      for (uint32_t q = t0; q < t1; ++q) contact_wall<NORMAL>(P, o, tris[tri_idx[q]], acc, nwall);
+
       // apply (simulation.rs:533-536) and second half kick (:543-561)
-      const dv3 a = mk(P.g[0] + f.x / mi, P.g[1] + f.y / mi, P.g[2] + f.z / mi);
-      const dv3 tau = mk(0.0 + t.x, 0.0 + t.y, 0.0 + t.z);
-      dv3 v = vi, w = wi;
-      dv3 rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
+      const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
+      const dv3 tau = mk(0.0 + acc.t.x, 0.0 + acc.t.y, 0.0 + acc.t.z);
+      dv3 v = o.v, w = o.w;
       if (!(flags & FLAG_FIXED)) {
         const double dt = P.dt;
         kahan1(v.x, rv.x, (0.5 * a.x) * dt);
         kahan1(v.y, rv.y, (0.5 * a.y) * dt);
         kahan1(v.z, rv.z, (0.5 * a.z) * dt);
-        const double inertia = 0.4 * mi * ri * ri;
+        const double inertia = 0.4 * o.m * o.r * o.r;
         kahan1(w.x, rw.x, (0.5 * (tau.x / inertia)) * dt);
         kahan1(w.y, rw.y, (0.5 * (tau.y / inertia)) * dt);
         kahan1(w.z, rw.z, (0.5 * (tau.z / inertia)) * dt);
       }
-      dv3 x = xi, rx = ld3(S.rx, i, cap);
+      dv3 x = o.x;
       bool alive = true;
       if (ADVANCE) {
-        if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, ri, mi, flags)) {
+        if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, o.r, o.m, flags)) {
           mark_removed(ctl, T, D, i, cap, flags);
           alive = false;
         } else {
@@ -787,37 +855,30 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, 
       }
     }
   }
-  // counters: one atomic per wave
-  unsigned long long c1 = wave_sum_u64(ncont), c2 = wave_sum_u64(nwall), c3 = wave_sum_u64(njkr);
-  bool anyo = __any(flag_overlap), anyf = __any(far);
+  // counters: per-workgroup plain stores; the rebuild request is the only
+  // atomic, issued by a wave only when one of its particles needs it
+  const unsigned int c1 = wave_sum_u32(acc.ncont), c2 = wave_sum_u32(nwall), c3 = wave_sum_u32(acc.njkr);
+  const bool anyo = __any(acc.overlap), anyf = __any(far);
+  __shared__ unsigned int red[kBlock / 64][4];
+  const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    if (c1) atomicAdd(&ctl->contacts, c1);
-    if (c2) atomicAdd(&ctl->wall_contacts, c2);
-    if (c3) atomicAdd(&ctl->jkr_failures, c3);
-    if (anyo) atomicOr(&ctl->overlap, 1);
+    red[wv][0] = c1;
+    red[wv][1] = c2;
+    red[wv][2] = c3;
+    red[wv][3] = anyo ? 1u : 0u;
     if (ADVANCE && anyf) atomicOr(&ctl->rebuild[par ^ 1], 1);
   }
-  // last workgroup: the buffer just written becomes current
   __syncthreads();
-  __shared__ bool last;
   if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(&ctl->done_force, 1u);
-    last = (prev == gridDim.x - 1);
+    BlockStats b{0u, 0u, 0u, 0u};
+    for (int q = 0; q < kBlock / 64; ++q) {
+      b.contacts += red[q][0];
+      b.wall_contacts += red[q][1];
+      b.jkr_failures += red[q][2];
+      b.overlap |= red[q][3];
+    }
+    bstats[blockIdx.x] = b;
   }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    ctl->done_force = 0;
-    ctl->cur_dyn ^= 1;
-  }
-}
-
-__global__ void k_reset_step_counters(Ctl* ctl) {
-  ctl->contacts = 0;
-  ctl->wall_contacts = 0;
-  ctl->jkr_failures = 0;
-  ctl->overlap = 0;
 }
 
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
@@ -852,6 +913,11 @@ struct rp_engine {
   // buffers
   void* d_dyn_mem[2] = {nullptr, nullptr};
   void* d_stat_mem[2] = {nullptr, nullptr};
+  Dyn dyn[2]{};       // dyn[cur] holds the state; the force kernel writes dyn[cur ^ 1]
+  Stat stat[2]{};     // stat[0] holds the constants; stat[1] is rebuild scratch
+  int cur = 0;
+  BlockStats* d_bstats = nullptr;
+  int force_grid = 0;
   float* d_xb = nullptr;
   uint32_t* d_key = nullptr;
   uint32_t* d_tmp = nullptr;
@@ -1141,6 +1207,8 @@ void free_particles(rp_engine* e) {
   free_ptr(e->d_nbr);
   free_ptr(e->d_pairbuf);
   free_ptr(e->d_paircount);
+  free_ptr(e->d_bstats);
+  e->d_bstats = nullptr;
   e->d_xb = nullptr;
   e->d_key = e->d_tmp = nullptr;
   e->d_nbr_count = e->d_nbr = nullptr;
@@ -1187,7 +1255,10 @@ rp_status launch_rebuild(rp_engine* e, int par) {
   int m = (int)K.ncells + 1;  // buckets incl. removed sentinel
   int nb = (m + kScanChunk - 1) / kScanChunk;
   int gs = std::min(nb, kMaxGrid);
-  hipLaunchKernelGGL(k_keys, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_key, e->d_cell_count);
+  const Dyn W = e->dyn[e->cur], D = e->dyn[e->cur ^ 1];
+  const Stat V = e->stat[0], U = e->stat[1];
+  hipLaunchKernelGGL(k_keys, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, W, V, e->d_key,
+                     e->d_cell_count);
   hipLaunchKernelGGL(k_scan_sums, dim3(gs), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_cell_count, m,
                      e->d_block_sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, e->stream, e->d_ctl, par, e->d_block_sums, nb);
@@ -1195,9 +1266,9 @@ rp_status launch_rebuild(rp_engine* e, int par) {
                      e->d_block_sums, e->d_cell_start);
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_key, e->d_cell_start,
                      e->d_cell_count, e->d_tmp);
-  hipLaunchKernelGGL(k_rank_permute, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_key,
+  hipLaunchKernelGGL(k_rank_permute, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, W, V, D, U, e->d_key,
                      e->d_cell_start, e->d_tmp);
-  hipLaunchKernelGGL(k_build, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, e->d_cell_start,
+  hipLaunchKernelGGL(k_build, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D, U, W, V, e->d_cell_start,
                      e->d_nbr_count, e->d_nbr, e->d_xb);
   HIP_TRY(hipGetLastError());
   return RP_OK;
@@ -1286,12 +1357,36 @@ void mark_end(rp_engine* e) {
   (void)hipEventRecord(e->ev_pool[e->ev_marks.back().second + 1], e->stream);
 }
 
+template <int NORMAL, int TANG, bool ADV, bool REC>
+void launch_force_t(rp_engine* e, int par) {
+  int g = (int)((e->cap + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp,
+                     e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
+                     e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount);
+  e->force_grid = g;
+  e->cur ^= 1;
+}
+
+// Tangential kinds compiled: Coulomb, and the spring law (LSC / Mindlin) with
+// the placeholder shear modulus.
+template <int NORMAL, bool ADV, bool REC>
+void launch_force_n(rp_engine* e, int par) {
+  switch (e->kp.tangential) {
+    case RP_TANGENTIAL_COULOMB: launch_force_t<NORMAL, RP_TANGENTIAL_COULOMB, ADV, REC>(e, par); break;
+    case RP_TANGENTIAL_LSC: launch_force_t<NORMAL, RP_TANGENTIAL_LSC, ADV, REC>(e, par); break;
+    default: launch_force_t<NORMAL, RP_TANGENTIAL_MINDLIN, ADV, REC>(e, par); break;
+  }
+}
+
 template <bool ADV, bool REC>
 void launch_force(rp_engine* e, int par) {
-  int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp,
-                     e->d_nbr_count, e->d_nbr, e->d_tris, e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_pairbuf,
-                     e->d_paircount);
+  switch (e->kp.normal) {
+    case RP_NORMAL_LINEAR: launch_force_n<RP_NORMAL_LINEAR, ADV, REC>(e, par); break;
+    case RP_NORMAL_HERTZ: launch_force_n<RP_NORMAL_HERTZ, ADV, REC>(e, par); break;
+    case RP_NORMAL_HYSTERETIC: launch_force_n<RP_NORMAL_HYSTERETIC, ADV, REC>(e, par); break;
+    case RP_NORMAL_JKR: launch_force_n<RP_NORMAL_JKR, ADV, REC>(e, par); break;
+    default: launch_force_n<RP_NORMAL_SJKR, ADV, REC>(e, par); break;
+  }
 }
 
 }  // namespace
@@ -1383,6 +1478,8 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   HIP_TRY(hipMalloc(&e->d_key, (size_t)cap * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_tmp, (size_t)cap * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * sizeof(int)));
+  HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
+  HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
   rp_status st = alloc_nbr(e, 32);
   if (st) return st;
   // host staging in plane layout
@@ -1420,13 +1517,13 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
                     hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.flags, fl.data(), (size_t)cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.sid, sid.data(), (size_t)cap * sizeof(uint32_t), hipMemcpyHostToDevice));
+  e->dyn[0] = d0;
+  e->dyn[1] = carve_dyn(e->d_dyn_mem[1], cap);
+  e->stat[0] = s0;
+  e->stat[1] = carve_stat(e->d_stat_mem[1], cap);
+  e->cur = 0;
+  e->force_grid = 0;
   Ctl c{};
-  c.dyn[0] = d0;
-  c.dyn[1] = carve_dyn(e->d_dyn_mem[1], cap);
-  c.stat[0] = s0;
-  c.stat[1] = carve_stat(e->d_stat_mem[1], cap);
-  c.cur_dyn = 0;
-  c.cur_stat = 0;
   c.n = (int)n;
   c.alive = (int)n;
   HIP_TRY(hipMemcpy(e->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
@@ -1479,7 +1576,8 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   int par = (int)(e->step_count & 1);
   mark_begin(e, 2);
-  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp, e->d_xb);
+  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp, e->dyn[e->cur],
+                     e->stat[0], e->d_xb);
   mark_end(e);
   for (uint64_t s = 0; s < nsteps; ++s) {
     par = (int)(e->step_count & 1);
@@ -1487,7 +1585,6 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
     if ((st = launch_rebuild(e, par))) return st;
     mark_end(e);
     bool last = (s + 1 == nsteps);
-    if (last) hipLaunchKernelGGL(k_reset_step_counters, dim3(1), dim3(1), 0, e->stream, e->d_ctl);
     mark_begin(e, 0);
     if (last) {
       if (e->record) launch_force<false, true>(e, par);
@@ -1526,8 +1623,8 @@ namespace {
 rp_status slot_order(rp_engine* e, std::vector<int>& order, Dyn& cur, Stat& cst, int& nslots) {
   rp_status st = read_ctl(e);
   if (st) return st;
-  cur = e->host_ctl.dyn[e->host_ctl.cur_dyn];
-  cst = e->host_ctl.stat[e->host_ctl.cur_stat];
+  cur = e->dyn[e->cur];
+  cst = e->stat[0];
   nslots = e->host_ctl.n;
   std::vector<uint32_t> sid((size_t)nslots);
   std::vector<uint8_t> fl((size_t)nslots);
@@ -1657,13 +1754,19 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
   const Ctl& c = e->host_ctl;
   out->steps = e->step_count;
   out->rebuilds = c.rebuilds;
-  out->contacts = c.contacts;
-  out->wall_contacts = c.wall_contacts;
   out->removed = (uint64_t)(e->n_uploaded - c.alive);
-  out->excessive_overlap = c.overlap;
   out->neighbour_capacity = e->nbr_cap;
-  out->jkr_failures = c.jkr_failures;
   out->max_neighbours = c.max_count;
+  if (e->force_grid > 0) {
+    std::vector<BlockStats> bs((size_t)e->force_grid);
+    HIP_TRY(hipMemcpy(bs.data(), e->d_bstats, bs.size() * sizeof(BlockStats), hipMemcpyDeviceToHost));
+    for (const BlockStats& b : bs) {
+      out->contacts += b.contacts;
+      out->wall_contacts += b.wall_contacts;
+      out->jkr_failures += b.jkr_failures;
+      out->excessive_overlap |= (int32_t)b.overlap;
+    }
+  }
   return RP_OK;
 }
 
