@@ -78,7 +78,7 @@ def cpu_baseline(p, x, r, m, meshes, steps: int, threads: int, cell: float):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="hopper")
     ap.add_argument("--n", type=int, default=None)

@@ -65,6 +65,7 @@ constexpr uint8_t FLAG_DEAD = 2;
 constexpr int kScanItems = 16;                   // items per thread in the scan
 constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
 constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
+constexpr uint64_t kRebuildInterval = 10;        // steps between launches of the rebuild chain
 constexpr double kPi = 3.14159265358979323846;
 
 // Per-step particle state of one buffer (device pointers, `cap` slots).
@@ -101,6 +102,7 @@ struct Ctl {
   int max_count;           // largest candidate count seen by k_build
   unsigned long long rebuilds;  // rebuilds done
   int pair_overflow;       // pair record overflow
+  int violation;           // a particle moved more than skin/2 since the last list build
 };
 
 // Per-workgroup counters of one force launch.
@@ -133,6 +135,8 @@ struct KParams {
   unsigned int ncells;
   double skin;
   double half_skin;
+  double lookahead_steps;  // steps between rebuild checks (see moved_too_far)
+  double safe_half_skin;   // skin/2 minus the float rounding of the stored build positions
   double r_max;
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
@@ -535,11 +539,24 @@ __device__ inline double mic_disp(double d, const KParams& P) {
   return d;
 }
 
-__device__ inline bool moved_too_far(const KParams& P, dv3 x, const float* xb, int i) {
+// The Verlet list stays a superset of the contact pairs while every particle
+// is within skin/2 of its position at the list build.  The host launches the
+// rebuild chain only every `lookahead_steps` steps, so a particle asks for a
+// rebuild once its displacement plus a bound on its motion over that many
+// steps (2 K |v| dt + K^2 |a| dt^2) reaches skin/2.  Passing skin/2 itself
+// sets the violation flag, which the host reports as an error.
+__device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, const float* xb, int i, Ctl* ctl) {
   const double d0 = mic_disp<0>(x.x - (double)xb[i], P);
   const double d1 = mic_disp<1>(x.y - (double)xb[i + P.cap], P);
   const double d2 = mic_disp<2>(x.z - (double)xb[i + 2 * P.cap], P);
-  return !(d0 * d0 + d1 * d1 + d2 * d2 < P.half_skin * P.half_skin);
+  const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+  if (!(dd < P.safe_half_skin * P.safe_half_skin)) {
+    ctl->violation = 1;
+    return true;
+  }
+  const double k = P.lookahead_steps, dt = P.dt;
+  const double ahead = 2.0 * k * sqrt(dot(v, v)) * dt + k * k * sqrt(dot(a, a)) * dt * dt;
+  return !(sqrt(dd) + ahead < P.safe_half_skin);
 }
 
 // A removed particle keeps slot i until the next rebuild drops it; its
@@ -556,7 +573,9 @@ __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i
 
 // First half of a step from the at-rest state (used once at the start of
 // each rp_engine_step call).
-__global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, Dyn S, Stat T, const float* xb) {
+// flag_slot: the rebuild-request slot read by the next rebuild chain.
+__global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KParams P, Dyn S, Stat T,
+                                                  const float* xb) {
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,11 +595,11 @@ __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int par, KParams P, 
         st3(S.rx, i, cap, rx);
         st3(S.rv, i, cap, rv);
         st3(S.rw, i, cap, rw);
-        far = moved_too_far(P, x, xb, i);
+        far = moved_too_far(P, x, v, a, xb, i, ctl);
       }
     }
   }
-  if (__any(far) && (threadIdx.x & 63) == 0) atomicOr(&ctl->rebuild[par], 1);
+  if (__any(far) && (threadIdx.x & 63) == 0) atomicOr(&ctl->rebuild[flag_slot], 1);
 }
 
 // Pair test of slot i against slot j (simulation.rs:323-357): minimum image
@@ -750,7 +769,7 @@ __device__ inline int sel4(int q, int a, int b, int c, int d) { return q == 0 ? 
 // the state drifted; otherwise stores a, tau and leaves the at-rest state.
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
 template <int NORMAL, int TANG, bool ADVANCE, bool RECORD>
-__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, Dyn S, Dyn D, Stat T,
+__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KParams P, Dyn S, Dyn D, Stat T,
                                                   const int* nbr_count, const int* nbr, const Tri* tris,
                                                   const uint32_t* tri_start, const uint32_t* tri_idx,
                                                   const float* xb, BlockStats* bstats,
@@ -839,7 +858,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, 
           mark_removed(ctl, T, D, i, cap, flags);
           alive = false;
         } else {
-          far = moved_too_far(P, x, xb, i);
+          far = moved_too_far(P, x, v, a, xb, i, ctl);
         }
       } else {
         st3(D.a, i, cap, a);
@@ -866,7 +885,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int par, KParams P, 
     red[wv][1] = c2;
     red[wv][2] = c3;
     red[wv][3] = anyo ? 1u : 0u;
-    if (ADVANCE && anyf) atomicOr(&ctl->rebuild[par ^ 1], 1);
+    if (ADVANCE && anyf) atomicOr(&ctl->rebuild[flag_slot], 1);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -903,6 +922,7 @@ struct rp_engine {
   int cap = 0;         // slot capacity
   int n_uploaded = 0;
   double r_max = 0.0, r_mean = 0.0;
+  double v_max = 0.0;  // fastest uploaded particle (sizes the automatic skin)
   uint64_t step_count = 0;
   bool have_particles = false;
   bool geometry_dirty = true;
@@ -916,6 +936,7 @@ struct rp_engine {
   Dyn dyn[2]{};       // dyn[cur] holds the state; the force kernel writes dyn[cur ^ 1]
   Stat stat[2]{};     // stat[0] holds the constants; stat[1] is rebuild scratch
   int cur = 0;
+  uint64_t chain_no = 0;  // rebuild chains launched; chain c reads request slot c & 1
   BlockStats* d_bstats = nullptr;
   int force_grid = 0;
   float* d_xb = nullptr;
@@ -1024,8 +1045,11 @@ rp_status compute_geometry(rp_engine* e) {
   double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
   double rmean = e->r_mean > 0.0 ? e->r_mean : rmax;
   K.r_max = rmax;
-  K.skin = p.skin > 0.0 ? p.skin : 0.1 * rmax;
+  // automatic skin: a tenth of the largest radius, and wide enough that the
+  // fastest uploaded particle cannot cover half of it between rebuild checks
+  K.skin = p.skin > 0.0 ? p.skin : std::max(0.1 * rmax, 4.0 * (double)kRebuildInterval * e->v_max * p.dt);
   K.half_skin = 0.5 * K.skin;
+  K.lookahead_steps = (double)kRebuildInterval;
   double cut_max = K.margin_factor * 2.0 * rmax + K.skin;
   double cs = p.cell_size > 0.0 ? p.cell_size : (rmax > 2.0 * rmean ? 0.5 * cut_max : cut_max);
   // non-periodic extent: domain bounds plus mesh bounding box, one cell of margin
@@ -1074,6 +1098,11 @@ rp_status compute_geometry(rp_engine* e) {
     stride *= K.gn[order[q]];
   }
   K.ncells = (unsigned int)stride;
+  // build positions are stored as float: |x - float(x)| <= |x| 2^-24 per axis
+  double xmax = 0.0;
+  for (int k = 0; k < 3; ++k)
+    xmax = std::max(xmax, std::max(std::fabs(K.go[k]), std::fabs(K.go[k] + K.gn[k] / K.ginv[k])));
+  K.safe_half_skin = K.half_skin - 2.0 * std::sqrt(3.0) * xmax * std::ldexp(1.0, -24);
   K.cap = e->cap;
   K.nbr_cap = e->nbr_cap;
   K.pair_cap = e->pair_cap;
@@ -1249,7 +1278,12 @@ int grid_for(size_t n) {
   return (int)g;
 }
 
-rp_status launch_rebuild(rp_engine* e, int par) {
+// Launches rebuild chain number e->chain_no; it acts only if its request
+// slot (chain_no & 1) is set, and it clears the other slot for the kernels
+// that follow it.
+rp_status launch_rebuild(rp_engine* e) {
+  const int par = (int)(e->chain_no & 1);
+  e->chain_no++;
   const KParams& K = e->kp;
   int g = grid_for((size_t)e->cap);
   int m = (int)K.ncells + 1;  // buckets incl. removed sentinel
@@ -1277,6 +1311,10 @@ rp_status launch_rebuild(rp_engine* e, int par) {
 rp_status read_ctl(rp_engine* e) {
   HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->host_ctl.violation)
+    return fail(RP_ERR_STATE,
+                "neighbour list invalidated: a particle moved more than half the Verlet skin between rebuild "
+                "checks; raise rp_params.skin");
   return RP_OK;
 }
 
@@ -1289,11 +1327,10 @@ rp_status prepare(rp_engine* e) {
   if (st) return st;
   if ((st = alloc_cells(e))) return st;
   if ((st = upload_meshes_binned(e))) return st;
-  int par = (int)(e->step_count & 1);
   // trial rebuild to size the neighbour lists; repeat while it overflows
   for (int attempt = 0; attempt < 8; ++attempt) {
-    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, par);
-    if ((st = launch_rebuild(e, par))) return st;
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1));
+    if ((st = launch_rebuild(e))) return st;
     if ((st = read_ctl(e))) return st;
     if (!e->host_ctl.overflow) break;
     int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 8), 8);
@@ -1307,8 +1344,8 @@ rp_status prepare(rp_engine* e) {
   int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 16), 8);
   if (want > e->nbr_cap) {
     if ((st = alloc_nbr(e, want))) return st;
-    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, par);
-    if ((st = launch_rebuild(e, par))) return st;
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1));
+    if ((st = launch_rebuild(e))) return st;
   }
   // the list is valid now; a later k_begin decides about the next rebuild
   int clear[2] = {0, 0};
@@ -1358,9 +1395,9 @@ void mark_end(rp_engine* e) {
 }
 
 template <int NORMAL, int TANG, bool ADV, bool REC>
-void launch_force_t(rp_engine* e, int par) {
+void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp,
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot, e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount);
   e->force_grid = g;
@@ -1370,22 +1407,22 @@ void launch_force_t(rp_engine* e, int par) {
 // Tangential kinds compiled: Coulomb, and the spring law (LSC / Mindlin) with
 // the placeholder shear modulus.
 template <int NORMAL, bool ADV, bool REC>
-void launch_force_n(rp_engine* e, int par) {
+void launch_force_n(rp_engine* e, int slot) {
   switch (e->kp.tangential) {
-    case RP_TANGENTIAL_COULOMB: launch_force_t<NORMAL, RP_TANGENTIAL_COULOMB, ADV, REC>(e, par); break;
-    case RP_TANGENTIAL_LSC: launch_force_t<NORMAL, RP_TANGENTIAL_LSC, ADV, REC>(e, par); break;
-    default: launch_force_t<NORMAL, RP_TANGENTIAL_MINDLIN, ADV, REC>(e, par); break;
+    case RP_TANGENTIAL_COULOMB: launch_force_t<NORMAL, RP_TANGENTIAL_COULOMB, ADV, REC>(e, slot); break;
+    case RP_TANGENTIAL_LSC: launch_force_t<NORMAL, RP_TANGENTIAL_LSC, ADV, REC>(e, slot); break;
+    default: launch_force_t<NORMAL, RP_TANGENTIAL_MINDLIN, ADV, REC>(e, slot); break;
   }
 }
 
 template <bool ADV, bool REC>
-void launch_force(rp_engine* e, int par) {
+void launch_force(rp_engine* e, int slot) {
   switch (e->kp.normal) {
-    case RP_NORMAL_LINEAR: launch_force_n<RP_NORMAL_LINEAR, ADV, REC>(e, par); break;
-    case RP_NORMAL_HERTZ: launch_force_n<RP_NORMAL_HERTZ, ADV, REC>(e, par); break;
-    case RP_NORMAL_HYSTERETIC: launch_force_n<RP_NORMAL_HYSTERETIC, ADV, REC>(e, par); break;
-    case RP_NORMAL_JKR: launch_force_n<RP_NORMAL_JKR, ADV, REC>(e, par); break;
-    default: launch_force_n<RP_NORMAL_SJKR, ADV, REC>(e, par); break;
+    case RP_NORMAL_LINEAR: launch_force_n<RP_NORMAL_LINEAR, ADV, REC>(e, slot); break;
+    case RP_NORMAL_HERTZ: launch_force_n<RP_NORMAL_HERTZ, ADV, REC>(e, slot); break;
+    case RP_NORMAL_HYSTERETIC: launch_force_n<RP_NORMAL_HYSTERETIC, ADV, REC>(e, slot); break;
+    case RP_NORMAL_JKR: launch_force_n<RP_NORMAL_JKR, ADV, REC>(e, slot); break;
+    default: launch_force_n<RP_NORMAL_SJKR, ADV, REC>(e, slot); break;
   }
 }
 
@@ -1497,7 +1534,13 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   put3(5, rw3);
   put3(6, a3);
   put3(7, tau3);
-  double rsum = 0.0, rmax = 0.0;
+  double rsum = 0.0, rmax = 0.0, vmax2 = 0.0;
+  if (v3)
+    for (size_t i = 0; i < n; ++i) {
+      double q = v3[3 * i] * v3[3 * i] + v3[3 * i + 1] * v3[3 * i + 1] + v3[3 * i + 2] * v3[3 * i + 2];
+      if (q > vmax2) vmax2 = q;
+    }
+  e->v_max = std::sqrt(vmax2);
   for (size_t i = 0; i < n; ++i) {
     planes[(size_t)24 * cap + i] = radius[i];
     planes[(size_t)25 * cap + i] = mass[i];
@@ -1574,23 +1617,26 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (st) return st;
   if (e->record && (st = ensure_pairs(e))) return st;
   int gcap = (int)((e->cap + kBlock - 1) / kBlock);
-  int par = (int)(e->step_count & 1);
   mark_begin(e, 2);
-  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->kp, e->dyn[e->cur],
-                     e->stat[0], e->d_xb);
+  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1), e->kp,
+                     e->dyn[e->cur], e->stat[0], e->d_xb);
   mark_end(e);
   for (uint64_t s = 0; s < nsteps; ++s) {
-    par = (int)(e->step_count & 1);
-    mark_begin(e, 1);
-    if ((st = launch_rebuild(e, par))) return st;
-    mark_end(e);
-    bool last = (s + 1 == nsteps);
+    // The rebuild chain runs at the first step of a call and every
+    // kRebuildInterval steps; the force kernels' look-ahead covers the gap.
+    if (s == 0 || e->step_count % kRebuildInterval == 0) {
+      mark_begin(e, 1);
+      if ((st = launch_rebuild(e))) return st;
+      mark_end(e);
+    }
+    const int slot = (int)(e->chain_no & 1);
+    const bool last = (s + 1 == nsteps);
     mark_begin(e, 0);
     if (last) {
-      if (e->record) launch_force<false, true>(e, par);
-      else launch_force<false, false>(e, par);
+      if (e->record) launch_force<false, true>(e, slot);
+      else launch_force<false, false>(e, slot);
     } else {
-      launch_force<true, false>(e, par);
+      launch_force<true, false>(e, slot);
     }
     mark_end(e);
     e->step_count++;
