@@ -87,8 +87,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run one 256 MiB device-to-device copy (torch) so a counter profile of this "
-                         "run can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
+                    help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
+                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not part of "
@@ -138,14 +138,9 @@ def main() -> int:
     log(f"force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
         f"rebuild chain avg {tm['rebuild'][0] / max(tm['rebuild'][1], 1) * 1e3:.1f} us")
     if args.calibrate_copy:
-        import torch
-        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 268435456 bytes read + 268435456 bytes written")
+        ms = native.probe_copy(256 * 1024 * 1024)
+        log(f"copy calibration: k_copy_probe read 268435456 B + wrote 268435456 B in {ms:.3f} ms "
+            f"({2 * 268435456 / ms / 1e6:.0f} GB/s)")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{args.workload}_force_traffic.json")
     if os.path.exists(tfile):

@@ -218,6 +218,12 @@ rp_status rp_engine_set_timing(rp_engine* e, int on);
  * 1 = rebuild chain (including its no-op launches), 2 = first half kernel. */
 rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3);
 
+/* Diagnostics, no counterpart in the reference: copies `bytes` bytes
+ * between two fresh device buffers with a plain streaming kernel and returns
+ * its time.  A counter profile of the same process uses it to scale
+ * FETCH_SIZE / WRITE_SIZE by a kernel of known traffic. */
+rp_status rp_probe_copy(size_t bytes, double* ms);
+
 /* Counters (rp_stats). */
 rp_status rp_engine_stats(rp_engine* e, rp_stats* out);
 

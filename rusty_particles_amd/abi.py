@@ -125,6 +125,7 @@ EXPORTS = (
     "rp_engine_stats",
     "rp_engine_set_timing",
     "rp_engine_timing",
+    "rp_probe_copy",
     "rp_engine_destroy",
     "rp_last_error",
 )

@@ -22,10 +22,12 @@
 //                   state makes one round trip through HBM per step.
 //
 // Memory layout (HBM): structure of arrays, FP64, each vec3 field stored as
-// three planes of `cap` doubles; two copies of the persistent state (x, v, w
-// and their Kahan residuals, radius, mass, flags, stable id) swap at each
-// rebuild; a, tau and the neighbour lists (column-major: entry k of slot i
-// at nbr[k*cap + i]) are single copies.  No float atomics anywhere: results
+// three planes of `cap` doubles, except position + radius, interleaved as one
+// 32-byte record per slot.  Two copies of the per-step state (x, v, w, their
+// Kahan residuals, a, tau): the force kernel reads one and writes the other.
+// Radius, mass, flags and stable id are one copy plus a scratch copy used
+// while a rebuild re-orders the particles.  Neighbour lists are column-major
+// (entry k of slot i at nbr[k*cap + i]).  No float atomics anywhere: results
 // are bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
 
@@ -66,13 +68,15 @@ constexpr int kScanItems = 16;                   // items per thread in the scan
 constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
 constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
 constexpr uint64_t kRebuildInterval = 10;        // steps between launches of the rebuild chain
+constexpr int kNbrBatch = 2;                     // neighbours whose loads the force kernel issues together
 constexpr double kPi = 3.14159265358979323846;
 
 // Per-step particle state of one buffer (device pointers, `cap` slots).
 // The force kernel reads one buffer and writes the other, so no thread can
 // see a neighbour's next-step values while gathering this step's forces.
 struct Dyn {
-  double* x;   // 3 planes
+  double* xr;  // position and radius, interleaved (x, y, z, r) per slot: a
+               // neighbour test touches one 32-byte record
   double* v;
   double* w;
   double* rx;  // Kahan residuals of x, v, w
@@ -146,6 +150,23 @@ struct KParams {
 // ---- small device helpers ---------------------------------------------------
 
 __device__ inline dv3 ld3(const double* p, int i, int cap) { return mk(p[i], p[i + cap], p[i + 2 * cap]); }
+__device__ inline void ldxr(const double* xr, int i, dv3& x, double& r) {
+  const double4 q = reinterpret_cast<const double4*>(xr)[i];
+  x = mk(q.x, q.y, q.z);
+  r = q.w;
+}
+__device__ inline dv3 ldx(const double* xr, int i) {
+  const double4 q = reinterpret_cast<const double4*>(xr)[i];
+  return mk(q.x, q.y, q.z);
+}
+__device__ inline void stxr(double* xr, int i, dv3 x, double r) {
+  double4 q;
+  q.x = x.x;
+  q.y = x.y;
+  q.z = x.z;
+  q.w = r;
+  reinterpret_cast<double4*>(xr)[i] = q;
+}
 __device__ inline void st3(double* p, int i, int cap, dv3 v) {
   p[i] = v.x;
   p[i + cap] = v.y;
@@ -234,9 +255,10 @@ __global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, D
     if (T.flags[i] & FLAG_DEAD) {
       c = P.ncells;
     } else {
-      int cx = fine_coord(S.x[i], 0, P, false);
-      int cy = fine_coord(S.x[i + cap], 1, P, false);
-      int cz = fine_coord(S.x[i + 2 * cap], 2, P, false);
+      const dv3 x = ldx(S.xr, i);
+      int cx = fine_coord(x.x, 0, P, false);
+      int cy = fine_coord(x.y, 1, P, false);
+      int cz = fine_coord(x.z, 2, P, false);
       c = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
     }
     key[i] = c;
@@ -358,7 +380,6 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
     int d = (int)(s0 + rank);
     for (int k = 0; k < 3; ++k) {
       int o = k * cap;
-      D.x[d + o] = S.x[i + o];
       D.v[d + o] = S.v[i + o];
       D.w[d + o] = S.w[i + o];
       D.rx[d + o] = S.rx[i + o];
@@ -367,6 +388,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
       D.a[d + o] = S.a[i + o];
       D.tau[d + o] = S.tau[i + o];
     }
+    reinterpret_cast<double4*>(D.xr)[d] = reinterpret_cast<const double4*>(S.xr)[i];
     U.r[d] = T.r[i];
     U.m[d] = T.m[i];
     U.flags[d] = T.flags[i];
@@ -385,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
   const int cap = P.cap;
   int local_max = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    dv3 xi = ld3(D.x, i, cap);
+    const dv3 xi = ldx(D.xr, i);
     double ri = U.r[i];
     int ci[3] = {fine_coord(xi.x, 0, P, false), fine_coord(xi.y, 1, P, false), fine_coord(xi.z, 2, P, false)};
     double lim_max = P.margin_factor * (ri + P.r_max) + P.skin;
@@ -418,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
           uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
           for (uint32_t j = s0; j < s1; ++j) {
             if ((int)j == i) continue;
-            dv3 xj = ld3(D.x, (int)j, cap);
+            const dv3 xj = ldx(D.xr, (int)j);
             dv3 d = sub(xi, xj);
             if (P.periodic[0]) {
               if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
@@ -463,7 +485,6 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
     xb[i + 2 * cap] = (float)xi.z;
     for (int k = 0; k < 3; ++k) {
       const int o = i + k * cap;
-      W.x[o] = D.x[o];
       W.v[o] = D.v[o];
       W.w[o] = D.w[o];
       W.rx[o] = D.rx[o];
@@ -472,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
       W.a[o] = D.a[o];
       W.tau[o] = D.tau[o];
     }
+    reinterpret_cast<double4*>(W.xr)[i] = reinterpret_cast<const double4*>(D.xr)[i];
     V.r[i] = U.r[i];
     V.m[i] = U.m[i];
     V.flags[i] = U.flags[i];
@@ -493,8 +515,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
 // Start of step s+1 for one particle, from its at-rest state after step s:
 // periodic wrap (simulation.rs:182-197), removal test (:200-214), first half
 // kick, drift and angular half kick (:235-263).  Returns false when removed.
+// alpha = tau / I with I = 0.4 m r^2 (simulation.rs:253-254), computed by the
+// caller (the second half kick of the previous step uses the same value).
 __device__ inline bool begin_step(const KParams& P, dv3& x, dv3& v, dv3& w, dv3& rx, dv3& rv, dv3& rw, dv3 a,
-                                  dv3 tau, double r, double m, uint8_t flags) {
+                                  dv3 alpha, uint8_t flags) {
   if (P.any_periodic) {
     if (P.periodic[0]) {
       if (x.x < P.bmin[0]) x.x += P.bsize[0];
@@ -521,12 +545,16 @@ __device__ inline bool begin_step(const KParams& P, dv3& x, dv3& v, dv3& w, dv3&
     kahan1(x.x, rx.x, v.x * dt);
     kahan1(x.y, rx.y, v.y * dt);
     kahan1(x.z, rx.z, v.z * dt);
-    double inertia = 0.4 * m * r * r;
-    kahan1(w.x, rw.x, (0.5 * (tau.x / inertia)) * dt);
-    kahan1(w.y, rw.y, (0.5 * (tau.y / inertia)) * dt);
-    kahan1(w.z, rw.z, (0.5 * (tau.z / inertia)) * dt);
+    kahan1(w.x, rw.x, (0.5 * alpha.x) * dt);
+    kahan1(w.y, rw.y, (0.5 * alpha.y) * dt);
+    kahan1(w.z, rw.z, (0.5 * alpha.z) * dt);
   }
   return true;
+}
+
+__device__ inline dv3 angular_acc(dv3 tau, double r, double m) {
+  const double inertia = 0.4 * m * r * r;
+  return mk(tau.x / inertia, tau.y / inertia, tau.z / inertia);
 }
 
 // Displacement since the last list build, minimum image on periodic axes.
@@ -565,9 +593,7 @@ __device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, cons
 __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i, int cap, uint8_t flags) {
   T.flags[i] = flags | FLAG_DEAD;
   const double qnan = __builtin_nan("");
-  D.x[i] = qnan;
-  D.x[i + cap] = qnan;
-  D.x[i + 2 * cap] = qnan;
+  stxr(D.xr, i, mk(qnan, qnan, qnan), qnan);
   atomicSub(&ctl->alive, 1);
 }
 
@@ -583,13 +609,16 @@ __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KPara
   if (i < n) {
     uint8_t flags = T.flags[i];
     if (!(flags & FLAG_DEAD)) {
-      dv3 x = ld3(S.x, i, cap), v = ld3(S.v, i, cap), w = ld3(S.w, i, cap);
+      dv3 x;
+      double r;
+      ldxr(S.xr, i, x, r);
+      dv3 v = ld3(S.v, i, cap), w = ld3(S.w, i, cap);
       dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
       dv3 a = ld3(S.a, i, cap), tau = ld3(S.tau, i, cap);
-      if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, T.r[i], T.m[i], flags)) {
+      if (!begin_step(P, x, v, w, rx, rv, rw, a, angular_acc(tau, T.r[i], T.m[i]), flags)) {
         mark_removed(ctl, T, S, i, cap, flags);
       } else {
-        st3(S.x, i, cap, x);
+        stxr(S.xr, i, x, r);
         st3(S.v, i, cap, v);
         st3(S.w, i, cap, w);
         st3(S.rx, i, cap, rx);
@@ -646,8 +675,9 @@ struct Acc {
 template <int NORMAL, int TANG, bool RECORD>
 __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T, const Own& o, int j, Acc& acc,
                                   Ctl* ctl, unsigned long long* prow) {
-  const dv3 xj = ld3(S.x, j, P.cap);
-  const double rj = T.r[j];
+  dv3 xj;
+  double rj;
+  ldxr(S.xr, j, xj, rj);
   const dv3 d = pair_delta(P, o.x, xj);
   const double dsq = dot(d, d);
   const double rsum = o.r + rj;
@@ -789,18 +819,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
   if (i < n) {
     const uint8_t flags = T.flags[i];
     Own o;
-    o.x = ld3(S.x, i, cap);
+    ldxr(S.xr, i, o.x, o.r);
     o.v = ld3(S.v, i, cap);
     o.w = ld3(S.w, i, cap);
-    dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
-    o.r = T.r[i];
     o.m = T.m[i];
     const int cnt = nbr_count[i];
     if (flags & FLAG_DEAD) {
       const double qnan = __builtin_nan("");
-      D.x[i] = qnan;
-      D.x[i + cap] = qnan;
-      D.x[i + 2 * cap] = qnan;
+      stxr(D.xr, i, mk(qnan, qnan, qnan), qnan);
     } else {
       const int cx = fine_coord(o.x.x, 0, P, true), cy = fine_coord(o.x.y, 1, P, true),
                 cz = fine_coord(o.x.z, 2, P, true);
@@ -810,28 +836,29 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
       o.rc1 = ref_coord<1>(o.x.y, P);
       o.rc2 = ref_coord<2>(o.x.z, P);
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
-      for (int base = 0; base < cnt; base += 4) {
-        int jj[4];
-        dv3 xq[4];
-        double rq[4];
+      for (int base = 0; base < cnt; base += kNbrBatch) {
+        int jj[kNbrBatch];
+        dv3 xq[kNbrBatch];
+        double rq[kNbrBatch];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
+        for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kNbrBatch; ++q) {
           const int j = jj[q] < 0 ? i : jj[q];
-          xq[q] = ld3(S.x, j, cap);
-          rq[q] = T.r[j];
+          ldxr(S.xr, j, xq[q], rq[q]);
         }
         int hit = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kNbrBatch; ++q) {
           const dv3 d = pair_delta(P, o.x, xq[q]);
           if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
         }
         while (hit) {
           const int q = __ffs(hit) - 1;
           hit &= hit - 1;
-          contact_pp<NORMAL, TANG, RECORD>(P, S, T, o, sel4(q, jj[0], jj[1], jj[2], jj[3]), acc, ctl, prow);
+          const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
+                                       : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
+          contact_pp<NORMAL, TANG, RECORD>(P, S, T, o, j, acc, ctl, prow);
         }
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
@@ -840,21 +867,22 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
       const dv3 tau = mk(0.0 + acc.t.x, 0.0 + acc.t.y, 0.0 + acc.t.z);
+      dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
       dv3 v = o.v, w = o.w;
+      const dv3 alpha = angular_acc(tau, o.r, o.m);
       if (!(flags & FLAG_FIXED)) {
         const double dt = P.dt;
         kahan1(v.x, rv.x, (0.5 * a.x) * dt);
         kahan1(v.y, rv.y, (0.5 * a.y) * dt);
         kahan1(v.z, rv.z, (0.5 * a.z) * dt);
-        const double inertia = 0.4 * o.m * o.r * o.r;
-        kahan1(w.x, rw.x, (0.5 * (tau.x / inertia)) * dt);
-        kahan1(w.y, rw.y, (0.5 * (tau.y / inertia)) * dt);
-        kahan1(w.z, rw.z, (0.5 * (tau.z / inertia)) * dt);
+        kahan1(w.x, rw.x, (0.5 * alpha.x) * dt);
+        kahan1(w.y, rw.y, (0.5 * alpha.y) * dt);
+        kahan1(w.z, rw.z, (0.5 * alpha.z) * dt);
       }
       dv3 x = o.x;
       bool alive = true;
       if (ADVANCE) {
-        if (!begin_step(P, x, v, w, rx, rv, rw, a, tau, o.r, o.m, flags)) {
+        if (!begin_step(P, x, v, w, rx, rv, rw, a, alpha, flags)) {
           mark_removed(ctl, T, D, i, cap, flags);
           alive = false;
         } else {
@@ -865,7 +893,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
         st3(D.tau, i, cap, tau);
       }
       if (alive) {
-        st3(D.x, i, cap, x);
+        stxr(D.xr, i, x, o.r);
         st3(D.v, i, cap, v);
         st3(D.w, i, cap, w);
         st3(D.rx, i, cap, rx);
@@ -898,6 +926,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
     }
     bstats[blockIdx.x] = b;
   }
+}
+
+// Plain streaming copy of known size: reference point for hardware traffic
+// counters (FETCH_SIZE / WRITE_SIZE) measured in the same profile.
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const double* __restrict__ src, double* __restrict__ dst,
+                                                       size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
@@ -1186,9 +1222,10 @@ rp_status upload_meshes_binned(rp_engine* e) {
   return RP_OK;
 }
 
-// Buffer carving: Dyn = x v w rx rv rw a tau (24 planes of doubles);
+// Buffer carving: Dyn = xr (4 doubles per slot) then v w rx rv rw a tau
+// (21 planes of doubles);
 // Stat = r m (2 planes of doubles), flags (bytes), sid (u32).
-size_t dyn_bytes(int cap) { return round_up((size_t)cap * 24 * sizeof(double), 256); }
+size_t dyn_bytes(int cap) { return round_up((size_t)cap * 25 * sizeof(double), 256); }
 size_t stat_bytes(int cap) {
   size_t c = (size_t)cap;
   return round_up(c * 2 * sizeof(double), 256) + round_up(c, 256) + round_up(c * sizeof(uint32_t), 256);
@@ -1198,14 +1235,14 @@ Dyn carve_dyn(void* mem, int cap) {
   Dyn s{};
   size_t c = (size_t)cap;
   double* d = (double*)mem;
-  s.x = d;
-  s.v = d + 3 * c;
-  s.w = d + 6 * c;
-  s.rx = d + 9 * c;
-  s.rv = d + 12 * c;
-  s.rw = d + 15 * c;
-  s.a = d + 18 * c;
-  s.tau = d + 21 * c;
+  s.xr = d;
+  s.v = d + 4 * c;
+  s.w = d + 7 * c;
+  s.rx = d + 10 * c;
+  s.rv = d + 13 * c;
+  s.rw = d + 16 * c;
+  s.a = d + 19 * c;
+  s.tau = d + 22 * c;
   return s;
 }
 
@@ -1519,21 +1556,26 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
   rp_status st = alloc_nbr(e, 32);
   if (st) return st;
-  // host staging in plane layout
-  std::vector<double> planes((size_t)cap * 26, 0.0);
-  auto put3 = [&](int field, const double* src) {
+  // host staging in the device layout: Dyn = xr records + 7 vec3 fields as
+  // planes (25 doubles per slot), Stat = r and m planes
+  const size_t cs = (size_t)cap;
+  std::vector<double> dynh(cs * 25, 0.0), stath(cs * 2, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) dynh[4 * i + k] = x3[3 * i + k];
+    dynh[4 * i + 3] = radius[i];
+  }
+  auto put3 = [&](size_t first_plane, const double* src) {
     if (!src) return;
     for (size_t i = 0; i < n; ++i)
-      for (int k = 0; k < 3; ++k) planes[(size_t)field * 3 * cap + (size_t)k * cap + i] = src[3 * i + k];
+      for (int k = 0; k < 3; ++k) dynh[(first_plane + k) * cs + i] = src[3 * i + k];
   };
-  put3(0, x3);
-  put3(1, v3);
-  put3(2, w3);
-  put3(3, rx3);
-  put3(4, rv3);
-  put3(5, rw3);
-  put3(6, a3);
-  put3(7, tau3);
+  put3(4, v3);
+  put3(7, w3);
+  put3(10, rx3);
+  put3(13, rv3);
+  put3(16, rw3);
+  put3(19, a3);
+  put3(22, tau3);
   double rsum = 0.0, rmax = 0.0, vmax2 = 0.0;
   if (v3)
     for (size_t i = 0; i < n; ++i) {
@@ -1542,8 +1584,8 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
     }
   e->v_max = std::sqrt(vmax2);
   for (size_t i = 0; i < n; ++i) {
-    planes[(size_t)24 * cap + i] = radius[i];
-    planes[(size_t)25 * cap + i] = mass[i];
+    stath[i] = radius[i];
+    stath[cs + i] = mass[i];
     rsum += radius[i];
     if (radius[i] > rmax) rmax = radius[i];
   }
@@ -1555,9 +1597,8 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   }
   Dyn d0 = carve_dyn(e->d_dyn_mem[0], cap);
   Stat s0 = carve_stat(e->d_stat_mem[0], cap);
-  HIP_TRY(hipMemcpy(d0.x, planes.data(), (size_t)cap * 24 * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(s0.r, planes.data() + (size_t)24 * cap, (size_t)cap * 2 * sizeof(double),
-                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d0.xr, dynh.data(), dynh.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s0.r, stath.data(), stath.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.flags, fl.data(), (size_t)cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.sid, sid.data(), (size_t)cap * sizeof(uint32_t), hipMemcpyHostToDevice));
   e->dyn[0] = d0;
@@ -1708,7 +1749,12 @@ rp_status rp_engine_download(rp_engine* e, double* x3, double* v3, double* w3, d
   int ns;
   rp_status st = slot_order(e, order, cur, cst, ns);
   if (st) return st;
-  if ((st = fetch3(cur.x, e->cap, ns, order, x3))) return st;
+  if (x3 && ns > 0) {
+    std::vector<double> tmp((size_t)ns * 4);
+    HIP_TRY(hipMemcpy(tmp.data(), cur.xr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < order.size(); ++q)
+      for (int k = 0; k < 3; ++k) x3[3 * q + k] = tmp[4 * (size_t)order[q] + k];
+  }
   if ((st = fetch3(cur.v, e->cap, ns, order, v3))) return st;
   if ((st = fetch3(cur.w, e->cap, ns, order, w3))) return st;
   if ((st = fetch3(cur.a, e->cap, ns, order, a3))) return st;
@@ -1840,6 +1886,30 @@ rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3) {
   }
   e->ev_used = 0;
   e->ev_marks.clear();
+  return RP_OK;
+}
+
+rp_status rp_probe_copy(size_t bytes, double* ms) {
+  if (!ms) return fail(RP_ERR_INVALID, "ms is NULL");
+  size_t n = bytes / sizeof(double);
+  double *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&b, n * sizeof(double)));
+  HIP_TRY(hipMemset(a, 0, n * sizeof(double)));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, nullptr, a, b, n);
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_TRY(hipFree(a));
+  HIP_TRY(hipFree(b));
   return RP_OK;
 }
 

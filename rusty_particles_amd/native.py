@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
     L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
     L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
     L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
+    L.rp_probe_copy.argtypes = [C.c_size_t, _dp]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -62,7 +63,7 @@ def lib() -> C.CDLL:
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
-                 "rp_engine_timing"):
+                 "rp_engine_timing", "rp_probe_copy"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -94,6 +95,13 @@ def _vec(a, n: int) -> np.ndarray | None:
 
 def device_count() -> int:
     return int(lib().rp_device_count())
+
+
+def probe_copy(nbytes: int) -> float:
+    """Runs the library's plain copy kernel over nbytes; returns milliseconds."""
+    ms = C.c_double(0.0)
+    _check(lib().rp_probe_copy(int(nbytes), C.byref(ms)))
+    return ms.value
 
 
 def read_stl(path: str) -> tuple[np.ndarray, np.ndarray]:
