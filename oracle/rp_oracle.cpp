@@ -42,6 +42,7 @@ struct Grid {
   V3<R> origin{};
   int32_t dims[3]{1, 1, 1};
   std::vector<std::vector<int64_t>> cells;
+  std::vector<size_t> used;  // cells that received a particle since the last clear
 
   void init(R cell_size, V3<R> mn, V3<R> mx) {
     cs = cell_size;
@@ -52,9 +53,12 @@ struct Grid {
     dims[2] = sat_i32(std::ceil(q.z)) + 1;
     size_t total = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
     cells.assign(total, {});
+    used.clear();
   }
+  // grid.rs:24-28 empties every cell; emptying the used ones is equivalent.
   void clear() {
-    for (auto& c : cells) c.clear();
+    for (size_t c : used) cells[c].clear();
+    used.clear();
   }
   void coords(V3<R> p, int32_t c[3]) const {
     V3<R> q = (p - origin) / cs;
@@ -73,7 +77,10 @@ struct Grid {
   }
   void insert(V3<R> p, int64_t id) {
     size_t idx;
-    if (cell_index(p, &idx)) cells[idx].push_back(id);
+    if (cell_index(p, &idx)) {
+      if (cells[idx].empty()) used.push_back(idx);
+      cells[idx].push_back(id);
+    }
   }
   static int32_t wrap_add(int32_t c, int32_t o) {  // i32 wrapping add (release-mode Rust)
     return (int32_t)(uint32_t)((uint32_t)c + (uint32_t)o);

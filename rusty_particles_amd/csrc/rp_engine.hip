@@ -106,7 +106,13 @@ struct Ctl {
   int max_count;           // largest candidate count seen by k_build
   unsigned long long rebuilds;  // rebuilds done
   int pair_overflow;       // pair record overflow
-  int violation;           // a particle moved more than skin/2 since the last list build
+  // First step number that must not run on the current neighbour lists: a
+  // particle moved more than skin/2 since the last build (its list is no
+  // longer a superset).  Force launches of steps >= stall_from leave the state
+  // unchanged; the next rebuild resets it and the host re-issues the steps.
+  unsigned long long stall_from;
+  unsigned long long steps_done;  // force launches that advanced the state
+  int at_rest;             // 1 when the last advancing launch left the at-rest state
 };
 
 // Per-workgroup counters of one force launch.
@@ -507,6 +513,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->n = n;
     ctl->rebuilds += 1;
+    ctl->stall_from = ~0ull;
   }
 }
 
@@ -571,15 +578,17 @@ __device__ inline double mic_disp(double d, const KParams& P) {
 // is within skin/2 of its position at the list build.  The host launches the
 // rebuild chain only every `lookahead_steps` steps, so a particle asks for a
 // rebuild once its displacement plus a bound on its motion over that many
-// steps (2 K |v| dt + K^2 |a| dt^2) reaches skin/2.  Passing skin/2 itself
-// sets the violation flag, which the host reports as an error.
-__device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, const float* xb, int i, Ctl* ctl) {
+// steps (2 K |v| dt + K^2 |a| dt^2) reaches skin/2.  A particle already past
+// skin/2 at the position of step `next_step` stalls that step (stall_from);
+// the step runs again after the rebuild.
+__device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, const float* xb, int i, Ctl* ctl,
+                                     unsigned long long next_step) {
   const double d0 = mic_disp<0>(x.x - (double)xb[i], P);
   const double d1 = mic_disp<1>(x.y - (double)xb[i + P.cap], P);
   const double d2 = mic_disp<2>(x.z - (double)xb[i + 2 * P.cap], P);
   const double dd = d0 * d0 + d1 * d1 + d2 * d2;
   if (!(dd < P.safe_half_skin * P.safe_half_skin)) {
-    ctl->violation = 1;
+    atomicMin(&ctl->stall_from, next_step);
     return true;
   }
   const double k = P.lookahead_steps, dt = P.dt;
@@ -601,7 +610,7 @@ __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i
 // each rp_engine_step call).
 // flag_slot: the rebuild-request slot read by the next rebuild chain.
 __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KParams P, Dyn S, Stat T,
-                                                  const float* xb) {
+                                                  const float* xb, unsigned long long step) {
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -624,11 +633,12 @@ __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KPara
         st3(S.rx, i, cap, rx);
         st3(S.rv, i, cap, rv);
         st3(S.rw, i, cap, rw);
-        far = moved_too_far(P, x, v, a, xb, i, ctl);
+        far = moved_too_far(P, x, v, a, xb, i, ctl, step);
       }
     }
   }
   if (__any(far) && (threadIdx.x & 63) == 0) atomicOr(&ctl->rebuild[flag_slot], 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->at_rest = 0;
 }
 
 // Pair test of slot i against slot j (simulation.rs:323-357): minimum image
@@ -790,6 +800,20 @@ __device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& t
 
 __device__ inline int sel4(int q, int a, int b, int c, int d) { return q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d)); }
 
+__device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
+  reinterpret_cast<double4*>(D.xr)[i] = reinterpret_cast<const double4*>(S.xr)[i];
+  for (int k = 0; k < 3; ++k) {
+    const int o = i + k * cap;
+    D.v[o] = S.v[o];
+    D.w[o] = S.w[o];
+    D.rx[o] = S.rx[o];
+    D.rv[o] = S.rv[o];
+    D.rw[o] = S.rw[o];
+    D.a[o] = S.a[o];
+    D.tau[o] = S.tau[o];
+  }
+}
+
 // Gather force kernel + integration epilogue, one thread per particle slot.
 // Reads Dyn buffer S and writes buffer D (so no thread can read a neighbour's
 // next-step state); the host swaps the two between steps.  All own-state loads are issued before the neighbour loop, and
@@ -799,12 +823,21 @@ __device__ inline int sel4(int q, int a, int b, int c, int d) { return q == 0 ? 
 // the state drifted; otherwise stores a, tau and leaves the at-rest state.
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
 template <int NORMAL, int TANG, bool ADVANCE, bool RECORD>
-__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KParams P, Dyn S, Dyn D, Stat T,
-                                                  const int* nbr_count, const int* nbr, const Tri* tris,
-                                                  const uint32_t* tri_start, const uint32_t* tri_idx,
-                                                  const float* xb, BlockStats* bstats,
+__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long step, KParams P,
+                                                  Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
+                                                  const Tri* tris, const uint32_t* tri_start,
+                                                  const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
                                                   unsigned long long* pairbuf, int* paircount) {
   const int n = ctl->n;
+  if (ctl->stall_from <= step || ctl->overflow) {
+    // The lists may miss a pair for this step: keep the state as it is (the
+    // host swaps the buffers after every launch, so copy it across) and let
+    // the host run this step again after a rebuild.
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) copy_slot(S, D, i, P.cap);
+    if (threadIdx.x == 0) bstats[blockIdx.x] = BlockStats{0u, 0u, 0u, 0u};
+    return;
+  }
   const int cap = P.cap;
   const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   unsigned int nwall = 0;
@@ -886,7 +919,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
           mark_removed(ctl, T, D, i, cap, flags);
           alive = false;
         } else {
-          far = moved_too_far(P, x, v, a, xb, i, ctl);
+          far = moved_too_far(P, x, v, a, xb, i, ctl, step + 1);
         }
       } else {
         st3(D.a, i, cap, a);
@@ -925,6 +958,10 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, KPara
       b.overlap |= red[q][3];
     }
     bstats[blockIdx.x] = b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->steps_done += 1;
+    ctl->at_rest = ADVANCE ? 0 : 1;
   }
 }
 
@@ -973,6 +1010,9 @@ struct rp_engine {
   Stat stat[2]{};     // stat[0] holds the constants; stat[1] is rebuild scratch
   int cur = 0;
   uint64_t chain_no = 0;  // rebuild chains launched; chain c reads request slot c & 1
+  uint64_t launch_no = 0; // force launches issued (labels for the stall watermark)
+  uint64_t steps_done = 0;  // steps that advanced the state (device count at the last sync)
+  bool at_rest = true;    // state is at rest (needs k_begin before the next force launch)
   BlockStats* d_bstats = nullptr;
   int force_grid = 0;
   float* d_xb = nullptr;
@@ -1348,10 +1388,6 @@ rp_status launch_rebuild(rp_engine* e) {
 rp_status read_ctl(rp_engine* e) {
   HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (e->host_ctl.violation)
-    return fail(RP_ERR_STATE,
-                "neighbour list invalidated: a particle moved more than half the Verlet skin between rebuild "
-                "checks; raise rp_params.skin");
   return RP_OK;
 }
 
@@ -1434,11 +1470,13 @@ void mark_end(rp_engine* e) {
 template <int NORMAL, int TANG, bool ADV, bool REC>
 void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot, e->kp,
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
+                     (unsigned long long)e->launch_no, e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount);
   e->force_grid = g;
   e->cur ^= 1;
+  e->launch_no++;
 }
 
 // Tangential kinds compiled: Coulomb, and the spring law (LSC / Mindlin) with
@@ -1610,8 +1648,12 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   Ctl c{};
   c.n = (int)n;
   c.alive = (int)n;
+  c.stall_from = ~0ull;
+  c.at_rest = 1;
   HIP_TRY(hipMemcpy(e->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
   e->n_uploaded = (int)n;
+  e->steps_done = 0;
+  e->at_rest = true;
   e->r_max = rmax;
   e->r_mean = n ? rsum / (double)n : 0.0;
   e->have_particles = true;
@@ -1656,33 +1698,61 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (nsteps == 0) return RP_OK;
   rp_status st = prepare(e);
   if (st) return st;
-  if (e->record && (st = ensure_pairs(e))) return st;
-  int gcap = (int)((e->cap + kBlock - 1) / kBlock);
-  mark_begin(e, 2);
-  hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1), e->kp,
-                     e->dyn[e->cur], e->stat[0], e->d_xb);
-  mark_end(e);
-  for (uint64_t s = 0; s < nsteps; ++s) {
-    // The rebuild chain runs at the first step of a call and every
-    // kRebuildInterval steps; the force kernels' look-ahead covers the gap.
-    if (s == 0 || e->step_count % kRebuildInterval == 0) {
-      mark_begin(e, 1);
-      if ((st = launch_rebuild(e))) return st;
+  const int gcap = (int)((e->cap + kBlock - 1) / kBlock);
+  uint64_t remaining = nsteps;
+  for (int round = 0; remaining > 0; ++round) {
+    if (round > 64) return fail(RP_ERR_STATE, "time step made no progress after 64 neighbour-list rebuilds");
+    if (e->record && (st = ensure_pairs(e))) return st;
+    if (e->at_rest) {
+      mark_begin(e, 2);
+      hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1),
+                         e->kp, e->dyn[e->cur], e->stat[0], e->d_xb, (unsigned long long)e->launch_no);
       mark_end(e);
     }
-    const int slot = (int)(e->chain_no & 1);
-    const bool last = (s + 1 == nsteps);
-    mark_begin(e, 0);
-    if (last) {
-      if (e->record) launch_force<false, true>(e, slot);
-      else launch_force<false, false>(e, slot);
-    } else {
-      launch_force<true, false>(e, slot);
+    for (uint64_t s = 0; s < remaining; ++s) {
+      // The rebuild chain runs at the first launch of a round and every
+      // kRebuildInterval launches; the force kernels' look-ahead covers the gap.
+      if (s == 0 || e->launch_no % kRebuildInterval == 0) {
+        mark_begin(e, 1);
+        if ((st = launch_rebuild(e))) return st;
+        mark_end(e);
+      }
+      const int slot = (int)(e->chain_no & 1);
+      const bool last = (s + 1 == remaining);
+      mark_begin(e, 0);
+      if (last) {
+        if (e->record) launch_force<false, true>(e, slot);
+        else launch_force<false, false>(e, slot);
+      } else {
+        launch_force<true, false>(e, slot);
+      }
+      mark_end(e);
     }
-    mark_end(e);
-    e->step_count++;
+    HIP_TRY(hipGetLastError());
+    // One sync per call: count the launches that advanced the state.  Launches
+    // stalled by a skin violation or a list overflow are issued again.
+    HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const uint64_t executed = e->host_ctl.steps_done - e->steps_done;
+    e->steps_done = e->host_ctl.steps_done;
+    e->at_rest = e->host_ctl.at_rest != 0;
+    remaining -= executed;
+    if (remaining == 0) break;
+    if (e->host_ctl.overflow) {
+      // a rebuild found more candidates than the lists hold: grow them
+      int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 16), 8);
+      if ((st = alloc_nbr(e, std::max(want, e->nbr_cap + 8)))) return st;
+      int zero = 0;
+      HIP_TRY(hipMemcpyAsync(&e->d_ctl->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    } else if (executed == 0) {
+      // a particle crosses half the skin within one step: widen the skin
+      e->prm.skin = 2.0 * (e->prm.skin > 0.0 ? e->prm.skin : e->kp.skin);
+      e->geometry_dirty = true;
+      if ((st = prepare(e))) return st;
+    }
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1));
   }
-  HIP_TRY(hipGetLastError());
+  e->step_count += nsteps;
   e->pairs_valid = e->record;
   return RP_OK;
 }
@@ -1844,7 +1914,7 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
   rp_status st = read_ctl(e);
   if (st) return st;
   const Ctl& c = e->host_ctl;
-  out->steps = e->step_count;
+  out->steps = e->host_ctl.steps_done;
   out->rebuilds = c.rebuilds;
   out->removed = (uint64_t)(e->n_uploaded - c.alive);
   out->neighbour_capacity = e->nbr_cap;

@@ -130,12 +130,13 @@ def write_stl_binary(path, v9, n3, header=b"rusty_particles_amd"):
 def write_stl_ascii(path, v9, n3, name="mesh"):
     v9 = np.asarray(v9, dtype=np.float32).reshape(-1, 9)
     n3 = np.asarray(n3, dtype=np.float32).reshape(-1, 3)
+    fmt = lambda v: np.format_float_scientific(np.float32(v), unique=True)  # shortest round-trip form
     with open(path, "w") as f:
         f.write(f"solid {name}\n")
         for t in range(v9.shape[0]):
-            f.write(f"  facet normal {n3[t,0]!r} {n3[t,1]!r} {n3[t,2]!r}\n    outer loop\n")
+            f.write(f"  facet normal {fmt(n3[t,0])} {fmt(n3[t,1])} {fmt(n3[t,2])}\n    outer loop\n")
             for k in range(3):
-                f.write(f"      vertex {v9[t,3*k]!r} {v9[t,3*k+1]!r} {v9[t,3*k+2]!r}\n")
+                f.write(f"      vertex {fmt(v9[t,3*k])} {fmt(v9[t,3*k+1])} {fmt(v9[t,3*k+2])}\n")
             f.write("    endloop\n  endfacet\n")
         f.write(f"endsolid {name}\n")
 

@@ -1,0 +1,88 @@
+"""The Python surface (src/lib.rs:15-277) without a GPU: argument handling,
+error types, host-side state and output format."""
+import numpy as np
+import pytest
+
+import rusty_particles_amd as rp
+from rusty_particles_amd import native, simulation
+
+
+def test_hello_and_threads():
+    assert rp.hello() == "Hello from Rust!"
+    simulation._threads_set = False
+    rp.set_num_threads(4)
+    with pytest.raises(RuntimeError):
+        rp.set_num_threads(2)
+
+
+def test_model_names_case_insensitive_and_invalid():
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    for nm in ("Hertz", "hertzian", "LINEAR", "spring_dashpot", "hysteretic", "JKR", "sJKR", "simplified_jkr"):
+        for tm in ("Mindlin", "coulomb", "Linear", "spring_coulomb"):
+            sim.set_contact_models(nm, tm)
+    with pytest.raises(ValueError):
+        sim.set_contact_models("invalid", "coulomb")
+    with pytest.raises(ValueError):
+        sim.set_contact_models("hertz", "nope")
+
+
+def test_particles_and_index_errors():
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 2)
+    sim.add_particle(0.1, 0.2, 0.3, 0.05, 1.0)
+    sim.add_particle(0.0, 0.0, 0.0, 0.05, 2.0, fixed=True)
+    assert sim.get_particle_count() == 2
+    assert sim.get_particle_position(0) == (0.1, 0.2, 0.3)
+    sim.set_particle_velocity(1, 1.0, 2.0, 3.0)
+    sim.set_particle_angular_velocity(1, 4.0, 5.0, 6.0)
+    assert sim.get_particle_velocity(1) == (1.0, 2.0, 3.0)
+    assert sim.get_particle_angular_velocity(1) == (4.0, 5.0, 6.0)
+    for f in (sim.get_particle_position, sim.get_particle_velocity, sim.get_particle_angular_velocity):
+        with pytest.raises(IndexError):
+            f(2)
+    with pytest.raises(IndexError):
+        sim.set_particle_velocity(5, 0, 0, 0)
+
+
+def test_step_without_device_raises_runtime_error():
+    if native.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 1)
+    sim.add_particle(0, 0, 0, 0.1, 1.0)
+    with pytest.raises(RuntimeError):
+        sim.step()
+    with pytest.raises(RuntimeError):
+        sim.enable_gpu()
+
+
+def test_run_step_count_uses_f32(monkeypatch):
+    """steps = ceil(f32(duration) / f32(dt)) (src/lib.rs:183)."""
+    seen = []
+    sim = rp.Simulation.create(1e-4, -1, -1, -1, 1, 1, 1, 1)
+    monkeypatch.setattr(sim, "_steps", lambda n: seen.append(n))
+    sim.run(1.0)
+    sim.run(0.1)
+    assert seen == [int(np.ceil(np.float32(1.0) / np.float32(1e-4))),
+                    int(np.ceil(np.float32(0.1) / np.float32(1e-4)))]
+
+
+def test_add_mesh_io_error(tmp_path):
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 1)
+    with pytest.raises(OSError):
+        sim.add_mesh(str(tmp_path / "nope.stl"))
+
+
+def test_write_vtk_layout(tmp_path):
+    """Legacy ASCII POLYDATA with velocity, radius and id (src/vtk.rs:5-44)."""
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 2)
+    sim.add_particle(0.5, 1.0, -0.25, 0.1, 1.0)
+    sim.add_particle(0.0, 0.0, 0.0, 0.2, 1.0)
+    sim.set_particle_velocity(0, 1.5, 0.0, 0.0)
+    path = tmp_path / "out.vtk"
+    sim.write_vtk(str(path))
+    lines = path.read_text().splitlines()
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "DEM Simulation Data", "ASCII", "DATASET POLYDATA",
+                         "POINTS 2 float"]
+    assert lines[5] == "0.5 1 -0.25" and lines[6] == "0 0 0"
+    assert lines[7:10] == ["POINT_DATA 2", "VECTORS velocity float", "1.5 0 0"]
+    assert lines[11:15] == ["SCALARS radius float 1", "LOOKUP_TABLE default", "0.1", "0.2"]
+    assert lines[15:19] == ["SCALARS id int 1", "LOOKUP_TABLE default", "0", "1"]
