@@ -11,6 +11,16 @@ A "step" is one full DEM time step over all particles (broad phase rebuild
 when the Verlet skin demands it, contact gather, walls, integration); the
 inputs are resident in HBM before timing starts.
 
+--workload jkr: 2,000,000 spheres (r lognormal, median 50 um, sigma 0.3,
+rho 1500) in a periodic box at solid fraction 0.35, JKR + Coulomb (E=1e8,
+nu=0.3, gamma=0.05, mu=0.3), dt=1e-8 s, velocities ~N(0, 0.05) m/s.  The
+step runs once per JKR solver on the same initial state: "exact" (the
+reference's Newton solve), "shortcut" (identical results; no iterations for
+contacts that cannot have a root) and "surrogate" (the BF16-MFMA network,
+trained on the GPU before timing, plus FP64 polish and exact fallback);
+value = the surrogate form, the others and the surrogate's force errors
+(check mode, exact solve alongside) are in the same line.
+
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
 HIP events on the engine's stream), and a CPU baseline: the oracle (a C++
@@ -45,15 +55,21 @@ def log(msg: str) -> None:
 
 
 def build_workload(name: str, n: int | None):
+    """(params, x, r, m, v or None, meshes, description)."""
     from rusty_particles_amd import scenes
     if name == "hopper":
-        return scenes.hopper_workload(n or 1_000_000)
+        p, x, r, m, meshes = scenes.hopper_workload(n or 1_000_000)
+        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper"
     if name == "box":
-        return scenes.box_settle_workload(n or 10_000)
+        p, x, r, m, meshes = scenes.box_settle_workload(n or 10_000)
+        return p, x, r, m, None, meshes, "Hertz+Coulomb box"
+    if name == "jkr":
+        p, x, r, m, v = scenes.jkr_periodic_workload(n or 2_000_000)
+        return p, x, r, m, v, [], "JKR+Coulomb periodic packing"
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(p, x, r, m, meshes, steps: int, threads: int, cell: float):
+def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
     largest contact cutoff (the cell-size heuristic of simulation.rs:64)."""
     import numpy as np
@@ -63,7 +79,7 @@ def cpu_baseline(p, x, r, m, meshes, steps: int, threads: int, cell: float):
     lib = oracle.lib()
     lib.rpo_set_threads(threads)
     o = oracle.OracleSimulation(p.dt, None, None, params=abi.RpParams.from_buffer_copy(p))
-    o.set_state(x, r, m)
+    o.set_state(x, r, m, v=v)
     for v9, n3 in meshes:
         o.add_mesh_arrays(v9, n3)
     o.set_cell_size(cell)
@@ -86,6 +102,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--jkr-forms", default="exact,shortcut,surrogate",
+                    help="JKR solvers to time on the jkr workload (the last one is the reported value)")
+    ap.add_argument("--train-samples", type=int, default=1_000_000)
+    ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -102,41 +122,90 @@ def main() -> int:
         raise SystemExit("bench.py: no HIP device visible")
 
     t0 = time.perf_counter()
-    p, x, r, m, meshes = build_workload(args.workload, args.n)
+    p, x, r, m, v, meshes, desc = build_workload(args.workload, args.n)
     n = x.shape[0]
     ntri = sum(v9.shape[0] for v9, _ in meshes)
     log(f"workload {args.workload}: {n} particles, {ntri} triangles, built in {time.perf_counter() - t0:.1f}s")
 
-    eng = native.Engine(abi.RpParams.from_buffer_copy(p))
-    eng.upload(x, r, m)
-    for v9, n3 in meshes:
-        eng.add_mesh(v9, n3)
-    eng.step(args.warmup)
-    eng.synchronize()
-    log(f"warmup {args.warmup} steps done")
+    forms = args.jkr_forms.split(",") if p.normal_model == abi.RP_NORMAL_JKR else ["plain"]
+    net = None
+    surr_info = {}
+    if "surrogate" in forms:
+        from rusty_particles_amd import surrogate
+        rs = r[:, None] * r[None, :200] / (r[:, None] + r[None, :200])
+        env = surrogate.envelope(-0.40, 30.0, (0.9 * rs.min(), 1.1 * rs.max()),
+                                 1.0 / (2 * (1 - p.particle.poissons_ratio ** 2) / p.particle.youngs_modulus),
+                                 p.particle.surface_energy)
+        t0 = time.perf_counter()
+        net = surrogate.train(env, nsamples=args.train_samples, epochs=args.train_epochs, batch=4096, lr=3e-3)
+        surr_info["train_s"] = time.perf_counter() - t0
+        surr_info["train_loss"] = [float(net.epoch_loss[0]), float(net.epoch_loss[-1])]
+        log(f"surrogate trained in {surr_info['train_s']:.1f}s, loss {net.epoch_loss[0]:.3g} -> "
+            f"{net.epoch_loss[-1]:.3g}")
 
-    # timed region: whole job, K steps, one enqueue, synchronised on both sides
-    eng.synchronize()
-    t0 = time.perf_counter()
-    eng.step(args.steps)
-    eng.synchronize()
-    wall = time.perf_counter() - t0
-    st = eng.stats()
-    value = n * args.steps / wall
-    log(f"timed {args.steps} steps: {wall:.3f}s, {value:.4g} particle-steps/s, rebuilds {st['rebuilds']}, "
-        f"max list length {st['max_neighbours']}, list capacity {st['neighbour_capacity']}")
+    results = {}
+    eng = None
+    for form in forms:
+        if eng is not None:
+            eng.close()
+        eng = native.Engine(abi.RpParams.from_buffer_copy(p))
+        eng.upload(x, r, m, v=v)
+        for v9, n3 in meshes:
+            eng.add_mesh(v9, n3)
+        if form != "plain":
+            eng.set_jkr_solver(form, net)
+        eng.step(args.warmup)
+        eng.synchronize()
+        log(f"[{form}] warmup {args.warmup} steps done")
 
-    # kernel roofline: HIP events around each launch on the engine stream
-    eng.set_timing(True)
-    eng.step(args.timing_steps)
-    tm = eng.timing()
-    eng.set_timing(False)
-    f_ms, f_cnt = tm["force"]
-    avg_s = f_ms / max(f_cnt, 1) * 1e-3
-    bytes_per_launch = FORCE_BYTES_PER_PARTICLE * n
-    achieved = bytes_per_launch / avg_s / 1e9
-    log(f"force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
-        f"rebuild chain avg {tm['rebuild'][0] / max(tm['rebuild'][1], 1) * 1e3:.1f} us")
+        # timed region: whole job, K steps, one enqueue, synchronised on both sides
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.step(args.steps)
+        eng.synchronize()
+        wall = time.perf_counter() - t0
+        st = eng.stats()
+        value = n * args.steps / wall
+        log(f"[{form}] timed {args.steps} steps: {wall:.3f}s, {value:.4g} particle-steps/s, rebuilds "
+            f"{st['rebuilds']}, max list length {st['max_neighbours']}, list capacity {st['neighbour_capacity']}")
+
+        # kernel roofline: HIP events around each launch on the engine stream
+        eng.set_timing(True)
+        eng.step(args.timing_steps)
+        tm = eng.timing()
+        eng.set_timing(False)
+        f_ms, f_cnt = tm["force"]
+        avg_s = f_ms / max(f_cnt, 1) * 1e-3
+        achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
+        res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
+               "achieved_gbs": achieved, "contacts_per_step": st["contacts"],
+               "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st}
+        if tm["surrogate"][1]:
+            res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
+        log(f"[{form}] force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
+            f"rebuild chain avg {tm['rebuild'][0] / max(tm['rebuild'][1], 1) * 1e3:.1f} us"
+            + (f"; collect+MLP avg {res['surrogate_kernels_avg_us']:.1f} us" if tm["surrogate"][1] else ""))
+        if form == "surrogate":
+            eng.set_jkr_solver("surrogate", net, check=True)
+            errs = []
+            for _ in range(5):
+                eng.step(1)
+                errs.append(eng.jkr_stats())
+            surr_info.update({
+                "served_per_step": float(np.mean([e["served"] for e in errs])),
+                "fallback_per_step": float(np.mean([e["fallback"] for e in errs])),
+                "force_rel_err_max": max(e["err_max"] for e in errs),
+                "force_rel_err_mean": float(np.mean([e["err_mean"] for e in errs])),
+                "network_force_rel_err_max": max(e["raw_max"] for e in errs),
+                "network_force_rel_err_mean": float(np.mean([e["raw_mean"] for e in errs])),
+                "rows_capacity": errs[-1]["rows_capacity"],
+            })
+            log(f"[surrogate] check: {surr_info}")
+        results[form] = res
+    main_form = forms[-1]
+    R = results[main_form]
+    value, wall, st, avg_s, achieved = R["value"], R["wall"], R["stats"], R["force_kernel_avg_us"] * 1e-6, \
+        R["achieved_gbs"]
     if args.calibrate_copy:
         ms = native.probe_copy(256 * 1024 * 1024)
         log(f"copy calibration: k_copy_probe read 268435456 B + wrote 268435456 B in {ms:.3f} ms "
@@ -152,7 +221,7 @@ def main() -> int:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cell = 2.0 * float(r.max()) * (1.5 if p.normal_model in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 1.0)
         log(f"cpu baseline: oracle, {threads} threads, {args.cpu_steps} step(s) of the same {n}-particle workload")
-        cval, csec = cpu_baseline(p, x, r, m, meshes, args.cpu_steps, threads, cell)
+        cval, csec = cpu_baseline(p, x, r, m, v, meshes, args.cpu_steps, threads, cell)
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} step(s) of the full {n}-particle workload after 1 warm-up step, "
                          f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff), brute-force "
@@ -172,16 +241,20 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} spheres, {ntri}-triangle STL, "
-                               f"{'Hertz+Mindlin' if args.workload == 'hopper' else 'Hertz+Coulomb'}, "
-                               f"dt={p.dt:g}, FP64", "particles": n, "triangles": ntri, "form": "plain"},
+        "config": {"workload": f"{args.workload}: {n} spheres, {ntri}-triangle STL, {desc}, dt={p.dt:g}, FP64",
+                   "particles": n, "triangles": ntri, "form": main_form},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "contacts_per_step": st["contacts"],
         "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
+        "jkr_failures_per_step": st["jkr_failures"],
     }
+    if len(results) > 1 or main_form != "plain":
+        out["forms"] = {f: {k: v for k, v in res.items() if k not in ("stats", "wall")} for f, res in results.items()}
+    if surr_info:
+        out["surrogate"] = surr_info
     print(json.dumps(out), flush=True)
     return 0
 

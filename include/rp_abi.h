@@ -116,6 +116,16 @@ typedef struct rp_stats {
 
 typedef struct rp_engine rp_engine;
 
+/* Operating envelope of the JKR surrogate: d = delta / t with
+ * s = (R*^2 8 pi gamma / E*)^(1/3), t = s^2 / R* (the scaled overlap), and the
+ * ranges of R*, E* and gamma.  Contacts outside it use the exact solve. */
+typedef struct rp_jkr_envelope {
+  double d_lo, d_hi;
+  double rstar_lo, rstar_hi;  /* m */
+  double estar_lo, estar_hi;  /* Pa */
+  double gamma_lo, gamma_hi;  /* J/m^2 */
+} rp_jkr_envelope;
+
 /* Fills *p with the reference defaults of Simulation::new
  * (src/simulation.rs:62-91): particle material (1e7,0.3,2500,0.5,0.5,0),
  * wall material (1e9,0.3,7800,0.5,0.5,0), Hertzian + Mindlin, no periodic
@@ -209,20 +219,98 @@ rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
 /* Turns per-kernel timing on or off (off by default).  When on, every
- * force-kernel launch, every rebuild chain and every first-half kernel is
- * bracketed by HIP events on the engine's stream. */
+ * force-kernel launch, every rebuild chain, every first-half kernel and every
+ * surrogate collect + inference pair is bracketed by HIP events on the
+ * engine's stream. */
 rp_status rp_engine_set_timing(rp_engine* e, int on);
 
 /* Synchronises, then returns the summed event times in milliseconds and the
- * number of timed launches since the last call: index 0 = force kernel,
- * 1 = rebuild chain (including its no-op launches), 2 = first half kernel. */
-rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3);
+ * number of timed launches since the last call, in arrays of RP_TIMING_KINDS:
+ * index 0 = force kernel, 1 = rebuild chain (including its no-op launches),
+ * 2 = first half kernel, 3 = JKR surrogate (collect + network). */
+#define RP_TIMING_KINDS 4
+rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count);
 
 /* Diagnostics, no counterpart in the reference: copies `bytes` bytes
  * between two fresh device buffers with a plain streaming kernel and returns
  * its time.  A counter profile of the same process uses it to scale
  * FETCH_SIZE / WRITE_SIZE by a kernel of known traffic. */
 rp_status rp_probe_copy(size_t bytes, double* ms);
+
+/* ---- JKR contact-radius surrogate (no counterpart in the reference) ----
+ * A 4 -> 64 -> 64 -> 2 MLP (tanh; BF16 MFMA, FP32 accumulate) mapping a
+ * contact's scaled (d, ln R*, ln E*, ln gamma) to its scaled contact radius
+ * a / s and force F / (pi gamma R*); it replaces the initial guess of the
+ * Newton solve of compute_jkr_force (physics.rs:132-219).  Parameters are a
+ * packed FP32 vector of RP_MLP_PARAMS floats: W1[4x64] b1[64] W2[64x64]
+ * b2[64] W3[64x2] b3[2] (row-major, input index first). */
+#define RP_MLP_PARAMS 4610
+
+/* Trains a network on the GPU: nsamples samples drawn uniformly over the
+ * envelope, labelled by the FP64 Newton solve; epochs of mini-batch Adam
+ * (batch a multiple of 16), cosine-decayed learning rate lr.  Writes the
+ * parameters and, when epoch_loss is not NULL, the last batch's mean squared
+ * error (scaled outputs) of every epoch. */
+rp_status rp_jkr_surrogate_train(const rp_jkr_envelope* env, uint64_t seed, size_t nsamples, int epochs,
+                                 int batch, float lr, float* params_out, double* epoch_loss);
+
+/* Labelled samples as used by the training (device generated): x_scaled
+ * rows*4, t_scaled rows*2, raw8 rows*8 = (delta, R*, E*, gamma, a, F at a
+ * without the pull-off clip, d, 1 if the Newton solve converged).  Any output
+ * may be NULL. */
+rp_status rp_jkr_samples(const rp_jkr_envelope* env, uint64_t seed, size_t n, float* x_scaled,
+                         float* t_scaled, double* raw8);
+
+/* Input and output scaling of an envelope (in: d, ln R*, ln E*, ln gamma;
+ * out: a / s, F / (pi gamma R*)); values map linearly to [-1, 1]. */
+rp_status rp_jkr_envelope_scale(const rp_jkr_envelope* env, float* in_lo4, float* in_hi4, float* out_lo2,
+                                float* out_hi2);
+
+/* Inference (the engine's fused MFMA kernel) on rows scaled inputs. */
+rp_status rp_mlp_infer(const float* params, const float* x_scaled, size_t rows, float* y_scaled);
+
+/* One forward + backward pass on rows (a multiple of 16) samples: gradient
+ * of the mean squared error of the scaled outputs, packed like the
+ * parameters, and the loss. */
+rp_status rp_mlp_gradient(const float* params, const float* x_scaled, const float* t_scaled, size_t rows,
+                          float* grad, double* loss);
+
+/* One 16x16x16 BF16 MFMA tile C = A B (row-major 16x16 FP32 in and out; A and
+ * B rounded to BF16) through the library's product kernel. */
+rp_status rp_mfma_probe(const float* a16x16, const float* b16x16, float* c16x16);
+
+/* Message of the last failed surrogate call on this thread. */
+const char* rp_surrogate_last_error(void);
+
+/* JKR solver of an engine (no counterpart in the reference, whose JKR force
+ * is always its Newton solve, physics.rs:132-219):
+ *   RP_JKR_EXACT      the reference's solve as written (default);
+ *   RP_JKR_SHORTCUT   the same, except that a contact whose overlap lies
+ *                     below t d_min - 1e-3 R* (no iterate can pass the
+ *                     residual check) returns the zero force and rejected
+ *                     root at once -- identical results;
+ *   RP_JKR_SURROGATE  the shortcut, plus the network for particle-particle
+ *                     contacts inside the envelope (away from
+ *                     0 < d < 0.2, where the reference's start lies on the
+ *                     wrong branch): its radius is polished by FP64 Newton
+ *                     steps; contacts outside the envelope, wall contacts
+ *                     and failed polishes use the exact solve.
+ * check != 0 also runs the exact solve for every served contact and records
+ * the relative force errors (rp_engine_jkr_stats). */
+enum { RP_JKR_EXACT = 0, RP_JKR_SHORTCUT = 1, RP_JKR_SURROGATE = 2 };
+rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_envelope* env,
+                                   const float* params, int32_t check);
+
+/* Surrogate counters of the most recent force launch. */
+typedef struct rp_jkr_stats {
+  uint64_t served;         /* contacts whose radius came from the network */
+  uint64_t fallback;       /* served-envelope contacts that took the exact solve */
+  uint64_t rows_max;       /* most feature rows any step wanted */
+  uint64_t rows_capacity;  /* feature rows the engine holds */
+  double err_max, err_mean;  /* check mode: |F - F_exact| / max(|F_exact|, (2/3) pi gamma R*) */
+  double raw_max, raw_mean;  /* check mode: the same for the network's force output */
+} rp_jkr_stats;
+rp_status rp_engine_jkr_stats(rp_engine* e, rp_jkr_stats* out);
 
 /* Counters (rp_stats). */
 rp_status rp_engine_stats(rp_engine* e, rp_stats* out);

@@ -69,6 +69,34 @@ class RpParams(C.Structure):
     ]
 
 
+RP_MLP_PARAMS = 4610
+RP_TIMING_KINDS = 4
+RP_JKR_EXACT, RP_JKR_SHORTCUT, RP_JKR_SURROGATE = 0, 1, 2
+JKR_SOLVERS = {"exact": RP_JKR_EXACT, "shortcut": RP_JKR_SHORTCUT, "surrogate": RP_JKR_SURROGATE}
+
+
+class RpJkrStats(C.Structure):
+    _fields_ = [
+        ("served", C.c_uint64),
+        ("fallback", C.c_uint64),
+        ("rows_max", C.c_uint64),
+        ("rows_capacity", C.c_uint64),
+        ("err_max", C.c_double),
+        ("err_mean", C.c_double),
+        ("raw_max", C.c_double),
+        ("raw_mean", C.c_double),
+    ]
+
+
+class RpJkrEnvelope(C.Structure):
+    _fields_ = [
+        ("d_lo", C.c_double), ("d_hi", C.c_double),
+        ("rstar_lo", C.c_double), ("rstar_hi", C.c_double),
+        ("estar_lo", C.c_double), ("estar_hi", C.c_double),
+        ("gamma_lo", C.c_double), ("gamma_hi", C.c_double),
+    ]
+
+
 class RpStats(C.Structure):
     _fields_ = [
         ("steps", C.c_uint64),
@@ -126,6 +154,15 @@ EXPORTS = (
     "rp_engine_set_timing",
     "rp_engine_timing",
     "rp_probe_copy",
+    "rp_jkr_surrogate_train",
+    "rp_jkr_samples",
+    "rp_jkr_envelope_scale",
+    "rp_mlp_infer",
+    "rp_mlp_gradient",
+    "rp_mfma_probe",
+    "rp_surrogate_last_error",
+    "rp_engine_set_jkr_solver",
+    "rp_engine_jkr_stats",
     "rp_engine_destroy",
     "rp_last_error",
 )

@@ -51,7 +51,8 @@ def _run(cmd: list[str]) -> None:
 
 def build_hip() -> str:
     tmp = HIP_LIB + ".tmp"
-    srcs = [os.path.join(CSRC, "rp_engine.hip"), os.path.join(CSRC, "rp_stl.cpp")]
+    srcs = [os.path.join(CSRC, "rp_engine.hip"), os.path.join(CSRC, "rp_surrogate.hip"),
+            os.path.join(CSRC, "rp_stl.cpp")]
     _run([_hipcc(), *HIP_FLAGS, *srcs, "-o", tmp])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB

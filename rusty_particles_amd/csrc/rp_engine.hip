@@ -42,6 +42,7 @@
 #include "../../include/rp_abi.h"
 #include "rp_physics.hpp"
 #include "rp_stl.hpp"
+#include "rp_surrogate.hpp"
 
 using namespace rp;
 
@@ -113,6 +114,8 @@ struct Ctl {
   unsigned long long stall_from;
   unsigned long long steps_done;  // force launches that advanced the state
   int at_rest;             // 1 when the last advancing launch left the at-rest state
+  unsigned surr_rows;      // JKR surrogate feature rows claimed by the current step
+  unsigned surr_rows_max;  // most rows any step wanted (sizes the row buffers)
 };
 
 // Per-workgroup counters of one force launch.
@@ -678,13 +681,67 @@ struct Acc {
   unsigned int ncont, njkr;
   int npair;
   bool overlap;
+  // JKR surrogate: first feature row of this particle (-1 = none), rows
+  // consumed so far, and counters
+  int srow, sk;
+  unsigned int sserved, sfall;
+  float emax, rmax;
+  double esum, rsum;
 };
+
+// JKR normal force with the exact no-root shortcut (JS >= 1) and the network
+// (JS == 2).  JS == 0 is the reference's solve as written.  The shortcut and
+// jkr() agree exactly (zero force, a rejected root); the network path polishes
+// the predicted radius to the root the reference converges to.
+template <int JS>
+__device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rstar, int* failed, int k,
+                               const SurrDev& sd, Acc& acc) {
+  if (JS == 0 || !(c.gamma > 0.0)) return jkr(ov, n, c, rstar, failed);
+  const double s = cbrt(rstar * rstar * c.jc);
+  const double t = s * s / rstar;
+  if (ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov))) {
+    *failed = 1;
+    return zero3();
+  }
+  if (JS == 2) {
+    double d = 0.0;
+    if (surr_route(ov, rstar, s, t, sd, k, d)) {
+      const int row = acc.srow < 0 ? -1 : acc.srow + acc.sk;
+      ++acc.sk;
+      if (row >= 0 && row < sd.rows_cap) {
+        const float y0 = sd.y[2 * (size_t)row], y1 = sd.y[2 * (size_t)row + 1];
+        double al = (double)unscale_out(y0, sd.sc.out_lo[0], sd.sc.out_hi[0]);
+        if (polish_alpha(d, al)) {
+          const dv3 f = jkr_finish(al * s, ov, n, c, rstar, failed);
+          ++acc.sserved;
+          if (sd.check) {
+            int ef = 0;
+            const dv3 fe = jkr(ov, n, c, rstar, &ef);
+            const double fpull = c.jpull * rstar;  // -(2/3) pi gamma R*
+            const double scale = fmax(len(fe), -fpull);
+            const double err = len(sub(f, fe)) / scale;
+            double fraw = (double)unscale_out(y1, sd.sc.out_lo[1], sd.sc.out_hi[1]) * (kPi * c.gamma * rstar);
+            if (fraw < fpull) fraw = 0.0;
+            const double raw = fabs(fraw - dot(fe, n)) / scale;
+            acc.emax = fmaxf(acc.emax, (float)err);
+            acc.rmax = fmaxf(acc.rmax, (float)raw);
+            acc.esum += err;
+            acc.rsum += raw;
+          }
+          return f;
+        }
+      }
+      ++acc.sfall;
+    }
+  }
+  return jkr(ov, n, c, rstar, failed);
+}
 
 // One particle-particle contact of the gather (simulation.rs:357-460), added
 // `mult` times when the reference's grid walk reaches j more than once.
-template <int NORMAL, int TANG, bool RECORD>
-__device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T, const Own& o, int j, Acc& acc,
-                                  Ctl* ctl, unsigned long long* prow) {
+template <int NORMAL, int TANG, bool RECORD, int JS>
+__device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T, const Own& o, int j, int k, Acc& acc,
+                                  Ctl* ctl, unsigned long long* prow, const SurrDev& sd) {
   dv3 xj;
   double rj;
   ldxr(S.xr, j, xj, rj);
@@ -722,7 +779,7 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
   } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
     fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
-    fn = jkr(overlap, nrm, P.mc.pp, rstar, &jfail);
+    fn = jkr_fast<JS>(overlap, nrm, P.mc.pp, rstar, &jfail, k, sd, acc);
   } else {
     fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar);
   }
@@ -762,8 +819,9 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
 }
 
 // One particle-triangle contact (simulation.rs:465-523).
-template <int NORMAL>
-__device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& tr, Acc& acc, unsigned int& nwall) {
+template <int NORMAL, int JS>
+__device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& tr, Acc& acc, unsigned int& nwall,
+                                    const SurrDev& sd) {
   const dv3 cp = closest_point(tr, o.x);
   const dv3 dv = sub(o.x, cp);
   const double dsq = dot(dv, dv);
@@ -785,7 +843,8 @@ __device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& t
   } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
     fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * o.r, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
-    fn = jkr(pen, nrm, P.mc.pw, o.r, &jfail);
+    // walls take the exact solve (with the no-root shortcut when JS >= 1)
+    fn = jkr_fast<JS == 0 ? 0 : 1>(pen, nrm, P.mc.pw, o.r, &jfail, 64, sd, acc);
   } else {
     fn = sjkr(pen, nrm, vrel, P.mc.pw, o.r);
   }
@@ -822,13 +881,19 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // ADVANCE: also performs the next step's first half (begin_step) and leaves
 // the state drifted; otherwise stores a, tau and leaves the at-rest state.
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
-template <int NORMAL, int TANG, bool ADVANCE, bool RECORD>
+// JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
+// 2 with the network's radii from k_jkr_collect + the MLP kernel).
+template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS>
 __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long step, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
                                                   const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
-                                                  unsigned long long* pairbuf, int* paircount) {
+                                                  unsigned long long* pairbuf, int* paircount, SurrDev sd,
+                                                  SurrStats* sstats) {
   const int n = ctl->n;
+  // the rows of this step have been read by the MLP kernel; the next step's
+  // collect kernel starts from zero
+  if (JS == 2 && blockIdx.x == 0 && threadIdx.x == 0) *sd.counter = 0u;
   if (ctl->stall_from <= step || ctl->overflow) {
     // The lists may miss a pair for this step: keep the state as it is (the
     // host swaps the buffers after every launch, so copy it across) and let
@@ -836,6 +901,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) copy_slot(S, D, i, P.cap);
     if (threadIdx.x == 0) bstats[blockIdx.x] = BlockStats{0u, 0u, 0u, 0u};
+    if (JS == 2 && threadIdx.x == 0) sstats[blockIdx.x] = SurrStats{0u, 0u, 0.f, 0.f, 0.0, 0.0};
     return;
   }
   const int cap = P.cap;
@@ -848,6 +914,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
   acc.njkr = 0;
   acc.npair = 0;
   acc.overlap = false;
+  acc.srow = -1;
+  acc.sk = 0;
+  acc.sserved = 0;
+  acc.sfall = 0;
+  acc.emax = 0.f;
+  acc.rmax = 0.f;
+  acc.esum = 0.0;
+  acc.rsum = 0.0;
   bool far = false;
   if (i < n) {
     const uint8_t flags = T.flags[i];
@@ -869,6 +943,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
       o.rc1 = ref_coord<1>(o.x.y, P);
       o.rc2 = ref_coord<2>(o.x.z, P);
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
+      if (JS == 2) acc.srow = sd.base[i];
       for (int base = 0; base < cnt; base += kNbrBatch) {
         int jj[kNbrBatch];
         dv3 xq[kNbrBatch];
@@ -891,11 +966,11 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
           hit &= hit - 1;
           const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
                                        : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-          contact_pp<NORMAL, TANG, RECORD>(P, S, T, o, j, acc, ctl, prow);
+          contact_pp<NORMAL, TANG, RECORD, JS>(P, S, T, o, j, base + q, acc, ctl, prow, sd);
         }
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
-      for (uint32_t q = t0; q < t1; ++q) contact_wall<NORMAL>(P, o, tris[tri_idx[q]], acc, nwall);
+      for (uint32_t q = t0; q < t1; ++q) contact_wall<NORMAL, JS>(P, o, tris[tri_idx[q]], acc, nwall, sd);
 
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
@@ -959,9 +1034,125 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
     }
     bstats[blockIdx.x] = b;
   }
+  if (JS == 2) {
+    // surrogate counters: wave reductions, then one store per workgroup
+    unsigned int s1 = wave_sum_u32(acc.sserved), s2 = wave_sum_u32(acc.sfall);
+    float em = acc.emax, rm = acc.rmax;
+    double es = acc.esum, rs = acc.rsum;
+    for (int off = 32; off > 0; off >>= 1) {
+      em = fmaxf(em, __shfl_down(em, off, 64));
+      rm = fmaxf(rm, __shfl_down(rm, off, 64));
+      es += __shfl_down(es, off, 64);
+      rs += __shfl_down(rs, off, 64);
+    }
+    __shared__ SurrStats sred[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) sred[wv] = SurrStats{s1, s2, em, rm, es, rs};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      SurrStats b = sred[0];
+      for (int q = 1; q < kBlock / 64; ++q) {
+        b.served += sred[q].served;
+        b.fallback += sred[q].fallback;
+        b.err_max = fmaxf(b.err_max, sred[q].err_max);
+        b.raw_max = fmaxf(b.raw_max, sred[q].raw_max);
+        b.err_sum += sred[q].err_sum;
+        b.raw_sum += sred[q].raw_sum;
+      }
+      sstats[blockIdx.x] = b;
+    }
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->steps_done += 1;
     ctl->at_rest = ADVANCE ? 0 : 1;
+  }
+}
+
+// Routing test of the JKR surrogate for the list entry k of particle i,
+// with exactly the force kernel's pair tests and arithmetic (in_contact,
+// reference-grid visibility, overlap, R*), so both kernels agree on which
+// contacts consume a feature row.  Returns the row's scaled features.
+__device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o, int j, int k, const SurrDev& sd,
+                                    float4* feat) {
+  dv3 xj;
+  double rj;
+  ldxr(S.xr, j, xj, rj);
+  const dv3 dd = pair_delta(P, o.x, xj);
+  const double dsq = dot(dd, dd);
+  const double rsum = o.r + rj;
+  if (!in_contact(P, dsq, rsum)) return false;
+  const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+  if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
+    return false;
+  const int mult = __popc(ref_mask<0>(o.rc0, cj0, P)) * __popc(ref_mask<1>(o.rc1, cj1, P)) *
+                   __popc(ref_mask<2>(o.rc2, cj2, P));
+  if (mult == 0) return false;
+  const double overlap = rsum - sqrt(dsq);
+  const double rstar = (o.r * rj) / (o.r + rj);
+  const PairConsts& c = P.mc.pp;
+  if (!(c.gamma > 0.0)) return false;
+  const double s = cbrt(rstar * rstar * c.jc);
+  const double t = s * s / rstar;
+  if (overlap < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(overlap))) return false;
+  double d = 0.0;
+  if (!surr_route(overlap, rstar, s, t, sd, k, d)) return false;
+  if (feat) {
+    feat->x = scale_in((float)sqrt(d - kJkrDMin), sd.sc.in_lo[0], sd.sc.in_hi[0]);
+    feat->y = scale_in((float)log(rstar), sd.sc.in_lo[1], sd.sc.in_hi[1]);
+    feat->z = sd.f_lne;
+    feat->w = sd.f_lng;
+  }
+  return true;
+}
+
+// Gathers the feature rows of every contact the JKR surrogate serves in
+// this step (one thread per particle, the force kernel's list order): a first
+// pass marks the served list entries, a wave-wide prefix sum plus one atomic
+// per wave places each particle's rows contiguously, a second pass writes
+// them.  base[i] = -1 when i has none or the rows would exceed the buffer
+// (its contacts then take the exact solve).
+__global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long long step, KParams P, Dyn S, Stat T,
+                                                        const int* nbr_count, const int* nbr, SurrDev sd) {
+  if (ctl->stall_from <= step || ctl->overflow) return;  // the force launch keeps the state: no rows needed
+  const int n = ctl->n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  unsigned long long mask = 0ull;
+  Own o;
+  if (i < n && !(T.flags[i] & FLAG_DEAD)) {
+    ldxr(S.xr, i, o.x, o.r);
+    o.rc0 = ref_coord<0>(o.x.x, P);
+    o.rc1 = ref_coord<1>(o.x.y, P);
+    o.rc2 = ref_coord<2>(o.x.z, P);
+    const int kmax = min(nbr_count[i], 64);
+    for (int k = 0; k < kmax; ++k)
+      if (collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, nullptr)) mask |= 1ull << k;
+  }
+  const int c = __popcll(mask);
+  int incl = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  unsigned int wbase = 0u;
+  if (lane == 63 && total > 0) {
+    wbase = atomicAdd(sd.counter, (unsigned int)total);
+    atomicMax(&ctl->surr_rows_max, wbase + (unsigned int)total);
+  }
+  wbase = __shfl(wbase, 63, 64);
+  if (i >= n) return;
+  const long long b = (long long)wbase + (incl - c);
+  const bool fits = c > 0 && b + c <= (long long)sd.rows_cap;
+  sd.base[i] = fits ? (int)b : -1;
+  if (!fits) return;
+  float4* rows = reinterpret_cast<float4*>(sd.x);
+  int r = (int)b;
+  while (mask) {
+    const int k = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    float4 f;
+    (void)collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, &f);
+    rows[r++] = f;
   }
 }
 
@@ -1038,6 +1229,17 @@ struct rp_engine {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event); end = start + 1
+  // JKR solver (RP_JKR_*) and the surrogate's network and buffers
+  int jkr_solver = RP_JKR_EXACT;
+  int surr_check = 0;
+  rp_jkr_envelope surr_env{};
+  MlpHost surr_net;
+  int* d_surr_base = nullptr;  // [cap]
+  float* d_surr_x = nullptr;   // [surr_rows_cap x 4]
+  float* d_surr_y = nullptr;   // [surr_rows_cap x 2]
+  int surr_rows_cap = 0;
+  SurrStats* d_sstats = nullptr;  // [cap / kBlock + 1]
+  bool sstats_valid = false;     // the last force launch wrote d_sstats
 };
 
 namespace {
@@ -1048,21 +1250,8 @@ void free_ptr(void* p) {
 
 MaterialConsts material_consts(const rp_params& p) {
   auto pair = [](double e1, double nu1, double e2, double nu2, double g1, double g2) {
-    PairConsts c{};
     double inv = (1.0 - nu1 * nu1) / e1 + (1.0 - nu2 * nu2) / e2;
-    c.estar = 1.0 / inv;
-    c.k43e = 4.0 / 3.0 * c.estar;
-    c.twoe = 2.0 * c.estar;
-    c.gamma = (g1 + g2) * 0.5;
-    double e8pg = 8.0 * kPi * c.gamma;
-    c.jc = e8pg / c.estar;
-    c.jsqrtc = std::sqrt(c.jc);
-    c.j8pg_e = e8pg * c.estar;
-    c.jpull = -2.0 / 3.0 * kPi * c.gamma;
-    c.spull = 1.5 * kPi * c.gamma;
-    c.sterm = 3.0 * (kPi * kPi) * (c.gamma * c.gamma);
-    c.estar_sq = c.estar * c.estar;
-    return c;
+    return pair_consts(1.0 / inv, (g1 + g2) * 0.5);
   };
   const rp_material& pm = p.particle;
   const rp_material& wm = p.wall;
@@ -1315,6 +1504,11 @@ void free_particles(rp_engine* e) {
   free_ptr(e->d_paircount);
   free_ptr(e->d_bstats);
   e->d_bstats = nullptr;
+  free_ptr(e->d_surr_base);
+  free_ptr(e->d_sstats);
+  e->d_surr_base = nullptr;
+  e->d_sstats = nullptr;
+  e->sstats_valid = false;
   e->d_xb = nullptr;
   e->d_key = e->d_tmp = nullptr;
   e->d_nbr_count = e->d_nbr = nullptr;
@@ -1467,16 +1661,81 @@ void mark_end(rp_engine* e) {
   (void)hipEventRecord(e->ev_pool[e->ev_marks.back().second + 1], e->stream);
 }
 
-template <int NORMAL, int TANG, bool ADV, bool REC>
+// Device view of the surrogate state (null pointers when it is not in use).
+SurrDev surr_dev(rp_engine* e) {
+  SurrDev sd{};
+  if (e->jkr_solver != RP_JKR_SURROGATE) return sd;
+  const rp_jkr_envelope& v = e->surr_env;
+  sd.d_lo = v.d_lo;
+  sd.d_hi = v.d_hi;
+  sd.r_lo = v.rstar_lo;
+  sd.r_hi = v.rstar_hi;
+  const PairConsts& pp = e->kp.mc.pp;
+  auto inside = [](double x, double lo, double hi) { return x >= lo * (1.0 - 1e-12) && x <= hi * (1.0 + 1e-12); };
+  sd.pp_ok = inside(pp.estar, v.estar_lo, v.estar_hi) && inside(pp.gamma, v.gamma_lo, v.gamma_hi) ? 1 : 0;
+  sd.check = e->surr_check;
+  sd.sc = make_scale(v);
+  sd.f_lne = scale_in((float)std::log(pp.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
+  sd.f_lng = scale_in((float)std::log(std::max(pp.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
+  sd.base = e->d_surr_base;
+  sd.x = e->d_surr_x;
+  sd.y = e->d_surr_y;
+  sd.counter = &e->d_ctl->surr_rows;
+  sd.rows_cap = e->surr_rows_cap;
+  return sd;
+}
+
+rp_status alloc_surr_rows(rp_engine* e, int rows) {
+  free_ptr(e->d_surr_x);
+  free_ptr(e->d_surr_y);
+  e->d_surr_x = e->d_surr_y = nullptr;
+  e->surr_rows_cap = 0;
+  HIP_TRY(hipMalloc(&e->d_surr_x, (size_t)rows * 4 * sizeof(float)));
+  HIP_TRY(hipMalloc(&e->d_surr_y, (size_t)rows * 2 * sizeof(float)));
+  e->surr_rows_cap = rows;
+  return RP_OK;
+}
+
+bool surr_active(const rp_engine* e) { return e->jkr_solver == RP_JKR_SURROGATE && e->kp.normal == RP_NORMAL_JKR; }
+
+// Feature rows of the surrogate-served contacts of the next force launch,
+// then the network on them.
+void launch_surrogate(rp_engine* e) {
+  const SurrDev sd = surr_dev(e);
+  const int g = (int)((e->cap + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_jkr_collect, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, (unsigned long long)e->launch_no,
+                     e->kp, e->dyn[e->cur], e->stat[0], e->d_nbr_count, e->d_nbr, sd);
+  mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
+}
+
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS>
 void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
                      (unsigned long long)e->launch_no, e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
-                     e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount);
+                     e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
+                     JS == 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
   e->force_grid = g;
+  e->sstats_valid = JS == 2;
   e->cur ^= 1;
   e->launch_no++;
+}
+
+// JKR solver variants are compiled for the JKR model only; a recording
+// launch uses the exact solve with the shortcut (no surrogate rows).
+template <int NORMAL, int TANG, bool ADV, bool REC>
+void launch_force_j(rp_engine* e, int flag_slot) {
+  if constexpr (NORMAL != RP_NORMAL_JKR) {
+    launch_force_t<NORMAL, TANG, ADV, REC, 0>(e, flag_slot);
+  } else {
+    constexpr int kSurr = REC ? 1 : 2;
+    switch (e->jkr_solver) {
+      case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1>(e, flag_slot); break;
+      case RP_JKR_SURROGATE: launch_force_t<NORMAL, TANG, ADV, REC, kSurr>(e, flag_slot); break;
+      default: launch_force_t<NORMAL, TANG, ADV, REC, 0>(e, flag_slot); break;
+    }
+  }
 }
 
 // Tangential kinds compiled: Coulomb, and the spring law (LSC / Mindlin) with
@@ -1484,9 +1743,9 @@ void launch_force_t(rp_engine* e, int flag_slot) {
 template <int NORMAL, bool ADV, bool REC>
 void launch_force_n(rp_engine* e, int slot) {
   switch (e->kp.tangential) {
-    case RP_TANGENTIAL_COULOMB: launch_force_t<NORMAL, RP_TANGENTIAL_COULOMB, ADV, REC>(e, slot); break;
-    case RP_TANGENTIAL_LSC: launch_force_t<NORMAL, RP_TANGENTIAL_LSC, ADV, REC>(e, slot); break;
-    default: launch_force_t<NORMAL, RP_TANGENTIAL_MINDLIN, ADV, REC>(e, slot); break;
+    case RP_TANGENTIAL_COULOMB: launch_force_j<NORMAL, RP_TANGENTIAL_COULOMB, ADV, REC>(e, slot); break;
+    case RP_TANGENTIAL_LSC: launch_force_j<NORMAL, RP_TANGENTIAL_LSC, ADV, REC>(e, slot); break;
+    default: launch_force_j<NORMAL, RP_TANGENTIAL_MINDLIN, ADV, REC>(e, slot); break;
   }
 }
 
@@ -1592,6 +1851,9 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * sizeof(int)));
   HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
+  HIP_TRY(hipMalloc(&e->d_surr_base, (size_t)cap * sizeof(int)));
+  HIP_TRY(hipMalloc(&e->d_sstats, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
+  HIP_TRY(hipMemset(e->d_sstats, 0, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
   rp_status st = alloc_nbr(e, 32);
   if (st) return st;
   // host staging in the device layout: Dyn = xr records + 7 vec3 fields as
@@ -1719,6 +1981,11 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
       }
       const int slot = (int)(e->chain_no & 1);
       const bool last = (s + 1 == remaining);
+      if (surr_active(e) && !(last && e->record)) {
+        mark_begin(e, 3);
+        launch_surrogate(e);
+        mark_end(e);
+      }
       mark_begin(e, 0);
       if (last) {
         if (e->record) launch_force<false, true>(e, slot);
@@ -1735,6 +2002,12 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
     HIP_TRY(hipStreamSynchronize(e->stream));
     const uint64_t executed = e->host_ctl.steps_done - e->steps_done;
     e->steps_done = e->host_ctl.steps_done;
+    if (surr_active(e) && e->host_ctl.surr_rows_max > (unsigned)e->surr_rows_cap) {
+      // some particles' contacts took the exact solve for want of rows: grow
+      const size_t want = round_up((size_t)e->host_ctl.surr_rows_max * 3 / 2 + 1024, 64);
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      if ((st = alloc_surr_rows(e, (int)std::min<size_t>(want, (size_t)1 << 30)))) return st;
+    }
     e->at_rest = e->host_ctl.at_rest != 0;
     remaining -= executed;
     if (remaining == 0) break;
@@ -1932,6 +2205,57 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
   return RP_OK;
 }
 
+rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_envelope* env, const float* params,
+                                   int32_t check) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (solver < RP_JKR_EXACT || solver > RP_JKR_SURROGATE) return fail(RP_ERR_INVALID, "unknown JKR solver");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (solver == RP_JKR_SURROGATE) {
+    if (!env || !params) return fail(RP_ERR_INVALID, "the surrogate needs an envelope and parameters");
+    const char* why = "";
+    if (envelope_check(*env, &why) != RP_OK) return fail(RP_ERR_INVALID, why);
+    for (int k = 0; k < RP_MLP_PARAMS; ++k)
+      if (!std::isfinite(params[k])) return fail(RP_ERR_INVALID, "surrogate parameters must be finite");
+    if (mlp_upload(e->surr_net, params, e->stream) != RP_OK) return fail(RP_ERR_DEVICE, "surrogate weight upload failed");
+    e->surr_env = *env;
+    if (e->surr_rows_cap == 0) {
+      rp_status st = alloc_surr_rows(e, 1 << 16);
+      if (st) return st;
+    }
+  }
+  e->jkr_solver = solver;
+  e->surr_check = check ? 1 : 0;
+  e->sstats_valid = false;
+  return RP_OK;
+}
+
+rp_status rp_engine_jkr_stats(rp_engine* e, rp_jkr_stats* out) {
+  if (!e || !out) return fail(RP_ERR_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof(*out));
+  out->rows_capacity = (uint64_t)e->surr_rows_cap;
+  if (!e->have_particles) return RP_OK;
+  rp_status st = read_ctl(e);
+  if (st) return st;
+  out->rows_max = e->host_ctl.surr_rows_max;
+  if (!e->sstats_valid || e->force_grid <= 0) return RP_OK;
+  std::vector<SurrStats> ss((size_t)e->force_grid);
+  HIP_TRY(hipMemcpy(ss.data(), e->d_sstats, ss.size() * sizeof(SurrStats), hipMemcpyDeviceToHost));
+  double es = 0.0, rs = 0.0;
+  for (const SurrStats& b : ss) {
+    out->served += b.served;
+    out->fallback += b.fallback;
+    out->err_max = std::max(out->err_max, (double)b.err_max);
+    out->raw_max = std::max(out->raw_max, (double)b.raw_max);
+    es += b.err_sum;
+    rs += b.raw_sum;
+  }
+  if (out->served) {
+    out->err_mean = es / (double)out->served;
+    out->raw_mean = rs / (double)out->served;
+  }
+  return RP_OK;
+}
+
 rp_status rp_engine_set_timing(rp_engine* e, int on) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1941,18 +2265,18 @@ rp_status rp_engine_set_timing(rp_engine* e, int on) {
   return RP_OK;
 }
 
-rp_status rp_engine_timing(rp_engine* e, double* ms3, uint64_t* count3) {
-  if (!e || !ms3 || !count3) return fail(RP_ERR_INVALID, "NULL argument");
+rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count) {
+  if (!e || !ms || !count) return fail(RP_ERR_INVALID, "NULL argument");
   HIP_TRY(hipStreamSynchronize(e->stream));
-  for (int k = 0; k < 3; ++k) {
-    ms3[k] = 0.0;
-    count3[k] = 0;
+  for (int k = 0; k < RP_TIMING_KINDS; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
   }
   for (auto& mk : e->ev_marks) {
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e->ev_pool[mk.second], e->ev_pool[mk.second + 1]));
-    ms3[mk.first] += (double)ms;
-    count3[mk.first] += 1;
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e->ev_pool[mk.second], e->ev_pool[mk.second + 1]));
+    ms[mk.first] += (double)t;
+    count[mk.first] += 1;
   }
   e->ev_used = 0;
   e->ev_marks.clear();
@@ -1995,6 +2319,9 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_tri_start);
   free_ptr(e->d_tri_idx);
   free_ptr(e->d_ctl);
+  free_ptr(e->d_surr_x);
+  free_ptr(e->d_surr_y);
+  mlp_free(e->surr_net);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -76,6 +76,26 @@ struct PairConsts {
   double estar_sq;  // E*·E*                       physics.rs:256
 };
 
+// PairConsts from E* and the pair's surface energy (same operation order as
+// the reference's per-contact expressions).
+__host__ __device__ inline PairConsts pair_consts(double estar, double gamma) {
+  const double pi = 3.14159265358979323846;
+  PairConsts c{};
+  c.estar = estar;
+  c.k43e = 4.0 / 3.0 * estar;
+  c.twoe = 2.0 * estar;
+  c.gamma = gamma;
+  const double e8pg = 8.0 * pi * gamma;
+  c.jc = e8pg / estar;
+  c.jsqrtc = sqrt(c.jc);
+  c.j8pg_e = e8pg * estar;
+  c.jpull = -2.0 / 3.0 * pi * gamma;
+  c.spull = 1.5 * pi * gamma;
+  c.sterm = 3.0 * (pi * pi) * (gamma * gamma);
+  c.estar_sq = estar * estar;
+  return c;
+}
+
 struct MaterialConsts {
   PairConsts pp, pw;
   double mu;         // particle friction (walls use it too, simulation.rs:514)
@@ -108,7 +128,27 @@ __device__ inline dv3 linear_sd(double ov, dv3 n, dv3 vrel, double kn, double gn
 
 // physics.rs:132-219.  Returns the force; *failed = 1 when the residual
 // check rejects the Newton root (the reference's zero-force exit).
-__device__ inline dv3 jkr(double ov, dv3 n, const PairConsts& c, double rstar, int* failed) {
+// Residual check, pull-off clip and force of a JKR contact radius a
+// (physics.rs:191-219); shared by the exact solve and the surrogate path.
+__device__ inline dv3 jkr_finish(double a, double ov, dv3 n, const PairConsts& c, double rstar, int* failed) {
+  double ts = sqrt(c.jc * a);
+  double f = a * a / rstar - ts - ov;
+  if (fabs(f) > 1e-3 * rstar) {
+    *failed = 1;
+    return zero3();
+  }
+  if (a <= 0.0) return zero3();
+  double a3 = a * a * a;
+  double term1 = 4.0 * c.estar * a3 / (3.0 * rstar);
+  double term2 = sqrt(c.j8pg_e * a3);
+  double fm = term1 - term2;
+  double fpull = c.jpull * rstar;
+  if (fm < fpull) return zero3();
+  return smul(fm, n);
+}
+
+__device__ inline dv3 jkr(double ov, dv3 n, const PairConsts& c, double rstar, int* failed,
+                          double* a_out = nullptr) {
   if (c.gamma <= 0.0) {
     double kn = c.k43e * sqrt(rstar);
     double fm = ov > 0.0 ? kn * pow15(ov) : 0.0;
@@ -127,20 +167,27 @@ __device__ inline dv3 jkr(double ov, dv3 n, const PairConsts& c, double rstar, i
     a -= st;
     if (fabs(st) < 1e-6 * rstar) break;
   }
-  double ts = sqrt(c.jc * a);
-  double f = a * a / rstar - ts - ov;
-  if (fabs(f) > 1e-3 * rstar) {
-    *failed = 1;
-    return zero3();
-  }
-  if (a <= 0.0) return zero3();
-  double a3 = a * a * a;
-  double term1 = 4.0 * c.estar * a3 / (3.0 * rstar);
-  double term2 = sqrt(c.j8pg_e * a3);
-  double fm = term1 - term2;
-  double fpull = c.jpull * rstar;
-  if (fm < fpull) return zero3();
-  return smul(fm, n);
+  if (a_out) *a_out = a;
+  return jkr_finish(a, ov, n, c, rstar, failed);
+}
+
+// Minimum over a > 0 of a^2/R* - sqrt(C a) - delta, in units of
+// t = (R*^2 C)^(2/3) / R*, plus the overlap: min_alpha (alpha^2 - sqrt(alpha))
+// = 4^(-4/3) - 4^(-1/3) at alpha = 4^(-2/3).
+constexpr double kJkrDMin = -0.47247039371057744;
+
+// True when the Newton solve of jkr() cannot pass its residual check: the
+// residual a^2/R* - sqrt(C a) - delta is >= t kJkrDMin - delta for every
+// a > 0, so delta < t kJkrDMin - 1e-3 R* rejects every iterate.  The 1e-8
+// term covers the rounding of both evaluations (the iterates stay within
+// 1.5^20 of the start, so the residual's rounding is below 1e-9 of
+// R* + t + |delta|).  Such a contact has zero force and counts as a rejected
+// root, exactly as the 20 iterations would end.
+__device__ inline bool jkr_no_root(double ov, double rstar, const PairConsts& c) {
+  if (!(c.gamma > 0.0)) return false;
+  const double s = cbrt(rstar * rstar * c.jc);
+  const double t = s * s / rstar;
+  return ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov));
 }
 
 // physics.rs:234-325

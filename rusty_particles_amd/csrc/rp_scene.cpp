@@ -8,6 +8,7 @@
 // splitmix64.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -42,63 +43,119 @@ struct Rng {
 
 extern "C" {
 
-// Places up to n spheres; returns the number placed.  x3: n*3, r: n.
-long long rps_rsa(long long n, const double* lo, const double* hi, double rmin, double rmax, unsigned long long seed,
-                  long long max_attempts, double* x3, double* r) {
-  Rng rng(seed);
-  double cs = 2.0 * rmax;
+// Places up to n spheres with the given radii (radii[i], i < n; spheres are
+// placed in the order given) or, when radii is NULL, radii drawn uniformly in
+// [rmin, rmax]; periodic[k] != 0 wraps axis k (overlaps tested with the
+// minimum image).  Returns the number placed.  x3: n*3, r: n.
+//
+// Two uniform grids keep each test O(1) for wide radius distributions: spheres
+// with r > rthr go to a coarse grid (cell >= 2 rmax), the others to a fine
+// grid (cell >= 2 rthr); a candidate checks the cells of both grids that can
+// hold an overlapping sphere.
+namespace {
+struct SGrid {
   int dims[3];
-  for (int k = 0; k < 3; ++k) {
-    dims[k] = (int)std::ceil((hi[k] - lo[k]) / cs);
-    if (dims[k] < 1) dims[k] = 1;
-  }
-  size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
-  std::vector<long long> head(ncell, -1), next((size_t)n, -1);
-  auto cell_of = [&](const double* p, int c[3]) {
+  double w[3];
+  double lo[3];
+  int per[3];
+  std::vector<long long> head;
+  void init(const double* lo_, const double* hi_, const int* per_, double cell) {
     for (int k = 0; k < 3; ++k) {
-      int q = (int)std::floor((p[k] - lo[k]) / cs);
-      c[k] = q < 0 ? 0 : (q >= dims[k] ? dims[k] - 1 : q);
+      lo[k] = lo_[k];
+      per[k] = per_[k];
+      double L = hi_[k] - lo_[k];
+      dims[k] = (int)std::floor(L / cell);
+      if (dims[k] < 1) dims[k] = 1;
+      w[k] = L / dims[k];
     }
+    head.assign((size_t)dims[0] * dims[1] * dims[2], -1);
+  }
+  int coord(double x, int k) const {
+    int q = (int)std::floor((x - lo[k]) / w[k]);
+    if (per[k]) return ((q % dims[k]) + dims[k]) % dims[k];
+    return q < 0 ? 0 : (q >= dims[k] ? dims[k] - 1 : q);
+  }
+  size_t id(int a, int b, int c) const { return (size_t)a + (size_t)dims[0] * ((size_t)b + (size_t)dims[1] * c); }
+};
+}  // namespace
+
+long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin, double rmax, const double* radii,
+                   const int* periodic, unsigned long long seed, long long max_attempts, double* x3, double* r) {
+  Rng rng(seed);
+  const int per[3] = {periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
+  double L[3];
+  for (int k = 0; k < 3; ++k) L[k] = hi[k] - lo[k];
+  // threshold radius: the 99th percentile of the given radii (or rmax)
+  double rthr = rmax;
+  if (radii && n > 100) {
+    std::vector<double> tmp(radii, radii + n);
+    std::nth_element(tmp.begin(), tmp.begin() + (size_t)(0.99 * n), tmp.end());
+    rthr = std::max(tmp[(size_t)(0.99 * n)], 1e-300);
+  }
+  SGrid big, small;
+  big.init(lo, hi, per, 2.0 * rmax);
+  small.init(lo, hi, per, 2.0 * rthr);
+  std::vector<long long> next((size_t)n, -1);
+  auto overlaps = [&](const SGrid& g, const double* p, double ri, double reach) {
+    int c[3], span[3];
+    for (int k = 0; k < 3; ++k) {
+      c[k] = g.coord(p[k], k);
+      span[k] = (int)std::ceil(reach / g.w[k]);
+      if (per[k] && 2 * span[k] + 1 > g.dims[k]) span[k] = -1;  // visit every cell once
+    }
+    int a0[3], a1[3];
+    for (int k = 0; k < 3; ++k) {
+      if (span[k] < 0) { a0[k] = 0; a1[k] = g.dims[k] - 1; }
+      else if (per[k]) { a0[k] = c[k] - span[k]; a1[k] = c[k] + span[k]; }
+      else { a0[k] = std::max(0, c[k] - span[k]); a1[k] = std::min(g.dims[k] - 1, c[k] + span[k]); }
+    }
+    for (int qz = a0[2]; qz <= a1[2]; ++qz)
+      for (int qy = a0[1]; qy <= a1[1]; ++qy)
+        for (int qx = a0[0]; qx <= a1[0]; ++qx) {
+          int q[3] = {qx, qy, qz};
+          for (int k = 0; k < 3; ++k)
+            if (per[k]) q[k] = ((q[k] % g.dims[k]) + g.dims[k]) % g.dims[k];
+          for (long long j = g.head[g.id(q[0], q[1], q[2])]; j >= 0; j = next[j]) {
+            double d[3] = {p[0] - x3[3 * j], p[1] - x3[3 * j + 1], p[2] - x3[3 * j + 2]};
+            for (int k = 0; k < 3; ++k)
+              if (per[k]) {
+                if (d[k] > 0.5 * L[k]) d[k] -= L[k];
+                else if (d[k] < -0.5 * L[k]) d[k] += L[k];
+              }
+            double rs = ri + r[j];
+            if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rs * rs) return true;
+          }
+        }
+    return false;
   };
   long long placed = 0;
   for (long long i = 0; i < n; ++i) {
-    double ri = rmin + (rmax - rmin) * rng.uniform();
+    double ri = radii ? radii[i] : rmin + (rmax - rmin) * rng.uniform();
     bool ok = false;
     double p[3];
     for (long long att = 0; att < max_attempts && !ok; ++att) {
-      for (int k = 0; k < 3; ++k) p[k] = lo[k] + ri + (hi[k] - lo[k] - 2.0 * ri) * rng.uniform();
-      int c[3];
-      cell_of(p, c);
-      ok = true;
-      for (int dz = -1; dz <= 1 && ok; ++dz)
-        for (int dy = -1; dy <= 1 && ok; ++dy)
-          for (int dx = -1; dx <= 1 && ok; ++dx) {
-            int q[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
-            if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= dims[0] || q[1] >= dims[1] || q[2] >= dims[2]) continue;
-            size_t id = (size_t)q[0] + (size_t)dims[0] * ((size_t)q[1] + (size_t)dims[1] * q[2]);
-            for (long long j = head[id]; j >= 0; j = next[j]) {
-              double d0 = p[0] - x3[3 * j], d1 = p[1] - x3[3 * j + 1], d2 = p[2] - x3[3 * j + 2];
-              double rs = ri + r[j];
-              if (d0 * d0 + d1 * d1 + d2 * d2 < rs * rs) {
-                ok = false;
-                break;
-              }
-            }
-          }
+      for (int k = 0; k < 3; ++k)
+        p[k] = per[k] ? lo[k] + L[k] * rng.uniform() : lo[k] + ri + (hi[k] - lo[k] - 2.0 * ri) * rng.uniform();
+      ok = !overlaps(big, p, ri, ri + rmax) && !overlaps(small, p, ri, ri + rthr);
     }
     if (!ok) break;
     x3[3 * placed] = p[0];
     x3[3 * placed + 1] = p[1];
     x3[3 * placed + 2] = p[2];
     r[placed] = ri;
-    int c[3];
-    cell_of(p, c);
-    size_t id = (size_t)c[0] + (size_t)dims[0] * ((size_t)c[1] + (size_t)dims[1] * c[2]);
-    next[placed] = head[id];
-    head[id] = placed;
+    SGrid& g = ri > rthr ? big : small;
+    size_t id = g.id(g.coord(p[0], 0), g.coord(p[1], 1), g.coord(p[2], 2));
+    next[placed] = g.head[id];
+    g.head[id] = placed;
     ++placed;
   }
   return placed;
+}
+
+
+long long rps_rsa(long long n, const double* lo, const double* hi, double rmin, double rmax, unsigned long long seed,
+                  long long max_attempts, double* x3, double* r) {
+  return rps_rsa2(n, lo, hi, rmin, rmax, nullptr, nullptr, seed, max_attempts, x3, r);
 }
 
 }  // extern "C"

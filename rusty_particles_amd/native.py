@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
     L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
     L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
     L.rp_probe_copy.argtypes = [C.c_size_t, _dp]
+    L.rp_engine_set_jkr_solver.argtypes = [_eng, C.c_int32, C.POINTER(abi.RpJkrEnvelope), _fp, C.c_int32]
+    L.rp_engine_jkr_stats.argtypes = [_eng, C.POINTER(abi.RpJkrStats)]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -63,7 +65,7 @@ def lib() -> C.CDLL:
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
-                 "rp_engine_timing", "rp_probe_copy"):
+                 "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -203,11 +205,30 @@ class Engine:
 
     def timing(self) -> dict:
         """Summed HIP-event times (ms) and launch counts since the last call."""
-        ms = np.zeros(3)
-        cnt = (C.c_uint64 * 3)()
+        ms = np.zeros(abi.RP_TIMING_KINDS)
+        cnt = (C.c_uint64 * abi.RP_TIMING_KINDS)()
         _check(self._L.rp_engine_timing(self._h, _ptr(ms), cnt))
-        names = ("force", "rebuild", "begin")
-        return {names[k]: (float(ms[k]), int(cnt[k])) for k in range(3)}
+        names = ("force", "rebuild", "begin", "surrogate")
+        return {names[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.RP_TIMING_KINDS)}
+
+    def set_jkr_solver(self, solver: str, surrogate=None, check: bool = False) -> None:
+        """'exact' (the reference's Newton solve), 'shortcut' (identical
+        results, no iterations for contacts that cannot have a root) or
+        'surrogate' (needs a trained surrogate.JkrSurrogate)."""
+        code = abi.JKR_SOLVERS[solver]
+        env = params = None
+        if code == abi.RP_JKR_SURROGATE:
+            if surrogate is None:
+                raise ValueError("the surrogate solver needs a trained network")
+            env = C.byref(surrogate.env)
+            self._surr_params = np.ascontiguousarray(surrogate.params, dtype=np.float32)
+            params = self._surr_params.ctypes.data_as(C.POINTER(C.c_float))
+        _check(self._L.rp_engine_set_jkr_solver(self._h, code, env, params, 1 if check else 0))
+
+    def jkr_stats(self) -> dict:
+        s = abi.RpJkrStats()
+        _check(self._L.rp_engine_jkr_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in abi.RpJkrStats._fields_}
 
     def stats(self) -> dict:
         s = abi.RpStats()

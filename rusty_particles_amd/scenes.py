@@ -27,6 +27,9 @@ def _lib():
         dp = C.POINTER(C.c_double)
         L.rps_rsa.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, C.c_ulonglong, C.c_longlong, dp, dp]
         L.rps_rsa.restype = C.c_longlong
+        L.rps_rsa2.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, dp, C.POINTER(C.c_int),
+                               C.c_ulonglong, C.c_longlong, dp, dp]
+        L.rps_rsa2.restype = C.c_longlong
         _scene = L
     return _scene
 
@@ -40,6 +43,25 @@ def rsa(n: int, lo, hi, rmin: float, rmax: float, seed: int, max_attempts: int =
     dp = C.POINTER(C.c_double)
     placed = _lib().rps_rsa(n, lo.ctypes.data_as(dp), hi.ctypes.data_as(dp), rmin, rmax, seed, max_attempts,
                             x.ctypes.data_as(dp), r.ctypes.data_as(dp))
+    if placed < n:
+        raise RuntimeError(f"RSA placed only {placed} of {n} spheres")
+    return x, r
+
+
+def rsa_radii(radii, lo, hi, periodic, seed: int, max_attempts: int = 200000):
+    """Random sequential addition of spheres with the given radii (placed in
+    the order given), periodic axes wrapped."""
+    radii = np.ascontiguousarray(radii, dtype=np.float64)
+    n = radii.size
+    x = np.zeros((n, 3))
+    r = np.zeros(n)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    per = (C.c_int * 3)(*[1 if q else 0 for q in periodic])
+    dp = C.POINTER(C.c_double)
+    placed = _lib().rps_rsa2(n, lo.ctypes.data_as(dp), hi.ctypes.data_as(dp), 0.0, float(radii.max()),
+                             radii.ctypes.data_as(dp), per, seed, max_attempts, x.ctypes.data_as(dp),
+                             r.ctypes.data_as(dp))
     if placed < n:
         raise RuntimeError(f"RSA placed only {placed} of {n} spheres")
     return x, r
@@ -189,3 +211,28 @@ def box_settle_workload(n: int = 10_000, seed: int = 0xD3E50001):
     p.particle = abi.RpMaterial(1e7, 0.3, 2500.0, 0.5, 0.5, 0.0)
     v9, n3 = box_mesh((-0.15, -0.15, -0.15), (0.15, 0.15, 0.15))
     return p, x, r, m, [(v9, n3)]
+
+
+def jkr_periodic_workload(n: int = 2_000_000, seed: int = 0xD3E50003, phi: float = 0.35):
+    """2,000,000 spheres, radius lognormal (median 50 um, sigma 0.3), rho 1500,
+    E=1e8, nu=0.3, surface energy 0.05 J/m^2, Coulomb mu=0.3, JKR normal;
+    random loose packing at solid fraction `phi` in a periodic cube,
+    Gaussian velocities (std 0.05 m/s per component), dt=1e-8 s.  Radii are
+    placed largest first (random sequential addition)."""
+    rng = np.random.default_rng(seed)
+    radii = 50e-6 * np.exp(0.3 * rng.standard_normal(n))
+    vol = 4.0 / 3.0 * np.pi * float(np.sum(radii ** 3))
+    L = (vol / phi) ** (1.0 / 3.0)
+    order = np.argsort(-radii, kind="stable")
+    x, r = rsa_radii(radii[order], (0.0, 0.0, 0.0), (L, L, L), (1, 1, 1), seed)
+    perm = rng.permutation(n)  # API order independent of the placement order
+    x, r = x[perm], r[perm]
+    m = sphere_mass(r, 1500.0)
+    v = rng.normal(0.0, 0.05, (n, 3))
+    p = abi.default_params(1e-8, (0.0, 0.0, 0.0), (L, L, L))
+    for k in range(3):
+        p.periodic[k] = 1
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, 0.05)
+    return p, x, r, m, v

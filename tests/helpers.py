@@ -114,3 +114,35 @@ def assert_state_close(o, e, xtol, vtol, wtol):
     assert dv <= vtol, f"max |dv| {dv} > {vtol}"
     assert dw <= wtol, f"max |dw| {dw} > {wtol}"
     return dx, dv, dw
+
+
+def jkr_dimers(d_values, r0=50e-6, E=1e8, nu=0.3, gamma=0.05, rho=1500.0, seed=5, rspread=0.2, vel=0.0):
+    """Isolated particle pairs, pair k at scaled overlap d_values[k]
+    (delta = d t with s = (R*^2 8 pi gamma / E*)^(1/3), t = s^2 / R*), on a
+    lattice 12 r0 apart so no pair sees another (JKR cutoff 1.5 r_sum)."""
+    rng = np.random.default_rng(seed)
+    d_values = np.asarray(d_values, float)
+    npair = d_values.size
+    side = int(np.ceil(npair ** (1 / 3)))
+    idx = np.arange(npair)
+    centres = np.stack([idx % side, (idx // side) % side, idx // (side * side)], 1) * (12 * r0)
+    centres -= centres.mean(0)
+    ra = r0 * rng.uniform(1 - rspread, 1 + rspread, npair)
+    rb = r0 * rng.uniform(1 - rspread, 1 + rspread, npair)
+    estar = 1.0 / (2 * (1 - nu * nu) / E)
+    rstar = ra * rb / (ra + rb)
+    s = (rstar * rstar * 8 * np.pi * gamma / estar) ** (1 / 3)
+    t = s * s / rstar
+    delta = d_values * t
+    u = rng.normal(size=(npair, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    sep = (ra + rb - delta)[:, None]
+    x = np.empty((2 * npair, 3))
+    x[0::2] = centres - 0.5 * sep * u
+    x[1::2] = centres + 0.5 * sep * u
+    rad = np.empty(2 * npair)
+    rad[0::2], rad[1::2] = ra, rb
+    m = scenes.sphere_mass(rad, rho)
+    v = rng.normal(0.0, vel, (2 * npair, 3))
+    return dict(x=x, radius=rad, mass=m, v=v, w=np.zeros((2 * npair, 3)), fixed=np.zeros(2 * npair, np.uint8),
+                rstar=rstar, t=t, s=s, d=d_values)
