@@ -151,6 +151,9 @@ struct KParams {
   double lookahead_steps;  // steps between rebuild checks (see moved_too_far)
   double safe_half_skin;   // skin/2 minus the float rounding of the stored build positions
   double r_max;
+  // JKR: every contact with overlap below this (less 1e-8 |overlap|) has no
+  // root (jkr_no_root holds for any R* up to r_max / 2, resp. r_max for walls)
+  double jkr_far_pp, jkr_far_pw;
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
   int pair_cap;      // recorded pair entries per particle
@@ -695,8 +698,12 @@ struct Acc {
 // the predicted radius to the root the reference converges to.
 template <int JS>
 __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rstar, int* failed, int k,
-                               const SurrDev& sd, Acc& acc) {
+                               const SurrDev& sd, Acc& acc, double far) {
   if (JS == 0 || !(c.gamma > 0.0)) return jkr(ov, n, c, rstar, failed);
+  if (ov < far - 1e-8 * fabs(ov)) {
+    *failed = 1;
+    return zero3();
+  }
   const double s = cbrt(rstar * rstar * c.jc);
   const double t = s * s / rstar;
   if (ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov))) {
@@ -737,6 +744,26 @@ __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rst
   return jkr(ov, n, c, rstar, failed);
 }
 
+// Pair entries of one contact in the reference's 27-cell walk order (rank
+// oz*9 + oy*3 + ox of each reaching offset) for rp_engine_pairs.
+__device__ inline void record_pairs(const KParams& P, const Stat& T, int j, int mask0, int mask1, int mask2, Acc& acc,
+                                    Ctl* ctl, unsigned long long* prow) {
+  const unsigned long long sj = T.sid[j];
+  for (int oz = 0; oz < 3; ++oz) {
+    if (!(mask2 >> oz & 1)) continue;
+    for (int oy = 0; oy < 3; ++oy) {
+      if (!(mask1 >> oy & 1)) continue;
+      for (int ox = 0; ox < 3; ++ox) {
+        if (!(mask0 >> ox & 1)) continue;
+        const unsigned long long rank = (unsigned long long)(oz * 9 + oy * 3 + ox);
+        if (acc.npair < P.pair_cap) prow[acc.npair] = (rank << 32) | sj;
+        else atomicOr(&ctl->pair_overflow, 1);
+        ++acc.npair;
+      }
+    }
+  }
+}
+
 // One particle-particle contact of the gather (simulation.rs:357-460), added
 // `mult` times when the reference's grid walk reaches j more than once.
 template <int NORMAL, int TANG, bool RECORD, int JS>
@@ -759,6 +786,18 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
   const double dist = sqrt(dsq);
   const double overlap = rsum - dist;
   if (overlap > 0.2 * fmin(o.r, rj)) acc.overlap = true;
+  if (NORMAL == RP_NORMAL_JKR && JS >= 1 && TANG != RP_TANGENTIAL_MINDLIN && P.mc.pp.gamma > 0.0 &&
+      overlap < P.jkr_far_pp - 1e-8 * fabs(overlap)) {
+    // no root: zero normal force and a rejected root; Coulomb and the spring
+    // law with G* = 0 then give a zero tangential force, so the contact adds
+    // nothing (zeros of either sign do not change a sum that starts at +0 and
+    // is re-based with 0.0 + in the epilogue) and j's velocities are not read.
+    // Mindlin's sqrt(R* delta) is NaN for delta < 0 and must propagate.
+    acc.njkr += (unsigned int)mult;
+    acc.ncont += (unsigned int)mult;
+    if (RECORD) record_pairs(P, T, j, mask0, mask1, mask2, acc, ctl, prow);
+    return;
+  }
   const dv3 nrm = dist > 1e-6 ? divs(d, dist) : mk(0.0, 1.0, 0.0);
   const dv3 ra = mul(neg(nrm), o.r);
   const dv3 rb = mul(nrm, rj);
@@ -779,7 +818,7 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
   } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
     fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
-    fn = jkr_fast<JS>(overlap, nrm, P.mc.pp, rstar, &jfail, k, sd, acc);
+    fn = jkr_fast<JS>(overlap, nrm, P.mc.pp, rstar, &jfail, k, sd, acc, P.jkr_far_pp);
   } else {
     fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar);
   }
@@ -800,22 +839,7 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
     acc.t = add(acc.t, tc);
   }
   acc.ncont += (unsigned int)mult;
-  if (RECORD) {
-    const unsigned long long sj = T.sid[j];
-    for (int oz = 0; oz < 3; ++oz) {
-      if (!(mask2 >> oz & 1)) continue;
-      for (int oy = 0; oy < 3; ++oy) {
-        if (!(mask1 >> oy & 1)) continue;
-        for (int ox = 0; ox < 3; ++ox) {
-          if (!(mask0 >> ox & 1)) continue;
-          const unsigned long long rank = (unsigned long long)(oz * 9 + oy * 3 + ox);
-          if (acc.npair < P.pair_cap) prow[acc.npair] = (rank << 32) | sj;
-          else atomicOr(&ctl->pair_overflow, 1);
-          ++acc.npair;
-        }
-      }
-    }
-  }
+  if (RECORD) record_pairs(P, T, j, mask0, mask1, mask2, acc, ctl, prow);
 }
 
 // One particle-triangle contact (simulation.rs:465-523).
@@ -844,7 +868,7 @@ __device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& t
     fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * o.r, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
     // walls take the exact solve (with the no-root shortcut when JS >= 1)
-    fn = jkr_fast<JS == 0 ? 0 : 1>(pen, nrm, P.mc.pw, o.r, &jfail, 64, sd, acc);
+    fn = jkr_fast<JS == 0 ? 0 : 1>(pen, nrm, P.mc.pw, o.r, &jfail, 64, sd, acc, P.jkr_far_pw);
   } else {
     fn = sjkr(pen, nrm, vrel, P.mc.pw, o.r);
   }
@@ -1090,6 +1114,7 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
   const double rstar = (o.r * rj) / (o.r + rj);
   const PairConsts& c = P.mc.pp;
   if (!(c.gamma > 0.0)) return false;
+  if (overlap < P.jkr_far_pp - 1e-8 * fabs(overlap)) return false;
   const double s = cbrt(rstar * rstar * c.jc);
   const double t = s * s / rstar;
   if (overlap < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(overlap))) return false;
@@ -1310,6 +1335,15 @@ rp_status compute_geometry(rp_engine* e) {
   double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
   double rmean = e->r_mean > 0.0 ? e->r_mean : rmax;
   K.r_max = rmax;
+  // t(R*) = R*^(1/3) (8 pi gamma / E*)^(2/3) grows with R*, so the shortcut
+  // threshold of the largest possible R* bounds every contact's from below
+  auto far = [](const PairConsts& c, double rs) {
+    if (!(c.gamma > 0.0)) return -HUGE_VAL;
+    const double t = std::cbrt(rs) * std::pow(c.jc, 2.0 / 3.0);
+    return (kJkrDMin * t - 1e-3 * rs - 1e-8 * (rs + t)) * (1.0 + 1e-12);
+  };
+  K.jkr_far_pp = far(K.mc.pp, 0.5 * rmax);
+  K.jkr_far_pw = far(K.mc.pw, rmax);
   // automatic skin: a tenth of the largest radius, and wide enough that the
   // fastest uploaded particle cannot cover half of it between rebuild checks
   K.skin = p.skin > 0.0 ? p.skin : std::max(0.1 * rmax, 4.0 * (double)kRebuildInterval * e->v_max * p.dt);

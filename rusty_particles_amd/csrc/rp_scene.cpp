@@ -6,6 +6,8 @@
 // box (shrunk by the radius) until one overlaps no placed sphere; a uniform
 // grid of cell 2*rmax makes each test O(1).  RNG: xoshiro256** seeded by
 // splitmix64.
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdint>
 #include <algorithm>
@@ -79,6 +81,11 @@ struct SGrid {
 };
 }  // namespace
 
+static int g_progress = 0;
+// Progress lines on stderr (every tenth of the particles placed) from the
+// placement loop itself, for callers that run long placements unattended.
+void rps_set_progress(int on) { g_progress = on; }
+
 long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin, double rmax, const double* radii,
                    const int* periodic, unsigned long long seed, long long max_attempts, double* x3, double* r) {
   Rng rng(seed);
@@ -129,11 +136,19 @@ long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin,
     return false;
   };
   long long placed = 0;
+  long long attempts = 0;
+  const auto t_start = std::chrono::steady_clock::now();
   for (long long i = 0; i < n; ++i) {
+    if (g_progress && n >= 100000 && i > 0 && i % (n / 10) == 0) {
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+      std::fprintf(stderr, "[scene] RSA placed %lld of %lld (%lld attempts, %.1f s)\n", i, n, attempts, el);
+      std::fflush(stderr);
+    }
     double ri = radii ? radii[i] : rmin + (rmax - rmin) * rng.uniform();
     bool ok = false;
     double p[3];
     for (long long att = 0; att < max_attempts && !ok; ++att) {
+      ++attempts;
       for (int k = 0; k < 3; ++k)
         p[k] = per[k] ? lo[k] + L[k] * rng.uniform() : lo[k] + ri + (hi[k] - lo[k] - 2.0 * ri) * rng.uniform();
       ok = !overlaps(big, p, ri, ri + rmax) && !overlaps(small, p, ri, ri + rthr);

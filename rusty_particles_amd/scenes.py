@@ -30,6 +30,8 @@ def _lib():
         L.rps_rsa2.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, dp, C.POINTER(C.c_int),
                                C.c_ulonglong, C.c_longlong, dp, dp]
         L.rps_rsa2.restype = C.c_longlong
+        L.rps_set_progress.argtypes = [C.c_int]
+        L.rps_set_progress.restype = None
         _scene = L
     return _scene
 
@@ -59,6 +61,7 @@ def rsa_radii(radii, lo, hi, periodic, seed: int, max_attempts: int = 200000):
     hi = np.ascontiguousarray(hi, dtype=np.float64)
     per = (C.c_int * 3)(*[1 if q else 0 for q in periodic])
     dp = C.POINTER(C.c_double)
+    _lib().rps_set_progress(1 if n >= 500_000 else 0)
     placed = _lib().rps_rsa2(n, lo.ctypes.data_as(dp), hi.ctypes.data_as(dp), 0.0, float(radii.max()),
                              radii.ctypes.data_as(dp), per, seed, max_attempts, x.ctypes.data_as(dp),
                              r.ctypes.data_as(dp))

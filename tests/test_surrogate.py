@@ -106,3 +106,23 @@ def test_no_root_shortcut_threshold():
         delta = (D_MIN + 0.05) * t
         f, a, conv = osim.normal_law(64, abi.RP_NORMAL_JKR, delta, N_Y, N_Y * 0.0, E, R, gamma=g)
         assert conv == 1
+
+
+def test_far_prefilter_bounds_the_shortcut():
+    """The engine skips the per-contact cube root below the shortcut threshold
+    of the largest possible R* (r_max / 2): the threshold falls with R*, so
+    that bound implies the shortcut for every smaller R*."""
+    from rusty_particles_amd.surrogate import D_MIN
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        E, g = 10 ** rng.uniform(5, 9), 10 ** rng.uniform(-3, 0)
+        C = 8 * np.pi * g / E
+        rmax = 10 ** rng.uniform(-6, -2)
+        rm = 0.5 * rmax
+        tm = np.cbrt(rm) * C ** (2 / 3)
+        far = (D_MIN * tm - 1e-3 * rm - 1e-8 * (rm + tm)) * (1 + 1e-12)
+        R = rm * rng.uniform(1e-3, 1.0, 50)
+        t = np.cbrt(R * R * C) ** 2 / R
+        ov = far * rng.uniform(1.0, 50.0, 50)
+        ov = ov - 1e-8 * np.abs(ov)
+        assert np.all(ov < D_MIN * t - 1e-3 * R - 1e-8 * (R + t + np.abs(ov)))
