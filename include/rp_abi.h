@@ -198,6 +198,25 @@ rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3,
                                  double* rw3, double* radius, double* mass,
                                  uint8_t* fixed);
 
+/* Batched ensemble (no counterpart in the reference, where each Simulation
+ * is its own object, src/simulation.rs:35-60): ninst independent
+ * simulations that share this engine's parameters (dt, bounds, periodic
+ * axes, models, gravity, meshes) but have their own particle and wall
+ * materials, stepped together by every rp_engine_step -- one set of kernel
+ * launches over all instances' particles.  counts[k] particles of instance k
+ * follow those of instance k-1 in the arrays; wall may be NULL (the
+ * engine's wall material for all).  Particles of different instances never
+ * interact.  Each instance evolves exactly as a single engine given that
+ * instance alone.  Downloads list the instances in order, each in API
+ * order.  Not available with pair recording or the JKR surrogate. */
+rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* counts,
+                                 const rp_material* particle, const rp_material* wall,
+                                 const double* x3, const double* v3, const double* w3,
+                                 const double* radius, const double* mass, const uint8_t* fixed);
+
+/* Alive particles per instance (ninst entries; 1 for a single engine). */
+rp_status rp_engine_instance_counts(rp_engine* e, size_t* counts);
+
 /* Turns recording of the pair list on or off (off by default).  When on,
  * the last step of each rp_engine_step call records its contact pairs for
  * rp_engine_pairs. */

@@ -162,6 +162,8 @@ EXPORTS = (
     "rp_mfma_probe",
     "rp_surrogate_last_error",
     "rp_engine_set_jkr_solver",
+    "rp_engine_upload_batch",
+    "rp_engine_instance_counts",
     "rp_engine_jkr_stats",
     "rp_engine_destroy",
     "rp_last_error",

@@ -94,6 +94,13 @@ struct Stat {
   double* m;
   uint8_t* flags;
   uint32_t* sid;  // stable id = insertion index (API order key)
+  uint16_t* inst; // instance of a batched engine (0 for a single simulation)
+};
+
+// Per-instance constants of a batched engine (materials differ by instance).
+struct InstConsts {
+  MaterialConsts mc;
+  double jkr_far_pp, jkr_far_pw;
 };
 
 // Device-resident control block: slot count and rare flags.  Per-step
@@ -145,7 +152,10 @@ struct KParams {
   double ginv[3];    // 1 / cell size
   int gn[3];         // cells per axis
   int gstride[3];    // linear index strides
-  unsigned int ncells;
+  unsigned int ncells;      // fine cells of one instance
+  unsigned int ncells_all;  // ncells * ninst: sort buckets (instance-major)
+  int ninst;
+  const InstConsts* itab;   // [ninst] per-instance materials (batched engines)
   double skin;
   double half_skin;
   double lookahead_steps;  // steps between rebuild checks (see moved_too_far)
@@ -265,13 +275,13 @@ __global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, D
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t c;
     if (T.flags[i] & FLAG_DEAD) {
-      c = P.ncells;
+      c = P.ncells_all;
     } else {
       const dv3 x = ldx(S.xr, i);
       int cx = fine_coord(x.x, 0, P, false);
       int cy = fine_coord(x.y, 1, P, false);
       int cz = fine_coord(x.z, 2, P, false);
-      c = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+      c = (uint32_t)T.inst[i] * P.ncells + (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
     }
     key[i] = c;
     atomicAdd(&cell_count[c], 1u);
@@ -384,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   const int cap = P.cap;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t c = key[i];
-    if (c >= P.ncells) continue;
+    if (c >= P.ncells_all) continue;
     uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
     uint32_t my = T.sid[i];
     uint32_t rank = 0;
@@ -405,6 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
     U.m[d] = T.m[i];
     U.flags[d] = T.flags[i];
     U.sid[d] = my;
+    U.inst[d] = T.inst[i];
   }
 }
 
@@ -415,12 +426,13 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
                                                   const uint32_t* cell_start, int* nbr_count, int* nbr,
                                                   float* xb) {
   if (!ctl->rebuild[par]) return;
-  const int n = (int)cell_start[P.ncells];  // alive particles after compaction
+  const int n = (int)cell_start[P.ncells_all];  // alive particles after compaction
   const int cap = P.cap;
   int local_max = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const dv3 xi = ldx(D.xr, i);
     double ri = U.r[i];
+    const uint32_t ibase = (uint32_t)U.inst[i] * P.ncells;  // candidates come from i's own instance only
     int ci[3] = {fine_coord(xi.x, 0, P, false), fine_coord(xi.y, 1, P, false), fine_coord(xi.z, 2, P, false)};
     double lim_max = P.margin_factor * (ri + P.r_max) + P.skin;
     int lo[3], hi[3];
@@ -448,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
         for (int cx = lo[0]; cx <= hi[0]; ++cx) {
           int wx = cx;
           if (P.periodic[0] && !all[0]) wx = (cx % P.gn[0] + P.gn[0]) % P.gn[0];
-          uint32_t c = (uint32_t)(wx * P.gstride[0] + wy * P.gstride[1] + wz * P.gstride[2]);
+          uint32_t c = ibase + (uint32_t)(wx * P.gstride[0] + wy * P.gstride[1] + wz * P.gstride[2]);
           uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
           for (uint32_t j = s0; j < s1; ++j) {
             if ((int)j == i) continue;
@@ -510,6 +522,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
     V.m[i] = U.m[i];
     V.flags[i] = U.flags[i];
     V.sid[i] = U.sid[i];
+    V.inst[i] = U.inst[i];
   }
   const int wmax = wave_max_i32(local_max);
   if ((threadIdx.x & 63) == 0 && wmax > 0) {
@@ -767,8 +780,9 @@ __device__ inline void record_pairs(const KParams& P, const Stat& T, int j, int 
 // One particle-particle contact of the gather (simulation.rs:357-460), added
 // `mult` times when the reference's grid walk reaches j more than once.
 template <int NORMAL, int TANG, bool RECORD, int JS>
-__device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T, const Own& o, int j, int k, Acc& acc,
-                                  Ctl* ctl, unsigned long long* prow, const SurrDev& sd) {
+__device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, double far_pp, const Dyn& S,
+                                  const Stat& T, const Own& o, int j, int k, Acc& acc, Ctl* ctl,
+                                  unsigned long long* prow, const SurrDev& sd) {
   dv3 xj;
   double rj;
   ldxr(S.xr, j, xj, rj);
@@ -786,8 +800,8 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
   const double dist = sqrt(dsq);
   const double overlap = rsum - dist;
   if (overlap > 0.2 * fmin(o.r, rj)) acc.overlap = true;
-  if (NORMAL == RP_NORMAL_JKR && JS >= 1 && TANG != RP_TANGENTIAL_MINDLIN && P.mc.pp.gamma > 0.0 &&
-      overlap < P.jkr_far_pp - 1e-8 * fabs(overlap)) {
+  if (NORMAL == RP_NORMAL_JKR && JS >= 1 && TANG != RP_TANGENTIAL_MINDLIN && mc.pp.gamma > 0.0 &&
+      overlap < far_pp - 1e-8 * fabs(overlap)) {
     // no root: zero normal force and a rejected root; Coulomb and the spring
     // law with G* = 0 then give a zero tangential force, so the contact adds
     // nothing (zeros of either sign do not change a sum that starts at +0 and
@@ -810,27 +824,27 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
   dv3 fn;
   int jfail = 0;
   if (NORMAL == RP_NORMAL_HERTZ) {
-    fn = hertz(overlap, nrm, vrel, P.mc.pp, rstar, mstar, P.mc.beta_abs);
+    fn = hertz(overlap, nrm, vrel, mc.pp, rstar, mstar, mc.beta_abs);
   } else if (NORMAL == RP_NORMAL_LINEAR) {
-    const double kn = P.mc.pp.estar * rstar;
+    const double kn = mc.pp.estar * rstar;
     const double gn = 0.5 * sqrt(mstar * kn);
     fn = linear_sd(overlap, nrm, vrel, kn, gn);
   } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
-    fn = linear_sd(overlap, nrm, zero3(), P.mc.pp.estar * rstar, 0.0);
+    fn = linear_sd(overlap, nrm, zero3(), mc.pp.estar * rstar, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
-    fn = jkr_fast<JS>(overlap, nrm, P.mc.pp, rstar, &jfail, k, sd, acc, P.jkr_far_pp);
+    fn = jkr_fast<JS>(overlap, nrm, mc.pp, rstar, &jfail, k, sd, acc, far_pp);
   } else {
-    fn = sjkr(overlap, nrm, vrel, P.mc.pp, rstar);
+    fn = sjkr(overlap, nrm, vrel, mc.pp, rstar);
   }
   acc.njkr += (unsigned int)(jfail * mult);
   const double fnmag = len(fn);
   dv3 ft;
   if (TANG == RP_TANGENTIAL_COULOMB) {
-    ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+    ft = coulomb(fnmag, vrel, nrm, mc.mu);
   } else {
     // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
     const double kt = TANG == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
-    ft = spring_coulomb_g0(fnmag, kt, P.mc.mu);
+    ft = spring_coulomb_g0(fnmag, kt, mc.mu);
   }
   const dv3 fc = add(fn, ft);
   const dv3 tc = cross(ra, fc);
@@ -844,8 +858,8 @@ __device__ inline void contact_pp(const KParams& P, const Dyn& S, const Stat& T,
 
 // One particle-triangle contact (simulation.rs:465-523).
 template <int NORMAL, int JS>
-__device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& tr, Acc& acc, unsigned int& nwall,
-                                    const SurrDev& sd) {
+__device__ inline void contact_wall(const KParams& P, const MaterialConsts& mc, double far_pw, const Own& o,
+                                    const Tri& tr, Acc& acc, unsigned int& nwall, const SurrDev& sd) {
   const dv3 cp = closest_point(tr, o.x);
   const dv3 dv = sub(o.x, cp);
   const double dsq = dot(dv, dv);
@@ -859,22 +873,22 @@ __device__ inline void contact_wall(const KParams& P, const Own& o, const Tri& t
   dv3 fn;
   int jfail = 0;
   if (NORMAL == RP_NORMAL_HERTZ) {
-    fn = hertz(pen, nrm, vrel, P.mc.pw, o.r, o.m, P.mc.beta_abs);
+    fn = hertz(pen, nrm, vrel, mc.pw, o.r, o.m, mc.beta_abs);
   } else if (NORMAL == RP_NORMAL_LINEAR) {
-    const double kn = P.mc.pw.estar * o.r;
+    const double kn = mc.pw.estar * o.r;
     const double gn = 0.5 * sqrt(o.m * kn);
     fn = linear_sd(pen, nrm, vrel, kn, gn);
   } else if (NORMAL == RP_NORMAL_HYSTERETIC) {
-    fn = linear_sd(pen, nrm, vrel, P.mc.pw.estar * o.r, 0.0);
+    fn = linear_sd(pen, nrm, vrel, mc.pw.estar * o.r, 0.0);
   } else if (NORMAL == RP_NORMAL_JKR) {
     // walls take the exact solve (with the no-root shortcut when JS >= 1)
-    fn = jkr_fast<JS == 0 ? 0 : 1>(pen, nrm, P.mc.pw, o.r, &jfail, 64, sd, acc, P.jkr_far_pw);
+    fn = jkr_fast<JS == 0 ? 0 : 1>(pen, nrm, mc.pw, o.r, &jfail, 64, sd, acc, far_pw);
   } else {
-    fn = sjkr(pen, nrm, vrel, P.mc.pw, o.r);
+    fn = sjkr(pen, nrm, vrel, mc.pw, o.r);
   }
   acc.njkr += (unsigned int)jfail;
   const double fnmag = len(fn);
-  const dv3 ft = coulomb(fnmag, vrel, nrm, P.mc.mu);
+  const dv3 ft = coulomb(fnmag, vrel, nrm, mc.mu);
   const dv3 fc = add(fn, ft);
   acc.f = add(acc.f, fc);
   acc.t = add(acc.t, cross(ra, fc));
@@ -907,7 +921,7 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
-template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS>
+template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long step, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
@@ -968,6 +982,10 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
       o.rc2 = ref_coord<2>(o.x.z, P);
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
       if (JS == 2) acc.srow = sd.base[i];
+      const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
+      const MaterialConsts& mc = BATCH ? ic->mc : P.mc;
+      const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
+      const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
       for (int base = 0; base < cnt; base += kNbrBatch) {
         int jj[kNbrBatch];
         dv3 xq[kNbrBatch];
@@ -990,11 +1008,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
           hit &= hit - 1;
           const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
                                        : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-          contact_pp<NORMAL, TANG, RECORD, JS>(P, S, T, o, j, base + q, acc, ctl, prow, sd);
+          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
         }
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
-      for (uint32_t q = t0; q < t1; ++q) contact_wall<NORMAL, JS>(P, o, tris[tri_idx[q]], acc, nwall, sd);
+      for (uint32_t q = t0; q < t1; ++q)
+        contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
 
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
@@ -1254,6 +1273,11 @@ struct rp_engine {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event); end = start + 1
+  // batched engine: instances, their materials, the device table
+  int ninst = 1;
+  std::vector<rp_material> inst_particle, inst_wall;
+  std::vector<size_t> inst_count;  // uploaded particles per instance
+  InstConsts* d_itab = nullptr;
   // JKR solver (RP_JKR_*) and the surrogate's network and buffers
   int jkr_solver = RP_JKR_EXACT;
   int surr_check = 0;
@@ -1273,13 +1297,11 @@ void free_ptr(void* p) {
   if (p) (void)hipFree(p);
 }
 
-MaterialConsts material_consts(const rp_params& p) {
+MaterialConsts material_consts(const rp_material& pm, const rp_material& wm) {
   auto pair = [](double e1, double nu1, double e2, double nu2, double g1, double g2) {
     double inv = (1.0 - nu1 * nu1) / e1 + (1.0 - nu2 * nu2) / e2;
     return pair_consts(1.0 / inv, (g1 + g2) * 0.5);
   };
-  const rp_material& pm = p.particle;
-  const rp_material& wm = p.wall;
   MaterialConsts m{};
   m.pp = pair(pm.youngs_modulus, pm.poissons_ratio, pm.youngs_modulus, pm.poissons_ratio, pm.surface_energy,
               pm.surface_energy);
@@ -1290,6 +1312,14 @@ MaterialConsts material_consts(const rp_params& p) {
   double beta = lne / std::sqrt(lne * lne + kPi * kPi);
   m.beta_abs = std::fabs(beta);
   return m;
+}
+
+// t(R*) = R*^(1/3) (8 pi gamma / E*)^(2/3) grows with R*, so the shortcut
+// threshold of the largest possible R* bounds every contact's from below.
+double jkr_far(const PairConsts& c, double rs) {
+  if (!(c.gamma > 0.0)) return -HUGE_VAL;
+  const double t = std::cbrt(rs) * std::pow(c.jc, 2.0 / 3.0);
+  return (kJkrDMin * t - 1e-3 * rs - 1e-8 * (rs + t)) * (1.0 + 1e-12);
 }
 
 rp_status validate(const rp_params* p) {
@@ -1326,7 +1356,7 @@ rp_status compute_geometry(rp_engine* e) {
   }
   K.normal = p.normal_model;
   K.tangential = p.tangential_model;
-  K.mc = material_consts(p);
+  K.mc = material_consts(p.particle, p.wall);
   K.margin_factor = (p.normal_model == RP_NORMAL_JKR || p.normal_model == RP_NORMAL_SJKR) ? 1.5 : 1.0;
   K.ref_cs = p.ref_cell_size;
   for (int k = 0; k < 3; ++k) K.ref_dims[k] = sat_i32(std::ceil(K.bsize[k] / K.ref_cs)) + 1;
@@ -1337,13 +1367,22 @@ rp_status compute_geometry(rp_engine* e) {
   K.r_max = rmax;
   // t(R*) = R*^(1/3) (8 pi gamma / E*)^(2/3) grows with R*, so the shortcut
   // threshold of the largest possible R* bounds every contact's from below
-  auto far = [](const PairConsts& c, double rs) {
-    if (!(c.gamma > 0.0)) return -HUGE_VAL;
-    const double t = std::cbrt(rs) * std::pow(c.jc, 2.0 / 3.0);
-    return (kJkrDMin * t - 1e-3 * rs - 1e-8 * (rs + t)) * (1.0 + 1e-12);
-  };
-  K.jkr_far_pp = far(K.mc.pp, 0.5 * rmax);
-  K.jkr_far_pw = far(K.mc.pw, rmax);
+  K.jkr_far_pp = jkr_far(K.mc.pp, 0.5 * rmax);
+  K.jkr_far_pw = jkr_far(K.mc.pw, rmax);
+  // per-instance constants of a batched engine
+  if (e->ninst > 1) {
+    std::vector<InstConsts> tab((size_t)e->ninst);
+    for (int q = 0; q < e->ninst; ++q) {
+      tab[q].mc = material_consts(e->inst_particle[q], e->inst_wall[q]);
+      tab[q].jkr_far_pp = jkr_far(tab[q].mc.pp, 0.5 * rmax);
+      tab[q].jkr_far_pw = jkr_far(tab[q].mc.pw, rmax);
+    }
+    free_ptr(e->d_itab);
+    e->d_itab = nullptr;
+    HIP_TRY(hipMalloc(&e->d_itab, tab.size() * sizeof(InstConsts)));
+    HIP_TRY(hipMemcpy(e->d_itab, tab.data(), tab.size() * sizeof(InstConsts), hipMemcpyHostToDevice));
+  }
+  K.itab = e->d_itab;
   // automatic skin: a tenth of the largest radius, and wide enough that the
   // fastest uploaded particle cannot cover half of it between rebuild checks
   K.skin = p.skin > 0.0 ? p.skin : std::max(0.1 * rmax, 4.0 * (double)kRebuildInterval * e->v_max * p.dt);
@@ -1385,7 +1424,7 @@ rp_status compute_geometry(rp_engine* e) {
       }
       total *= (double)K.gn[k];
     }
-    if (total <= kMaxCells) break;
+    if (total * (double)std::max(e->ninst, 1) <= kMaxCells) break;
     cs *= 1.25;
   }
   // slowest-varying axis = the one with most cells
@@ -1397,6 +1436,8 @@ rp_status compute_geometry(rp_engine* e) {
     stride *= K.gn[order[q]];
   }
   K.ncells = (unsigned int)stride;
+  K.ninst = std::max(e->ninst, 1);
+  K.ncells_all = K.ncells * (unsigned int)K.ninst;
   // build positions are stored as float: |x - float(x)| <= |x| 2^-24 per axis
   double xmax = 0.0;
   for (int k = 0; k < 3; ++k)
@@ -1491,7 +1532,8 @@ rp_status upload_meshes_binned(rp_engine* e) {
 size_t dyn_bytes(int cap) { return round_up((size_t)cap * 25 * sizeof(double), 256); }
 size_t stat_bytes(int cap) {
   size_t c = (size_t)cap;
-  return round_up(c * 2 * sizeof(double), 256) + round_up(c, 256) + round_up(c * sizeof(uint32_t), 256);
+  return round_up(c * 2 * sizeof(double), 256) + round_up(c, 256) + round_up(c * sizeof(uint32_t), 256) +
+         round_up(c * sizeof(uint16_t), 256);
 }
 
 Dyn carve_dyn(void* mem, int cap) {
@@ -1519,6 +1561,8 @@ Stat carve_stat(void* mem, int cap) {
   s.flags = (uint8_t*)p;
   p += round_up(c, 256);
   s.sid = (uint32_t*)p;
+  p += round_up(c * sizeof(uint32_t), 256);
+  s.inst = (uint16_t*)p;
   return s;
 }
 
@@ -1552,7 +1596,7 @@ void free_particles(rp_engine* e) {
 }
 
 rp_status alloc_cells(rp_engine* e) {
-  size_t need = (size_t)e->kp.ncells + 2;
+  size_t need = (size_t)e->kp.ncells_all + 2;
   if (need <= e->cells_alloc) return RP_OK;
   free_ptr(e->d_cell_count);
   free_ptr(e->d_cell_start);
@@ -1591,7 +1635,7 @@ rp_status launch_rebuild(rp_engine* e) {
   e->chain_no++;
   const KParams& K = e->kp;
   int g = grid_for((size_t)e->cap);
-  int m = (int)K.ncells + 1;  // buckets incl. removed sentinel
+  int m = (int)K.ncells_all + 1;  // buckets incl. removed sentinel
   int nb = (m + kScanChunk - 1) / kScanChunk;
   int gs = std::min(nb, kMaxGrid);
   const Dyn W = e->dyn[e->cur], D = e->dyn[e->cur ^ 1];
@@ -1742,10 +1786,10 @@ void launch_surrogate(rp_engine* e) {
   mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
 }
 
-template <int NORMAL, int TANG, bool ADV, bool REC, int JS>
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
 void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
                      (unsigned long long)e->launch_no, e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
@@ -1758,16 +1802,30 @@ void launch_force_t(rp_engine* e, int flag_slot) {
 
 // JKR solver variants are compiled for the JKR model only; a recording
 // launch uses the exact solve with the shortcut (no surrogate rows).
+// Batched engines (several instances) use the per-instance material table;
+// they have no pair recording and no surrogate.
 template <int NORMAL, int TANG, bool ADV, bool REC>
 void launch_force_j(rp_engine* e, int flag_slot) {
+  if constexpr (!REC) {
+    if (e->ninst > 1) {
+      if constexpr (NORMAL == RP_NORMAL_JKR) {
+        if (e->jkr_solver != RP_JKR_EXACT) {
+          launch_force_t<NORMAL, TANG, ADV, false, 1, true>(e, flag_slot);
+          return;
+        }
+      }
+      launch_force_t<NORMAL, TANG, ADV, false, 0, true>(e, flag_slot);
+      return;
+    }
+  }
   if constexpr (NORMAL != RP_NORMAL_JKR) {
-    launch_force_t<NORMAL, TANG, ADV, REC, 0>(e, flag_slot);
+    launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot);
   } else {
     constexpr int kSurr = REC ? 1 : 2;
     switch (e->jkr_solver) {
-      case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1>(e, flag_slot); break;
-      case RP_JKR_SURROGATE: launch_force_t<NORMAL, TANG, ADV, REC, kSurr>(e, flag_slot); break;
-      default: launch_force_t<NORMAL, TANG, ADV, REC, 0>(e, flag_slot); break;
+      case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1, false>(e, flag_slot); break;
+      case RP_JKR_SURROGATE: launch_force_t<NORMAL, TANG, ADV, REC, kSurr, false>(e, flag_slot); break;
+      default: launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot); break;
     }
   }
 }
@@ -1864,11 +1922,15 @@ rp_status rp_engine_set_params(rp_engine* e, const rp_params* params) {
   return RP_OK;
 }
 
-rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const double* v3, const double* w3,
-                                const double* a3, const double* tau3, const double* rx3, const double* rv3,
-                                const double* rw3, const double* radius, const double* mass,
-                                const uint8_t* fixed) {
-  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+}  // extern "C"
+
+namespace {
+
+// Shared by the single and the batched upload: inst (NULL = all 0) gives
+// each particle's instance.
+rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3, const double* w3, const double* a3,
+                      const double* tau3, const double* rx3, const double* rv3, const double* rw3,
+                      const double* radius, const double* mass, const uint8_t* fixed, const uint16_t* inst) {
   if (n > 0 && (!x3 || !radius || !mass)) return fail(RP_ERR_INVALID, "x3, radius and mass are required");
   if (n > (size_t)1 << 30) return fail(RP_ERR_INVALID, "too many particles");
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1935,6 +1997,9 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   HIP_TRY(hipMemcpy(s0.r, stath.data(), stath.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.flags, fl.data(), (size_t)cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s0.sid, sid.data(), (size_t)cap * sizeof(uint32_t), hipMemcpyHostToDevice));
+  std::vector<uint16_t> ins((size_t)cap, 0);
+  if (inst) std::copy(inst, inst + n, ins.begin());
+  HIP_TRY(hipMemcpy(s0.inst, ins.data(), (size_t)cap * sizeof(uint16_t), hipMemcpyHostToDevice));
   e->dyn[0] = d0;
   e->dyn[1] = carve_dyn(e->d_dyn_mem[1], cap);
   e->stat[0] = s0;
@@ -1955,6 +2020,62 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
   e->have_particles = true;
   e->geometry_dirty = true;
   e->pairs_valid = false;
+  return RP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const double* v3, const double* w3,
+                                const double* a3, const double* tau3, const double* rx3, const double* rv3,
+                                const double* rw3, const double* radius, const double* mass,
+                                const uint8_t* fixed) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  e->ninst = 1;
+  e->inst_particle.clear();
+  e->inst_wall.clear();
+  e->inst_count.assign(1, n);
+  return upload_impl(e, n, x3, v3, w3, a3, tau3, rx3, rv3, rw3, radius, mass, fixed, nullptr);
+}
+
+rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* counts, const rp_material* particle,
+                                 const rp_material* wall, const double* x3, const double* v3, const double* w3,
+                                 const double* radius, const double* mass, const uint8_t* fixed) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (ninst < 1 || ninst > 65535) return fail(RP_ERR_INVALID, "ninst must lie in [1, 65535]");
+  if (!counts || !particle) return fail(RP_ERR_INVALID, "counts and particle materials are required");
+  if (e->record && ninst > 1) return fail(RP_ERR_STATE, "pair recording is not available for batched engines");
+  if (e->jkr_solver == RP_JKR_SURROGATE && ninst > 1)
+    return fail(RP_ERR_STATE, "the JKR surrogate is not available for batched engines");
+  size_t n = 0;
+  for (int q = 0; q < ninst; ++q) n += counts[q];
+  std::vector<uint16_t> inst(n);
+  size_t o = 0;
+  for (int q = 0; q < ninst; ++q)
+    for (size_t k = 0; k < counts[q]; ++k) inst[o++] = (uint16_t)q;
+  e->ninst = ninst;
+  e->inst_particle.assign(particle, particle + ninst);
+  e->inst_wall.assign((size_t)ninst, e->prm.wall);
+  if (wall) e->inst_wall.assign(wall, wall + ninst);
+  e->inst_count.assign(counts, counts + ninst);
+  return upload_impl(e, n, x3, v3, w3, nullptr, nullptr, nullptr, nullptr, nullptr, radius, mass, fixed,
+                     inst.data());
+}
+
+rp_status rp_engine_instance_counts(rp_engine* e, size_t* counts) {
+  if (!e || !counts) return fail(RP_ERR_INVALID, "NULL argument");
+  for (int q = 0; q < e->ninst; ++q) counts[q] = 0;
+  if (!e->have_particles) return RP_OK;
+  rp_status st = read_ctl(e);
+  if (st) return st;
+  const int n = e->host_ctl.n;
+  std::vector<uint16_t> ins((size_t)n);
+  std::vector<uint8_t> fl((size_t)n);
+  HIP_TRY(hipMemcpy(ins.data(), e->stat[0].inst, (size_t)n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(fl.data(), e->stat[0].flags, (size_t)n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i)
+    if (!(fl[i] & FLAG_DEAD)) counts[ins[i]]++;
   return RP_OK;
 }
 
@@ -1985,6 +2106,7 @@ rp_status rp_engine_add_mesh_stl(rp_engine* e, const char* path) {
 
 rp_status rp_engine_set_record(rp_engine* e, int on) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (on && e->ninst > 1) return fail(RP_ERR_STATE, "pair recording is not available for batched engines");
   e->record = on != 0;
   return RP_OK;
 }
@@ -2245,6 +2367,7 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
   if (solver < RP_JKR_EXACT || solver > RP_JKR_SURROGATE) return fail(RP_ERR_INVALID, "unknown JKR solver");
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (solver == RP_JKR_SURROGATE) {
+    if (e->ninst > 1) return fail(RP_ERR_STATE, "the JKR surrogate is not available for batched engines");
     if (!env || !params) return fail(RP_ERR_INVALID, "the surrogate needs an envelope and parameters");
     const char* why = "";
     if (envelope_check(*env, &why) != RP_OK) return fail(RP_ERR_INVALID, why);
@@ -2353,6 +2476,7 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_tri_start);
   free_ptr(e->d_tri_idx);
   free_ptr(e->d_ctl);
+  free_ptr(e->d_itab);
   free_ptr(e->d_surr_x);
   free_ptr(e->d_surr_y);
   mlp_free(e->surr_net);

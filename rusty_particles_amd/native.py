@@ -57,6 +57,9 @@ def lib() -> C.CDLL:
     L.rp_probe_copy.argtypes = [C.c_size_t, _dp]
     L.rp_engine_set_jkr_solver.argtypes = [_eng, C.c_int32, C.POINTER(abi.RpJkrEnvelope), _fp, C.c_int32]
     L.rp_engine_jkr_stats.argtypes = [_eng, C.POINTER(abi.RpJkrStats)]
+    L.rp_engine_upload_batch.argtypes = [_eng, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(abi.RpMaterial),
+                                         C.POINTER(abi.RpMaterial), _dp, _dp, _dp, _dp, _dp, _u8p]
+    L.rp_engine_instance_counts.argtypes = [_eng, C.POINTER(C.c_size_t)]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -65,7 +68,8 @@ def lib() -> C.CDLL:
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
-                 "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats"):
+                 "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
+                 "rp_engine_upload_batch", "rp_engine_instance_counts"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -157,6 +161,33 @@ class Engine:
         arrs = [_vec(q, n) for q in (v, w, a, tau, rx, rv, rw)]
         _check(self._L.rp_engine_upload_full(self._h, n, _ptr(xs), *[_ptr(q) for q in arrs], _ptr(r), _ptr(m),
                                              _ptr(f, C.c_uint8)))
+        self._ninst = 1
+
+    def upload_batch(self, counts, materials, x, radius, mass, fixed=None, v=None, w=None,
+                     wall_materials=None) -> None:
+        """Independent instances in one engine: counts[k] particles of
+        instance k (arrays concatenated in instance order), each with its own
+        particle material (abi.RpMaterial) and optionally wall material."""
+        counts = [int(c) for c in counts]
+        k = len(counts)
+        n = sum(counts)
+        xs = _vec(np.asarray(x, dtype=np.float64), n)
+        r = np.ascontiguousarray(np.asarray(radius, dtype=np.float64).reshape(n))
+        m = np.ascontiguousarray(np.asarray(mass, dtype=np.float64).reshape(n))
+        f = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, dtype=np.uint8).reshape(n))
+        vs, ws = _vec(v, n), _vec(w, n)
+        cnt = (C.c_size_t * k)(*counts)
+        pm = (abi.RpMaterial * k)(*materials)
+        wm = None if wall_materials is None else (abi.RpMaterial * k)(*wall_materials)
+        _check(self._L.rp_engine_upload_batch(self._h, k, cnt, pm, wm, _ptr(xs), _ptr(vs), _ptr(ws), _ptr(r),
+                                              _ptr(m), _ptr(f, C.c_uint8)))
+        self._ninst = k
+
+    def instance_counts(self) -> np.ndarray:
+        k = getattr(self, "_ninst", 1)
+        cnt = (C.c_size_t * k)()
+        _check(self._L.rp_engine_instance_counts(self._h, cnt))
+        return np.array(list(cnt), dtype=np.int64)
 
     def add_mesh(self, v9, n3) -> None:
         v9 = np.ascontiguousarray(np.asarray(v9, dtype=np.float32).reshape(-1, 9))

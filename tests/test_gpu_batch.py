@@ -1,0 +1,101 @@
+"""Batched ensemble: several independent instances in one engine (one set of
+launches per step) must each evolve exactly as a single engine given that
+instance alone -- bitwise, since the per-instance arithmetic and summation
+order are the same."""
+import numpy as np
+import pytest
+
+from helpers import copy_params, dense_scene, params
+from rusty_particles_amd import abi, native, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(k, seed):
+    rng = np.random.default_rng(seed)
+    inst = []
+    for q in range(k):
+        c = tuple(int(v) for v in rng.integers(3, 9, 3))
+        sc = dense_scene(counts=c, seed=seed * 100 + q, centre=tuple(rng.uniform(-0.02, 0.02, 3)))
+        mat = abi.RpMaterial(1e7, 0.3, 2500.0, float(rng.uniform(0.1, 0.8)), float(rng.uniform(0.2, 0.9)),
+                             float(rng.uniform(0.0, 0.02)))
+        inst.append((sc, mat))
+    return inst
+
+
+def _single(p, sc, mat, meshes, steps, solver=None):
+    q = copy_params(p)
+    q.particle = mat
+    e = native.Engine(q)
+    e.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+    for v9, n3 in meshes:
+        e.add_mesh(v9, n3)
+    if solver:
+        e.set_jkr_solver(solver)
+    e.step(steps)
+    return e.download()
+
+
+def _batched(p, inst, meshes, steps, solver=None, chunks=None):
+    e = native.Engine(copy_params(p))
+    cat = lambda key: np.concatenate([sc[key] for sc, _ in inst])
+    e.upload_batch([sc["x"].shape[0] for sc, _ in inst], [m for _, m in inst], cat("x"), cat("radius"),
+                   cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"))
+    for v9, n3 in meshes:
+        e.add_mesh(v9, n3)
+    if solver:
+        e.set_jkr_solver(solver)
+    for c in (chunks or [steps]):
+        e.step(c)
+    return e, e.download()
+
+
+@pytest.mark.parametrize("normal,tangential,solver", [
+    (abi.RP_NORMAL_SJKR, abi.RP_TANGENTIAL_COULOMB, None),
+    (abi.RP_NORMAL_HERTZ, abi.RP_TANGENTIAL_COULOMB, None),
+    (abi.RP_NORMAL_JKR, abi.RP_TANGENTIAL_COULOMB, "shortcut"),
+    (abi.RP_NORMAL_LINEAR, abi.RP_TANGENTIAL_MINDLIN, None),
+])
+def test_instances_match_single_runs(normal, tangential, solver):
+    inst = _instances(7, seed=normal * 10 + tangential)
+    p = params(1e-5, (-0.06, -0.06, -0.06), (0.06, 0.06, 0.06), normal=normal, tangential=tangential)
+    v9, n3 = scenes.box_mesh((-0.05, -0.05, -0.05), (0.05, 0.05, 0.05))
+    meshes = [(v9, n3)]
+    eng, got = _batched(p, inst, meshes, 60, solver, chunks=[25, 35])
+    counts = eng.instance_counts()
+    off = 0
+    for q, (sc, mat) in enumerate(inst):
+        ref = _single(p, sc, mat, meshes, 60, solver)
+        n = ref["x"].shape[0]
+        assert counts[q] == n
+        for key in ("x", "v", "w", "a", "tau", "rx", "rv", "rw", "radius", "mass"):
+            assert np.array_equal(got[key][off:off + n], ref[key]), (q, key)
+        off += n
+    assert off == got["x"].shape[0]
+
+
+def test_removal_counts_per_instance():
+    """Particles that leave the bounds are removed per instance; the others
+    keep stepping and the per-instance counts follow."""
+    inst = _instances(4, seed=3)
+    for q, (sc, _) in enumerate(inst):
+        if q % 2 == 0:
+            sc["v"][: 5] = [0.0, 0.0, 40.0]  # these leave through +z within the run
+    p = params(1e-4, (-0.08, -0.08, -0.08), (0.08, 0.08, 0.08))
+    eng, got = _batched(p, inst, [], 30)
+    counts = eng.instance_counts()
+    off = 0
+    for q, (sc, mat) in enumerate(inst):
+        ref = _single(p, sc, mat, [], 30)
+        n = ref["x"].shape[0]
+        assert counts[q] == n and (n < sc["x"].shape[0]) == (q % 2 == 0)
+        assert np.array_equal(got["x"][off:off + n], ref["x"]), q
+        off += n
+
+
+def test_batched_rejects_record_and_surrogate():
+    inst = _instances(2, seed=5)
+    p = params(1e-5, (-0.06,) * 3, (0.06,) * 3, normal=abi.RP_NORMAL_JKR)
+    e, _ = _batched(p, inst, [], 1)
+    with pytest.raises(native.EngineError):
+        e.set_record(True)
