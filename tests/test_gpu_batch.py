@@ -80,7 +80,9 @@ def test_removal_counts_per_instance():
     inst = _instances(4, seed=3)
     for q, (sc, _) in enumerate(inst):
         if q % 2 == 0:
-            sc["v"][: 5] = [0.0, 0.0, 40.0]  # these leave through +z within the run
+            # removal happens 1 m below bounds_min.y (simulation.rs:200-214): start just above that line
+            sc["x"][:5, 1] = -0.08 - 0.9995
+            sc["v"][:5] = [0.0, -10.0, 0.0]
     p = params(1e-4, (-0.08, -0.08, -0.08), (0.08, 0.08, 0.08))
     eng, got = _batched(p, inst, [], 30)
     counts = eng.instance_counts()
