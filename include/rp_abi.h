@@ -237,6 +237,13 @@ rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
 rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
+/* Step loop in HIP graphs (on by default; no counterpart in the reference):
+ * rp_engine_step replays a captured loop of 20 launches (two rebuild-chain
+ * slots, the force kernels, the surrogate's kernels when active) where the
+ * step count allows; the results are the same either way.  Timing runs
+ * (rp_engine_set_timing) launch every kernel individually. */
+rp_status rp_engine_set_graph(rp_engine* e, int on);
+
 /* Turns per-kernel timing on or off (off by default).  When on, every
  * force-kernel launch, every rebuild chain, every first-half kernel and every
  * surrogate collect + inference pair is bracketed by HIP events on the

@@ -69,6 +69,8 @@ constexpr int kScanItems = 16;                   // items per thread in the scan
 constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
 constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
 constexpr uint64_t kRebuildInterval = 10;        // steps between launches of the rebuild chain
+constexpr uint64_t kGraphSteps = 2 * kRebuildInterval;  // launches per captured step loop (two rebuild chains,
+                                                        // so buffer and request parities repeat)
 constexpr int kNbrBatch = 2;                     // neighbours whose loads the force kernel issues together
 constexpr double kPi = 3.14159265358979323846;
 
@@ -121,6 +123,10 @@ struct Ctl {
   unsigned long long stall_from;
   unsigned long long steps_done;  // force launches that advanced the state
   int at_rest;             // 1 when the last advancing launch left the at-rest state
+  // Step labels: a launch's label is label_base + the offset it was given.
+  // Host launches keep label_base fixed; a captured step loop adds its
+  // length at its end, so each replay labels its launches afresh.
+  unsigned long long label_base;
   unsigned surr_rows;      // JKR surrogate feature rows claimed by the current step
   unsigned surr_rows_max;  // most rows any step wanted (sizes the row buffers)
 };
@@ -629,7 +635,8 @@ __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i
 // each rp_engine_step call).
 // flag_slot: the rebuild-request slot read by the next rebuild chain.
 __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KParams P, Dyn S, Stat T,
-                                                  const float* xb, unsigned long long step) {
+                                                  const float* xb, unsigned long long off) {
+  const unsigned long long step = ctl->label_base + off;
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -922,13 +929,14 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
 template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long step, KParams P,
+__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
                                                   const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
                                                   unsigned long long* pairbuf, int* paircount, SurrDev sd,
                                                   SurrStats* sstats) {
   const int n = ctl->n;
+  const unsigned long long step = ctl->label_base + off;
   // the rows of this step have been read by the MLP kernel; the next step's
   // collect kernel starts from zero
   if (JS == 2 && blockIdx.x == 0 && threadIdx.x == 0) *sd.counter = 0u;
@@ -1154,8 +1162,9 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
 // per wave places each particle's rows contiguously, a second pass writes
 // them.  base[i] = -1 when i has none or the rows would exceed the buffer
 // (its contacts then take the exact solve).
-__global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long long step, KParams P, Dyn S, Stat T,
+__global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long long off, KParams P, Dyn S, Stat T,
                                                         const int* nbr_count, const int* nbr, SurrDev sd) {
+  const unsigned long long step = ctl->label_base + off;
   if (ctl->stall_from <= step || ctl->overflow) return;  // the force launch keeps the state: no rows needed
   const int n = ctl->n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1208,6 +1217,9 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const double* __restrict_
     dst[i] = src[i];
 }
 
+__global__ void k_set_label(Ctl* ctl, unsigned long long v) { ctl->label_base = v; }
+__global__ void k_advance_label(Ctl* ctl, unsigned long long by) { ctl->label_base += by; }
+
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
   ctl->rebuild[par] = 1;
   ctl->rebuild[par ^ 1] = 0;
@@ -1246,6 +1258,14 @@ struct rp_engine {
   int cur = 0;
   uint64_t chain_no = 0;  // rebuild chains launched; chain c reads request slot c & 1
   uint64_t launch_no = 0; // force launches issued (labels for the stall watermark)
+  uint64_t label_host = 0; // the device's label_base as of the last enqueued work
+  // captured step loops (kGraphSteps launches), one per rebuild-request
+  // parity at their start; valid while gen matches
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  uint64_t graph_gen[2] = {0, 0};
+  uint64_t gen = 1;        // bumped by every change to what launches capture
+  bool use_graph = true;
+  uint64_t graph_replays = 0;
   uint64_t steps_done = 0;  // steps that advanced the state (device count at the last sync)
   bool at_rest = true;    // state is at rest (needs k_begin before the next force launch)
   BlockStats* d_bstats = nullptr;
@@ -1341,6 +1361,7 @@ rp_status validate(const rp_params* p) {
 
 // Host-side kernel parameters from rp_params + particle radii + meshes.
 rp_status compute_geometry(rp_engine* e) {
+  e->gen++;
   const rp_params& p = e->prm;
   KParams& K = e->kp;
   K.dt = p.dt;
@@ -1450,6 +1471,7 @@ rp_status compute_geometry(rp_engine* e) {
 }
 
 rp_status upload_meshes_binned(rp_engine* e) {
+  e->gen++;
   KParams& K = e->kp;
   std::vector<Tri> tris;
   for (const HostMesh& m : e->meshes) {
@@ -1596,6 +1618,7 @@ void free_particles(rp_engine* e) {
 }
 
 rp_status alloc_cells(rp_engine* e) {
+  e->gen++;
   size_t need = (size_t)e->kp.ncells_all + 2;
   if (need <= e->cells_alloc) return RP_OK;
   free_ptr(e->d_cell_count);
@@ -1612,6 +1635,7 @@ rp_status alloc_cells(rp_engine* e) {
 }
 
 rp_status alloc_nbr(rp_engine* e, int nbr_cap) {
+  e->gen++;
   free_ptr(e->d_nbr);
   e->d_nbr = nullptr;
   HIP_TRY(hipMalloc(&e->d_nbr, (size_t)nbr_cap * (size_t)e->cap * sizeof(int)));
@@ -1724,6 +1748,9 @@ hipEvent_t next_event(rp_engine* e) {
   return e->ev_pool[e->ev_used++];
 }
 
+// Offset of the next launch's label from the device's label_base.
+unsigned long long label_off(const rp_engine* e) { return (unsigned long long)(e->launch_no - e->label_host); }
+
 void mark_begin(rp_engine* e, int kind) {
   if (!e->timing) return;
   size_t idx = e->ev_used;
@@ -1764,6 +1791,7 @@ SurrDev surr_dev(rp_engine* e) {
 }
 
 rp_status alloc_surr_rows(rp_engine* e, int rows) {
+  e->gen++;
   free_ptr(e->d_surr_x);
   free_ptr(e->d_surr_y);
   e->d_surr_x = e->d_surr_y = nullptr;
@@ -1781,7 +1809,7 @@ bool surr_active(const rp_engine* e) { return e->jkr_solver == RP_JKR_SURROGATE 
 void launch_surrogate(rp_engine* e) {
   const SurrDev sd = surr_dev(e);
   const int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_jkr_collect, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, (unsigned long long)e->launch_no,
+  hipLaunchKernelGGL(k_jkr_collect, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, label_off(e),
                      e->kp, e->dyn[e->cur], e->stat[0], e->d_nbr_count, e->d_nbr, sd);
   mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
 }
@@ -1790,7 +1818,7 @@ template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
 void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
-                     (unsigned long long)e->launch_no, e->kp,
+                     label_off(e), e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
                      JS == 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
@@ -1850,6 +1878,62 @@ void launch_force(rp_engine* e, int slot) {
     case RP_NORMAL_JKR: launch_force_n<RP_NORMAL_JKR, ADV, REC>(e, slot); break;
     default: launch_force_n<RP_NORMAL_SJKR, ADV, REC>(e, slot); break;
   }
+}
+
+// Records kGraphSteps advancing launches (rebuild chain every
+// kRebuildInterval launches, surrogate rows if active, force kernel) and the
+// label advance into a graph.  Capture enqueues nothing: the host counters
+// are restored afterwards and advanced by each replay instead.
+rp_status capture_steps(rp_engine* e, hipGraphExec_t* exec) {
+  const uint64_t launch0 = e->launch_no, chain0 = e->chain_no, label0 = e->label_host;
+  const int cur0 = e->cur;
+  e->label_host = e->launch_no;  // offsets 0 .. kGraphSteps-1 relative to the label at replay
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  rp_status st = RP_OK;
+  for (uint64_t s = 0; s < kGraphSteps && st == RP_OK; ++s) {
+    if (e->launch_no % kRebuildInterval == 0) st = launch_rebuild(e);
+    if (st == RP_OK && surr_active(e)) launch_surrogate(e);
+    if (st == RP_OK) launch_force<true, false>(e, (int)(e->chain_no & 1));
+  }
+  hipLaunchKernelGGL(k_advance_label, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (unsigned long long)kGraphSteps);
+  const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+  e->launch_no = launch0;
+  e->chain_no = chain0;
+  e->label_host = label0;
+  e->cur = cur0;
+  if (st) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (ce != hipSuccess) return fail(RP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(RP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  return RP_OK;
+}
+
+// One replay of the captured loop for the current request parity
+// (captured first if what the launches bake in has changed).
+rp_status replay_steps(rp_engine* e) {
+  const int slot = (int)(e->chain_no & 1);
+  if (!e->graph_exec[slot] || e->graph_gen[slot] != e->gen) {
+    if (e->graph_exec[slot]) (void)hipGraphExecDestroy(e->graph_exec[slot]);
+    e->graph_exec[slot] = nullptr;
+    rp_status st = capture_steps(e, &e->graph_exec[slot]);
+    if (st) return st;
+    e->graph_gen[slot] = e->gen;
+  }
+  if (e->label_host != e->launch_no) {
+    hipLaunchKernelGGL(k_set_label, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (unsigned long long)e->launch_no);
+    e->label_host = e->launch_no;
+  }
+  HIP_TRY(hipGraphLaunch(e->graph_exec[slot], e->stream));
+  e->launch_no += kGraphSteps;
+  e->chain_no += kGraphSteps / kRebuildInterval;
+  e->label_host += kGraphSteps;
+  e->graph_replays++;
+  return RP_OK;
 }
 
 }  // namespace
@@ -1935,6 +2019,8 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   if (n > (size_t)1 << 30) return fail(RP_ERR_INVALID, "too many particles");
   HIP_TRY(hipStreamSynchronize(e->stream));
   free_particles(e);
+  e->gen++;
+  e->label_host = 0;  // the control block below starts with label_base 0
   int cap = (int)std::max<size_t>(round_up(n, 256), 256);
   e->cap = cap;
   for (int b = 0; b < 2; ++b) {
@@ -2124,10 +2210,17 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
     if (e->at_rest) {
       mark_begin(e, 2);
       hipLaunchKernelGGL(k_begin, dim3(gcap), dim3(kBlock), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1),
-                         e->kp, e->dyn[e->cur], e->stat[0], e->d_xb, (unsigned long long)e->launch_no);
+                         e->kp, e->dyn[e->cur], e->stat[0], e->d_xb, label_off(e));
       mark_end(e);
     }
     for (uint64_t s = 0; s < remaining; ++s) {
+      // Whole captured loops where they fit (aligned to the rebuild cadence,
+      // never the call's last launch, which leaves the at-rest state).
+      if (e->use_graph && !e->timing && s > 0 && e->launch_no % kGraphSteps == 0 && remaining - s > kGraphSteps) {
+        if ((st = replay_steps(e))) return st;
+        s += kGraphSteps - 1;
+        continue;
+      }
       // The rebuild chain runs at the first launch of a round and every
       // kRebuildInterval launches; the force kernels' look-ahead covers the gap.
       if (s == 0 || e->launch_no % kRebuildInterval == 0) {
@@ -2382,6 +2475,7 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
   }
   e->jkr_solver = solver;
   e->surr_check = check ? 1 : 0;
+  e->gen++;
   e->sstats_valid = false;
   return RP_OK;
 }
@@ -2410,6 +2504,12 @@ rp_status rp_engine_jkr_stats(rp_engine* e, rp_jkr_stats* out) {
     out->err_mean = es / (double)out->served;
     out->raw_mean = rs / (double)out->served;
   }
+  return RP_OK;
+}
+
+rp_status rp_engine_set_graph(rp_engine* e, int on) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  e->use_graph = on != 0;
   return RP_OK;
 }
 
@@ -2468,6 +2568,8 @@ void rp_engine_destroy(rp_engine* e) {
   if (!e) return;
   (void)hipStreamSynchronize(e->stream);
   for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+  for (hipGraphExec_t g : e->graph_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   free_particles(e);
   free_ptr(e->d_cell_count);
   free_ptr(e->d_cell_start);

@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
     L.rp_engine_upload_batch.argtypes = [_eng, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(abi.RpMaterial),
                                          C.POINTER(abi.RpMaterial), _dp, _dp, _dp, _dp, _dp, _u8p]
     L.rp_engine_instance_counts.argtypes = [_eng, C.POINTER(C.c_size_t)]
+    L.rp_engine_set_graph.argtypes = [_eng, C.c_int]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -69,7 +70,7 @@ def lib() -> C.CDLL:
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
-                 "rp_engine_upload_batch", "rp_engine_instance_counts"):
+                 "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -230,6 +231,10 @@ class Engine:
         _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
                                        C.byref(tot)))
         return offs, js[: tot.value]
+
+    def set_graph(self, on: bool) -> None:
+        """Replay captured 20-launch step loops (default on)."""
+        _check(self._L.rp_engine_set_graph(self._h, 1 if on else 0))
 
     def set_timing(self, on: bool) -> None:
         _check(self._L.rp_engine_set_timing(self._h, 1 if on else 0))
