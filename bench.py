@@ -21,6 +21,14 @@ trained on the GPU before timing, plus FP64 polish and exact fallback);
 value = the surrogate form, the others and the surrogate's force errors
 (check mode, exact solve alongside) are in the same line.
 
+--workload batch: an ensemble of 256 independent simulations (N_k ~
+U{2000..50000}, 6.7M spheres in total, r ~ U[1, 3] mm, per-instance friction,
+restitution and surface energy; simplified JKR + Coulomb, dt = 5e-6 s,
+jittered lattices in a 0.2 m box STL column) stepped by one batched engine
+(one set of launches per step over all instances, HIP-graph step loop);
+value = total particle-steps/s.  The same instances stepped one engine at a
+time (each with its own launches) are timed beside it.
+
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
 HIP events on the engine's stream), and a CPU baseline: the oracle (a C++
@@ -69,6 +77,109 @@ def build_workload(name: str, n: int | None):
     raise SystemExit(f"unknown workload {name}")
 
 
+def run_batch(args) -> int:
+    """Batched ensemble against one engine per instance."""
+    import numpy as np
+
+    from rusty_particles_amd import abi, native, scenes
+    t0 = time.perf_counter()
+    p, counts, mats, x, r, m, meshes = scenes.batch_ensemble_workload(args.instances)
+    n = int(sum(counts))
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    log(f"workload batch: {len(counts)} instances, {n} particles, built in {time.perf_counter() - t0:.1f}s")
+
+    def engine(lo, hi, batch=True):
+        q = abi.RpParams.from_buffer_copy(p)
+        sl = slice(int(offs[lo]), int(offs[hi]))
+        if batch:
+            e = native.Engine(q)
+            e.upload_batch(counts[lo:hi], mats[lo:hi], x[sl], r[sl], m[sl])
+        else:
+            q.particle = mats[lo]
+            e = native.Engine(q)
+            e.upload(x[sl], r[sl], m[sl])
+        for v9, n3 in meshes:
+            e.add_mesh(v9, n3)
+        return e
+
+    res = {}
+    for graph in (True, False):
+        e = engine(0, len(counts))
+        e.set_graph(graph)
+        e.step(args.warmup)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.step(args.steps)
+        e.synchronize()
+        wall = time.perf_counter() - t0
+        res[graph] = (n * args.steps / wall, wall, e)
+        log(f"batched, graph {'on' if graph else 'off'}: {args.steps} steps in {wall:.3f}s, "
+            f"{n * args.steps / wall:.4g} particle-steps/s")
+        if not graph:
+            e.close()
+    value, wall, eng = res[True]
+    st = eng.stats()
+    eng.set_timing(True)
+    eng.step(args.timing_steps)
+    tm = eng.timing()
+    eng.set_timing(False)
+    f_ms, f_cnt = tm["force"]
+    avg_s = f_ms / max(f_cnt, 1) * 1e-3
+    achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
+    log(f"force kernel avg {avg_s * 1e6:.1f} us ({achieved:.0f} GB/s algorithmic), contacts/step {st['contacts']}")
+    eng.close()
+
+    # the same instances, one engine each, run back to back (graph on)
+    sep_steps = max(args.steps // 4, 20)
+    total = 0.0
+    for k in range(len(counts)):
+        e = engine(k, k + 1, batch=False)
+        e.step(args.warmup)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.step(sep_steps)
+        e.synchronize()
+        total += time.perf_counter() - t0
+        e.close()
+    sep_value = n * sep_steps / total
+    log(f"one engine per instance: {sep_steps} steps each, {total:.3f}s in all, {sep_value:.4g} particle-steps/s")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        sample = list(range(0, len(counts), max(len(counts) // 4, 1)))[:4]
+        csec, cwork = 0.0, 0
+        for k in sample:
+            sl = slice(int(offs[k]), int(offs[k + 1]))
+            q = abi.RpParams.from_buffer_copy(p)
+            q.particle = mats[k]
+            cell = 2.0 * float(r.max()) * 1.5
+            val, sec = cpu_baseline(q, x[sl], r[sl], m[sl], None, meshes, args.cpu_steps, threads, cell)
+            csec += sec
+            cwork += counts[k] * args.cpu_steps
+        cpu = {"value": cwork / csec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"instances {sample} ({sum(counts[k] for k in sample)} particles), {args.cpu_steps} "
+                         f"step(s) each after 1 warm-up step; {csec:.1f}s"}
+        log(f"cpu baseline {cwork / csec:.4g} particle-steps/s")
+    out = {
+        "metric": "particle-steps/s on 1 GPU, plain vs surrogate+batched at 1e-3 force err; HBM GB/s",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch: {len(counts)} instances, {n} spheres in all, sJKR+Coulomb, material sweep, "
+                               f"dt={p.dt:g}, FP64", "particles": n, "instances": len(counts), "form": "batched"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "contacts_per_step": st["contacts"], "wall_contacts_per_step": st["wall_contacts"],
+        "force_kernel_avg_us": avg_s * 1e6,
+        "graph_off_value": res[False][0],
+        "one_engine_per_instance_value": sep_value,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
     largest contact cutoff (the cell-size heuristic of simulation.rs:64)."""
@@ -106,6 +217,7 @@ def main() -> int:
                     help="JKR solvers to time on the jkr workload (the last one is the reported value)")
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -120,6 +232,8 @@ def main() -> int:
     from rusty_particles_amd import abi, native
     if native.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
+    if args.workload == "batch":
+        return run_batch(args)
 
     t0 = time.perf_counter()
     p, x, r, m, v, meshes, desc = build_workload(args.workload, args.n)

@@ -239,3 +239,45 @@ def jkr_periodic_workload(n: int = 2_000_000, seed: int = 0xD3E50003, phi: float
     p.tangential_model = abi.RP_TANGENTIAL_COULOMB
     p.particle = abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, 0.05)
     return p, x, r, m, v
+
+
+def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int = 2000, nmax: int = 50000):
+    """Ensemble of `ninst` independent simulations for a material sweep:
+    N_k ~ U{nmin..nmax} spheres, r ~ U[1, 3] mm, rho 2500, per-instance
+    friction mu ~ U[0.1, 0.8], restitution e ~ U[0.2, 0.9], surface energy
+    gamma ~ U[0, 0.02] J/m^2 (E = 1e7, nu = 0.3), simplified JKR + Coulomb,
+    dt = 5e-6 s.  Each instance fills a jittered lattice (spacing 6.5 mm,
+    jitter +-0.2 mm, so no initial overlap) from the floor of a 0.2 m square
+    column upward -- a 0.2 m cube holds only ~25k such sites, so the column is
+    0.45 m tall --
+    closed by a 12-triangle box STL.  Returns (params, counts, materials, x,
+    r, m, meshes)."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(nmin, nmax + 1, ninst)
+    spacing, side = 0.0065, 0.2
+    per_axis = int(side / spacing) - 1            # 31 sites across, 0.2 m
+    layers = int(np.ceil(nmax / per_axis ** 2))   # tallest instance
+    origin = np.array([0.5 * (side - (per_axis - 1) * spacing), 0.005, 0.5 * (side - (per_axis - 1) * spacing)])
+    xs, rs, mats = [], [], []
+    for k in range(ninst):
+        n = int(counts[k])
+        # sites layer by layer (y, the vertical axis, slowest)
+        g = np.stack(np.meshgrid(np.arange(per_axis), np.arange(per_axis), np.arange(layers), indexing="ij"), -1)
+        sites = g.transpose(2, 1, 0, 3).reshape(-1, 3)[:n]  # (x, z, y) with y slowest
+        jit = np.random.default_rng(seed + 17 * k).uniform(-0.0002, 0.0002, (n, 3))
+        x = origin + np.stack([sites[:, 0], sites[:, 2], sites[:, 1]], 1) * spacing + jit
+        r = np.random.default_rng(seed + 17 * k + 1).uniform(0.001, 0.003, n)
+        xs.append(x)
+        rs.append(r)
+        mats.append(abi.RpMaterial(1e7, 0.3, 2500.0, float(rng.uniform(0.1, 0.8)), float(rng.uniform(0.2, 0.9)),
+                                   float(rng.uniform(0.0, 0.02))))
+    x = np.concatenate(xs)
+    r = np.concatenate(rs)
+    m = sphere_mass(r, 2500.0)
+    top = 0.45
+    p = abi.default_params(5e-6, (-0.01, -0.01, -0.01), (side + 0.01, top + 0.01, side + 0.01))
+    p.normal_model = abi.RP_NORMAL_SJKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(1e7, 0.3, 2500.0, 0.5, 0.5, 0.0)
+    v9, n3 = box_mesh((0.0, 0.0, 0.0), (side, top, side))
+    return p, [int(c) for c in counts], mats, x, r, m, [(v9, n3)]
