@@ -331,7 +331,7 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
 typedef struct rp_jkr_stats {
   uint64_t served;         /* contacts whose radius came from the network */
   uint64_t fallback;       /* served-envelope contacts that took the exact solve */
-  uint64_t rows_max;       /* most feature rows any step wanted */
+  uint64_t rows_max;       /* most rows one of the 16 claim ranges was asked for in a step */
   uint64_t rows_capacity;  /* feature rows the engine holds */
   double err_max, err_mean;  /* check mode: |F - F_exact| / max(|F_exact|, (2/3) pi gamma R*) */
   double raw_max, raw_mean;  /* check mode: the same for the network's force output */

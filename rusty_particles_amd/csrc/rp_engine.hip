@@ -69,10 +69,16 @@ constexpr int kScanItems = 16;                   // items per thread in the scan
 constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
 constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
 constexpr uint64_t kRebuildInterval = 10;        // steps between launches of the rebuild chain
+constexpr long long kNoStall = 0x7fffffffffffffffLL;  // no stalled step recorded
 constexpr uint64_t kGraphSteps = 2 * kRebuildInterval;  // launches per captured step loop (two rebuild chains,
                                                         // so buffer and request parities repeat)
 constexpr int kNbrBatch = 2;                     // neighbours whose loads the force kernel issues together
 constexpr double kPi = 3.14159265358979323846;
+// Margin (in d = delta / t) of the surrogate's near masks: a contact whose d
+// changes by more than this in one step, into the served set, takes the exact
+// solve that step.  A step moves d by (relative speed) dt / t, about 2e-3 in
+// the 2M-particle JKR packing.
+constexpr double kSurrMaskMargin = 0.5;
 
 // Per-step particle state of one buffer (device pointers, `cap` slots).
 // The force kernel reads one buffer and writes the other, so no thread can
@@ -116,19 +122,22 @@ struct Ctl {
   int max_count;           // largest candidate count seen by k_build
   unsigned long long rebuilds;  // rebuilds done
   int pair_overflow;       // pair record overflow
-  // First step number that must not run on the current neighbour lists: a
-  // particle moved more than skin/2 since the last build (its list is no
-  // longer a superset).  Force launches of steps >= stall_from leave the state
-  // unchanged; the next rebuild resets it and the host re-issues the steps.
-  unsigned long long stall_from;
+  // First step that must not run on the current neighbour lists: a particle
+  // moved more than skin/2 since the last build (its list is no longer a
+  // superset).  Force launches of steps at or after it leave the state
+  // unchanged; the next rebuild resets it (kNoStall) and the host re-issues
+  // the steps.  Steps are labelled by their offset from a base the host
+  // moves (k_shift_labels): plain launches carry offsets that grow, and a
+  // captured step loop shifts the base by its length at its end, so each
+  // replay labels its launches afresh.
+  long long stall_off;
   unsigned long long steps_done;  // force launches that advanced the state
   int at_rest;             // 1 when the last advancing launch left the at-rest state
-  // Step labels: a launch's label is label_base + the offset it was given.
-  // Host launches keep label_base fixed; a captured step loop adds its
-  // length at its end, so each replay labels its launches afresh.
-  unsigned long long label_base;
-  unsigned surr_rows;      // JKR surrogate feature rows claimed by the current step
-  unsigned surr_rows_max;  // most rows any step wanted (sizes the row buffers)
+  // rebuilds + 1 when the force kernel last wrote the surrogate's near masks
+  // (a rebuild re-orders the lists, so the next collect scans every entry)
+  unsigned long long surr_mask_epoch;
+  unsigned surr_rows[kSurrRanges];  // JKR surrogate feature rows claimed by the current step, per range
+  unsigned surr_rows_max;  // most rows any range was asked for in one step (sizes the row buffers)
 };
 
 // Per-workgroup counters of one force launch.
@@ -538,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->n = n;
     ctl->rebuilds += 1;
-    ctl->stall_from = ~0ull;
+    ctl->stall_off = kNoStall;
   }
 }
 
@@ -604,8 +613,8 @@ __device__ inline double mic_disp(double d, const KParams& P) {
 // rebuild chain only every `lookahead_steps` steps, so a particle asks for a
 // rebuild once its displacement plus a bound on its motion over that many
 // steps (2 K |v| dt + K^2 |a| dt^2) reaches skin/2.  A particle already past
-// skin/2 at the position of step `next_step` stalls that step (stall_from);
-// the step runs again after the rebuild.
+// skin/2 at the position of step `next_step` (its label offset) stalls that
+// step (stall_off); the step runs again after the rebuild.
 __device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, const float* xb, int i, Ctl* ctl,
                                      unsigned long long next_step) {
   const double d0 = mic_disp<0>(x.x - (double)xb[i], P);
@@ -613,7 +622,7 @@ __device__ inline bool moved_too_far(const KParams& P, dv3 x, dv3 v, dv3 a, cons
   const double d2 = mic_disp<2>(x.z - (double)xb[i + 2 * P.cap], P);
   const double dd = d0 * d0 + d1 * d1 + d2 * d2;
   if (!(dd < P.safe_half_skin * P.safe_half_skin)) {
-    atomicMin(&ctl->stall_from, next_step);
+    atomicMin(&ctl->stall_off, (long long)next_step);
     return true;
   }
   const double k = P.lookahead_steps, dt = P.dt;
@@ -636,7 +645,6 @@ __device__ inline void mark_removed(Ctl* ctl, const Stat& T, const Dyn& D, int i
 // flag_slot: the rebuild-request slot read by the next rebuild chain.
 __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KParams P, Dyn S, Stat T,
                                                   const float* xb, unsigned long long off) {
-  const unsigned long long step = ctl->label_base + off;
   const int n = ctl->n;
   const int cap = P.cap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -659,7 +667,7 @@ __global__ void __launch_bounds__(kBlock) k_begin(Ctl* ctl, int flag_slot, KPara
         st3(S.rx, i, cap, rx);
         st3(S.rv, i, cap, rv);
         st3(S.rw, i, cap, rw);
-        far = moved_too_far(P, x, v, a, xb, i, ctl, step);
+        far = moved_too_far(P, x, v, a, xb, i, ctl, off);
       }
     }
   }
@@ -704,16 +712,19 @@ struct Acc {
   unsigned int ncont, njkr;
   int npair;
   bool overlap;
-  // JKR surrogate: first feature row of this particle (-1 = none), rows
-  // consumed so far, and counters
-  int srow, sk;
+  // JKR surrogate: first feature row of this particle (-1 = none), the list
+  // entries that have rows, the entries near the served set (next step's
+  // collect visits only these), and counters
+  int srow;
+  unsigned long long rmask, nmask;
   unsigned int sserved, sfall;
   float emax, rmax;
   double esum, rsum;
 };
 
 // JKR normal force with the exact no-root shortcut (JS >= 1) and the network
-// (JS == 2).  JS == 0 is the reference's solve as written.  The shortcut and
+// (JS >= 2; JS == 3 also runs the exact solve and records the force errors).
+// JS == 0 is the reference's solve as written.  The shortcut and
 // jkr() agree exactly (zero force, a rejected root); the network path polishes
 // the predicted radius to the root the reference converges to.
 template <int JS>
@@ -730,18 +741,21 @@ __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rst
     *failed = 1;
     return zero3();
   }
-  if (JS == 2) {
+  if (JS >= 2) {
+    if (k < 64 && sd.pp_ok && rstar >= sd.r_lo && rstar <= sd.r_hi && s >= kSurrSLo * rstar &&
+        s <= kSurrSHi * rstar && ov >= sd.md_lo * t && ov <= sd.md_hi * t)
+      acc.nmask |= 1ull << k;
     double d = 0.0;
     if (surr_route(ov, rstar, s, t, sd, k, d)) {
-      const int row = acc.srow < 0 ? -1 : acc.srow + acc.sk;
-      ++acc.sk;
+      const unsigned long long bit = 1ull << k;  // k < 64 here
+      const int row = (acc.rmask & bit) ? acc.srow + __popcll(acc.rmask & (bit - 1ull)) : -1;
       if (row >= 0 && row < sd.rows_cap) {
         const float y0 = sd.y[2 * (size_t)row], y1 = sd.y[2 * (size_t)row + 1];
         double al = (double)unscale_out(y0, sd.sc.out_lo[0], sd.sc.out_hi[0]);
         if (polish_alpha(d, al)) {
           const dv3 f = jkr_finish(al * s, ov, n, c, rstar, failed);
           ++acc.sserved;
-          if (sd.check) {
+          if (JS == 3) {
             int ef = 0;
             const dv3 fe = jkr(ov, n, c, rstar, &ef);
             const double fpull = c.jpull * rstar;  // -(2/3) pi gamma R*
@@ -918,6 +932,45 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
   }
 }
 
+// Contacts of particle i: its neighbour list (loads issued kNbrBatch at a
+// time, so a wave waits for memory a few times per particle instead of once
+// per neighbour) and the triangles binned to its cell.
+template <int NORMAL, int TANG, bool RECORD, int JS>
+__device__ inline void gather(const KParams& P, const MaterialConsts& mc, double far_pp, double far_pw, const Dyn& S,
+                              const Stat& T, const Own& o, int i, int cnt, const int* nbr, const Tri* tris,
+                              uint32_t t0, uint32_t t1, const uint32_t* tri_idx, Acc& acc, unsigned int& nwall,
+                              Ctl* ctl, unsigned long long* prow, int* paircount, const SurrDev& sd) {
+  const int cap = P.cap;
+  for (int base = 0; base < cnt; base += kNbrBatch) {
+    int jj[kNbrBatch];
+    dv3 xq[kNbrBatch];
+    double rq[kNbrBatch];
+#pragma unroll
+    for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
+#pragma unroll
+    for (int q = 0; q < kNbrBatch; ++q) {
+      const int j = jj[q] < 0 ? i : jj[q];
+      ldxr(S.xr, j, xq[q], rq[q]);
+    }
+    int hit = 0;
+#pragma unroll
+    for (int q = 0; q < kNbrBatch; ++q) {
+      const dv3 d = pair_delta(P, o.x, xq[q]);
+      if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
+    }
+    while (hit) {
+      const int q = __ffs(hit) - 1;
+      hit &= hit - 1;
+      const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
+                                   : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
+      contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
+    }
+  }
+  if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
+  for (uint32_t q = t0; q < t1; ++q)
+    contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
+}
+
 // Gather force kernel + integration epilogue, one thread per particle slot.
 // Reads Dyn buffer S and writes buffer D (so no thread can read a neighbour's
 // next-step state); the host swaps the two between steps.  All own-state loads are issued before the neighbour loop, and
@@ -936,18 +989,24 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
                                                   unsigned long long* pairbuf, int* paircount, SurrDev sd,
                                                   SurrStats* sstats) {
   const int n = ctl->n;
-  const unsigned long long step = ctl->label_base + off;
   // the rows of this step have been read by the MLP kernel; the next step's
   // collect kernel starts from zero
-  if (JS == 2 && blockIdx.x == 0 && threadIdx.x == 0) *sd.counter = 0u;
-  if (ctl->stall_from <= step || ctl->overflow) {
+  if (JS >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned mx = 0u;
+    for (int r = 0; r < kSurrRanges; ++r) {
+      mx = max(mx, sd.counter[r]);
+      sd.counter[r] = 0u;
+    }
+    ctl->surr_rows_max = max(ctl->surr_rows_max, mx);
+  }
+  if (ctl->stall_off <= (long long)off || ctl->overflow) {
     // The lists may miss a pair for this step: keep the state as it is (the
     // host swaps the buffers after every launch, so copy it across) and let
     // the host run this step again after a rebuild.
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) copy_slot(S, D, i, P.cap);
     if (threadIdx.x == 0) bstats[blockIdx.x] = BlockStats{0u, 0u, 0u, 0u};
-    if (JS == 2 && threadIdx.x == 0) sstats[blockIdx.x] = SurrStats{0u, 0u, 0.f, 0.f, 0.0, 0.0};
+    if (JS >= 2 && threadIdx.x == 0) sstats[blockIdx.x] = SurrStats{0u, 0u, 0.f, 0.f, 0.0, 0.0};
     return;
   }
   const int cap = P.cap;
@@ -961,7 +1020,8 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
   acc.npair = 0;
   acc.overlap = false;
   acc.srow = -1;
-  acc.sk = 0;
+  acc.rmask = 0ull;
+  acc.nmask = 0ull;
   acc.sserved = 0;
   acc.sfall = 0;
   acc.emax = 0.f;
@@ -989,39 +1049,18 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
       o.rc1 = ref_coord<1>(o.x.y, P);
       o.rc2 = ref_coord<2>(o.x.z, P);
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
-      if (JS == 2) acc.srow = sd.base[i];
-      const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
-      const MaterialConsts& mc = BATCH ? ic->mc : P.mc;
-      const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
-      const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
-      for (int base = 0; base < cnt; base += kNbrBatch) {
-        int jj[kNbrBatch];
-        dv3 xq[kNbrBatch];
-        double rq[kNbrBatch];
-#pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
-#pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) {
-          const int j = jj[q] < 0 ? i : jj[q];
-          ldxr(S.xr, j, xq[q], rq[q]);
-        }
-        int hit = 0;
-#pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) {
-          const dv3 d = pair_delta(P, o.x, xq[q]);
-          if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
-        }
-        while (hit) {
-          const int q = __ffs(hit) - 1;
-          hit &= hit - 1;
-          const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
-                                       : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
-        }
+      if (JS >= 2) {
+        acc.srow = sd.base[i];
+        if (acc.srow >= 0) acc.rmask = sd.rowmask[i];
       }
-      if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
-      for (uint32_t q = t0; q < t1; ++q)
-        contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
+      if constexpr (BATCH) {
+        const InstConsts& ic = P.itab[T.inst[i]];
+        gather<NORMAL, TANG, RECORD, JS>(P, ic.mc, ic.jkr_far_pp, ic.jkr_far_pw, S, T, o, i, cnt, nbr, tris, t0, t1,
+                                         tri_idx, acc, nwall, ctl, prow, paircount, sd);
+      } else {
+        gather<NORMAL, TANG, RECORD, JS>(P, P.mc, P.jkr_far_pp, P.jkr_far_pw, S, T, o, i, cnt, nbr, tris, t0, t1,
+                                         tri_idx, acc, nwall, ctl, prow, paircount, sd);
+      }
 
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
@@ -1045,7 +1084,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
           mark_removed(ctl, T, D, i, cap, flags);
           alive = false;
         } else {
-          far = moved_too_far(P, x, v, a, xb, i, ctl, step + 1);
+          far = moved_too_far(P, x, v, a, xb, i, ctl, off + 1);
         }
       } else {
         st3(D.a, i, cap, a);
@@ -1061,6 +1100,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
       }
     }
   }
+  if (JS >= 2 && i < n) sd.nmask[i] = acc.nmask;
   // counters: per-workgroup plain stores; the rebuild request is the only
   // atomic, issued by a wave only when one of its particles needs it
   const unsigned int c1 = wave_sum_u32(acc.ncont), c2 = wave_sum_u32(nwall), c3 = wave_sum_u32(acc.njkr);
@@ -1085,7 +1125,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
     }
     bstats[blockIdx.x] = b;
   }
-  if (JS == 2) {
+  if (JS >= 2) {
     // surrogate counters: wave reductions, then one store per workgroup
     unsigned int s1 = wave_sum_u32(acc.sserved), s2 = wave_sum_u32(acc.sfall);
     float em = acc.emax, rm = acc.rmax;
@@ -1113,6 +1153,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (JS >= 2) ctl->surr_mask_epoch = ctl->rebuilds + 1ull;
     ctl->steps_done += 1;
     ctl->at_rest = ADVANCE ? 0 : 1;
   }
@@ -1157,52 +1198,74 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
 }
 
 // Gathers the feature rows of every contact the JKR surrogate serves in
-// this step (one thread per particle, the force kernel's list order): a first
-// pass marks the served list entries, a wave-wide prefix sum plus one atomic
-// per wave places each particle's rows contiguously, a second pass writes
-// them.  base[i] = -1 when i has none or the rows would exceed the buffer
-// (its contacts then take the exact solve).
+// this step (one thread per particle).  It visits the neighbour-list entries
+// the last force launch marked as near the served set (all entries after a
+// rebuild, which re-orders the lists), marks those served now, places each
+// particle's rows contiguously (a wave-wide prefix sum plus one atomic per
+// wave) and writes them.  base[i] = -1 when i has no rows or they would not
+// fit; rowmask[i] names the entries that have rows, in row order.  A served
+// contact without a row takes the exact solve in the force kernel.
 __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long long off, KParams P, Dyn S, Stat T,
                                                         const int* nbr_count, const int* nbr, SurrDev sd) {
-  const unsigned long long step = ctl->label_base + off;
-  if (ctl->stall_from <= step || ctl->overflow) return;  // the force launch keeps the state: no rows needed
+  if (ctl->stall_off <= (long long)off || ctl->overflow) return;  // the force launch keeps the state: no rows
   const int n = ctl->n;
+  const bool full = ctl->surr_mask_epoch != ctl->rebuilds + 1ull;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   unsigned long long mask = 0ull;
   Own o;
   if (i < n && !(T.flags[i] & FLAG_DEAD)) {
-    ldxr(S.xr, i, o.x, o.r);
-    o.rc0 = ref_coord<0>(o.x.x, P);
-    o.rc1 = ref_coord<1>(o.x.y, P);
-    o.rc2 = ref_coord<2>(o.x.z, P);
     const int kmax = min(nbr_count[i], 64);
-    for (int k = 0; k < kmax; ++k)
-      if (collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, nullptr)) mask |= 1ull << k;
+    unsigned long long visit = full ? ~0ull : sd.nmask[i];
+    if (kmax < 64) visit &= (1ull << kmax) - 1ull;
+    if (visit) {
+      ldxr(S.xr, i, o.x, o.r);
+      o.rc0 = ref_coord<0>(o.x.x, P);
+      o.rc1 = ref_coord<1>(o.x.y, P);
+      o.rc2 = ref_coord<2>(o.x.z, P);
+      while (visit) {
+        const int k = __ffsll((long long)visit) - 1;
+        visit &= visit - 1ull;
+        if (collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, nullptr)) mask |= 1ull << k;
+      }
+    }
   }
+  // rows of this workgroup: wave prefix sums, then one claim per workgroup
+  // from range blockIdx.x % kSurrRanges
   const int c = __popcll(mask);
   int incl = c;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += v;
+  for (int sh = 1; sh < 64; sh <<= 1) {
+    const int v = __shfl_up(incl, sh, 64);
+    if (lane >= sh) incl += v;
   }
-  const int total = __shfl(incl, 63, 64);
-  unsigned int wbase = 0u;
-  if (lane == 63 && total > 0) {
-    wbase = atomicAdd(sd.counter, (unsigned int)total);
-    atomicMax(&ctl->surr_rows_max, wbase + (unsigned int)total);
+  __shared__ unsigned wsum[kBlock / 64];
+  __shared__ unsigned bbase;
+  const int wv = threadIdx.x >> 6;
+  if (lane == 63) wsum[wv] = (unsigned)incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0u;
+    for (int q = 0; q < kBlock / 64; ++q) {
+      const unsigned t = wsum[q];
+      wsum[q] = tot;
+      tot += t;
+    }
+    const int range = blockIdx.x % kSurrRanges;
+    bbase = tot ? (unsigned)range * (unsigned)(sd.rows_cap / kSurrRanges) + atomicAdd(&sd.counter[range], tot) : 0u;
   }
-  wbase = __shfl(wbase, 63, 64);
+  __syncthreads();
   if (i >= n) return;
-  const long long b = (long long)wbase + (incl - c);
-  const bool fits = c > 0 && b + c <= (long long)sd.rows_cap;
+  const long long b = (long long)bbase + wsum[wv] + (incl - c);
+  const long long range_end = (long long)(blockIdx.x % kSurrRanges + 1) * (sd.rows_cap / kSurrRanges);
+  const bool fits = c > 0 && b + c <= range_end;
   sd.base[i] = fits ? (int)b : -1;
   if (!fits) return;
+  sd.rowmask[i] = mask;
   float4* rows = reinterpret_cast<float4*>(sd.x);
   int r = (int)b;
   while (mask) {
     const int k = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
+    mask &= mask - 1ull;
     float4 f;
     (void)collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, &f);
     rows[r++] = f;
@@ -1217,8 +1280,10 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const double* __restrict_
     dst[i] = src[i];
 }
 
-__global__ void k_set_label(Ctl* ctl, unsigned long long v) { ctl->label_base = v; }
-__global__ void k_advance_label(Ctl* ctl, unsigned long long by) { ctl->label_base += by; }
+// Moves the label base forward by `by` steps: offsets are relative to it.
+__global__ void k_shift_labels(Ctl* ctl, unsigned long long by) {
+  if (ctl->stall_off != kNoStall) ctl->stall_off -= (long long)by;
+}
 
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
   ctl->rebuild[par] = 1;
@@ -1258,7 +1323,7 @@ struct rp_engine {
   int cur = 0;
   uint64_t chain_no = 0;  // rebuild chains launched; chain c reads request slot c & 1
   uint64_t launch_no = 0; // force launches issued (labels for the stall watermark)
-  uint64_t label_host = 0; // the device's label_base as of the last enqueued work
+  uint64_t label_host = 0; // launch number of the device's label base as of the last enqueued work
   // captured step loops (kGraphSteps launches), one per rebuild-request
   // parity at their start; valid while gen matches
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
@@ -1304,6 +1369,8 @@ struct rp_engine {
   rp_jkr_envelope surr_env{};
   MlpHost surr_net;
   int* d_surr_base = nullptr;  // [cap]
+  unsigned long long* d_surr_rowmask = nullptr;  // [cap]
+  unsigned long long* d_surr_nmask = nullptr;    // [cap]
   float* d_surr_x = nullptr;   // [surr_rows_cap x 4]
   float* d_surr_y = nullptr;   // [surr_rows_cap x 2]
   int surr_rows_cap = 0;
@@ -1605,8 +1672,11 @@ void free_particles(rp_engine* e) {
   free_ptr(e->d_bstats);
   e->d_bstats = nullptr;
   free_ptr(e->d_surr_base);
+  free_ptr(e->d_surr_rowmask);
+  free_ptr(e->d_surr_nmask);
   free_ptr(e->d_sstats);
   e->d_surr_base = nullptr;
+  e->d_surr_rowmask = e->d_surr_nmask = nullptr;
   e->d_sstats = nullptr;
   e->sstats_valid = false;
   e->d_xb = nullptr;
@@ -1748,7 +1818,7 @@ hipEvent_t next_event(rp_engine* e) {
   return e->ev_pool[e->ev_used++];
 }
 
-// Offset of the next launch's label from the device's label_base.
+// Offset of the next launch's label from the device's label base.
 unsigned long long label_off(const rp_engine* e) { return (unsigned long long)(e->launch_no - e->label_host); }
 
 void mark_begin(rp_engine* e, int kind) {
@@ -1783,9 +1853,13 @@ SurrDev surr_dev(rp_engine* e) {
   sd.f_lne = scale_in((float)std::log(pp.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
   sd.f_lng = scale_in((float)std::log(std::max(pp.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
   sd.base = e->d_surr_base;
+  sd.rowmask = e->d_surr_rowmask;
+  sd.nmask = e->d_surr_nmask;
+  sd.md_lo = v.d_lo - kSurrMaskMargin;
+  sd.md_hi = v.d_hi + kSurrMaskMargin;
   sd.x = e->d_surr_x;
   sd.y = e->d_surr_y;
-  sd.counter = &e->d_ctl->surr_rows;
+  sd.counter = e->d_ctl->surr_rows;
   sd.rows_cap = e->surr_rows_cap;
   return sd;
 }
@@ -1821,9 +1895,9 @@ void launch_force_t(rp_engine* e, int flag_slot) {
                      label_off(e), e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
-                     JS == 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+                     JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
   e->force_grid = g;
-  e->sstats_valid = JS == 2;
+  e->sstats_valid = JS >= 2;
   e->cur ^= 1;
   e->launch_no++;
 }
@@ -1849,10 +1923,13 @@ void launch_force_j(rp_engine* e, int flag_slot) {
   if constexpr (NORMAL != RP_NORMAL_JKR) {
     launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot);
   } else {
-    constexpr int kSurr = REC ? 1 : 2;
+    constexpr int kSurr = REC ? 1 : 2, kSurrCheck = REC ? 1 : 3;
     switch (e->jkr_solver) {
       case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1, false>(e, flag_slot); break;
-      case RP_JKR_SURROGATE: launch_force_t<NORMAL, TANG, ADV, REC, kSurr, false>(e, flag_slot); break;
+      case RP_JKR_SURROGATE:
+        if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, REC, kSurrCheck, false>(e, flag_slot);
+        else launch_force_t<NORMAL, TANG, ADV, REC, kSurr, false>(e, flag_slot);
+        break;
       default: launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot); break;
     }
   }
@@ -1896,7 +1973,7 @@ rp_status capture_steps(rp_engine* e, hipGraphExec_t* exec) {
     if (st == RP_OK && surr_active(e)) launch_surrogate(e);
     if (st == RP_OK) launch_force<true, false>(e, (int)(e->chain_no & 1));
   }
-  hipLaunchKernelGGL(k_advance_label, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (unsigned long long)kGraphSteps);
+  hipLaunchKernelGGL(k_shift_labels, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (unsigned long long)kGraphSteps);
   const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
   e->launch_no = launch0;
   e->chain_no = chain0;
@@ -1925,7 +2002,8 @@ rp_status replay_steps(rp_engine* e) {
     e->graph_gen[slot] = e->gen;
   }
   if (e->label_host != e->launch_no) {
-    hipLaunchKernelGGL(k_set_label, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (unsigned long long)e->launch_no);
+    hipLaunchKernelGGL(k_shift_labels, dim3(1), dim3(1), 0, e->stream, e->d_ctl,
+                       (unsigned long long)(e->launch_no - e->label_host));
     e->label_host = e->launch_no;
   }
   HIP_TRY(hipGraphLaunch(e->graph_exec[slot], e->stream));
@@ -2020,7 +2098,7 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   HIP_TRY(hipStreamSynchronize(e->stream));
   free_particles(e);
   e->gen++;
-  e->label_host = 0;  // the control block below starts with label_base 0
+  e->label_host = 0;  // the control block below has no stall recorded
   int cap = (int)std::max<size_t>(round_up(n, 256), 256);
   e->cap = cap;
   for (int b = 0; b < 2; ++b) {
@@ -2034,6 +2112,8 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMalloc(&e->d_surr_base, (size_t)cap * sizeof(int)));
+  HIP_TRY(hipMalloc(&e->d_surr_rowmask, (size_t)cap * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&e->d_surr_nmask, (size_t)cap * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&e->d_sstats, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
   HIP_TRY(hipMemset(e->d_sstats, 0, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
   rp_status st = alloc_nbr(e, 32);
@@ -2095,7 +2175,7 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   Ctl c{};
   c.n = (int)n;
   c.alive = (int)n;
-  c.stall_from = ~0ull;
+  c.stall_off = kNoStall;
   c.at_rest = 1;
   HIP_TRY(hipMemcpy(e->d_ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
   e->n_uploaded = (int)n;
@@ -2251,9 +2331,10 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
     HIP_TRY(hipStreamSynchronize(e->stream));
     const uint64_t executed = e->host_ctl.steps_done - e->steps_done;
     e->steps_done = e->host_ctl.steps_done;
-    if (surr_active(e) && e->host_ctl.surr_rows_max > (unsigned)e->surr_rows_cap) {
+    if (surr_active(e) && e->host_ctl.surr_rows_max > (unsigned)(e->surr_rows_cap / kSurrRanges)) {
       // some particles' contacts took the exact solve for want of rows: grow
-      const size_t want = round_up((size_t)e->host_ctl.surr_rows_max * 3 / 2 + 1024, 64);
+      const size_t want = round_up((size_t)e->host_ctl.surr_rows_max * kSurrRanges * 3 / 2 + 1024,
+                                   (size_t)64 * kSurrRanges);
       HIP_TRY(hipStreamSynchronize(e->stream));
       if ((st = alloc_surr_rows(e, (int)std::min<size_t>(want, (size_t)1 << 30)))) return st;
     }
@@ -2469,13 +2550,17 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
     if (mlp_upload(e->surr_net, params, e->stream) != RP_OK) return fail(RP_ERR_DEVICE, "surrogate weight upload failed");
     e->surr_env = *env;
     if (e->surr_rows_cap == 0) {
-      rp_status st = alloc_surr_rows(e, 1 << 16);
+      rp_status st = alloc_surr_rows(e, 1 << 17);
       if (st) return st;
     }
   }
   e->jkr_solver = solver;
   e->surr_check = check ? 1 : 0;
   e->gen++;
+  if (e->d_ctl) {  // near masks of an earlier surrogate period are stale: the next collect scans everything
+    const unsigned long long zero = 0ull;
+    HIP_TRY(hipMemcpy(&e->d_ctl->surr_mask_epoch, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
   e->sstats_valid = false;
   return RP_OK;
 }

@@ -180,14 +180,22 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __re
   mlp_tile(net, x, y, rows, blockIdx.x * 4 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
 }
 
-// Row count read on the device; a fixed grid walks the tiles (the loop bound
-// is the same for every wave of a workgroup, so all meet the barriers).
+// Row counts read on the device: range r of [0, cap) holds counter[r] rows.
+// A fixed grid walks 64-row groups (4 tiles, one per wave); a group lies in
+// one range (ranges are multiples of 64 rows), so the skip test is the same
+// for every wave of a workgroup and all meet the barriers.
 __global__ void __launch_bounds__(256) k_mlp_counted(MlpDev net, const float* __restrict__ x,
                                                      float* __restrict__ y, const unsigned* counter, int cap) {
   __shared__ float hs[4][16][kMlpHid + 1];
-  const int rows = (int)min(*counter, (unsigned)cap);
-  for (int t0 = blockIdx.x * 4; t0 * 16 < rows; t0 += gridDim.x * 4)
-    mlp_tile(net, x, y, rows, t0 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
+  const int range_rows = cap / kSurrRanges;
+  for (int g = blockIdx.x; g * 64 < cap; g += gridDim.x) {
+    const int r = (g * 64) / range_rows;
+    const int first = r * range_rows;
+    const int rows = (int)min(counter[r], (unsigned)range_rows);
+    if (g * 64 - first >= rows) continue;
+    mlp_tile(net, x + (size_t)first * kMlpIn, y + (size_t)first * kMlpOut, rows, (g * 64 - first) / 16 + (threadIdx.x >> 6),
+             hs[threadIdx.x >> 6]);
+  }
 }
 
 // ---- training kernels ----------------------------------------------------------
@@ -550,7 +558,7 @@ void mlp_infer(const MlpDev& net, const float* x, float* y, int rows, hipStream_
 
 void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsigned* counter, int cap,
                        hipStream_t stream) {
-  const int blocks = std::max(1, std::min(512, (cap + 63) / 64));
+  const int blocks = std::max(1, std::min(1024, (cap + 63) / 64));
   hipLaunchKernelGGL(k_mlp_counted, dim3(blocks), dim3(256), 0, stream, net, x, y, counter, cap);
 }
 

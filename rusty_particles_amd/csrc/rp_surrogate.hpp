@@ -48,6 +48,10 @@ constexpr double kJkrDMinS = -0.47247039371057744;  // min of alpha^2 - sqrt(alp
 constexpr double kSurrDLoMin = -0.40;  // lowest d served (the slope 2 alpha - 1/(2 sqrt(alpha)) >= 0.6)
 constexpr double kSurrGapHi = 0.20;    // 0 < d < kSurrGapHi keeps the exact solve
 constexpr double kSurrSLo = 1e-3, kSurrSHi = 0.9;  // s / R* range served
+// Feature rows are claimed by workgroups from kSurrRanges separate ranges
+// (workgroup b from range b % kSurrRanges), so the claiming atomics spread
+// over as many addresses.
+constexpr int kSurrRanges = 16;
 
 // Scaling of inputs and outputs (the training envelope).
 struct MlpScale {
@@ -79,10 +83,13 @@ struct SurrDev {
   MlpScale sc;
   float f_lne, f_lng; // scaled ln E*, ln gamma of particle-particle pairs
   int* base;          // [slots] first feature row of each particle, -1 = none
+  unsigned long long* rowmask;  // [slots] neighbour-list entries that have a row (in row order)
+  unsigned long long* nmask;    // [slots] entries near the served set: the next collect visits only these
+  double md_lo, md_hi;  // d range of nmask (the served range widened by a margin)
   float* x;           // [rows_cap x 4] scaled features
   float* y;           // [rows_cap x 2] scaled predictions
-  unsigned* counter;  // rows claimed in this step
-  int rows_cap;
+  unsigned* counter;  // [kSurrRanges] rows claimed in this step, per range
+  int rows_cap;       // rows in all; range r holds rows [r R, r R + R), R = rows_cap / kSurrRanges
 };
 
 // Per-workgroup surrogate counters of one force launch.
@@ -140,7 +147,8 @@ void mlp_free(MlpHost& m);
 // scaled) and writes 2 scaled outputs per row.  x and y are device pointers.
 void mlp_infer(const MlpDev& net, const float* x, float* y, int rows, hipStream_t stream);
 
-// The same on the first min(*counter, cap) rows, the count read on the device.
+// The same on the claimed rows of each of the kSurrRanges ranges of
+// [0, cap): counter[r] rows from r cap / kSurrRanges (counts read on the device).
 void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsigned* counter, int cap,
                        hipStream_t stream);
 

@@ -158,4 +158,6 @@ def test_surrogate_trajectory_tracks_exact(net):
     su = _run(p, sc, "surrogate", net, steps=300)
     xe, xs = ex.download()["x"], su.download()["x"]
     assert np.max(np.abs(xe - xs)) < 1e-9 * sc["radius"].min()
-    assert su.jkr_stats()["served"] > 0
+    st = su.jkr_stats()
+    print(st)
+    assert st["served"] > 0 and st["fallback"] <= st["served"] // 100
