@@ -164,6 +164,7 @@ EXPORTS = (
     "rp_engine_set_jkr_solver",
     "rp_engine_upload_batch",
     "rp_engine_set_graph",
+    "rp_engine_autotune",
     "rp_engine_instance_counts",
     "rp_engine_jkr_stats",
     "rp_engine_destroy",

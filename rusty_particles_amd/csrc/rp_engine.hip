@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2589,6 +2590,84 @@ rp_status rp_engine_jkr_stats(rp_engine* e, rp_jkr_stats* out) {
     out->err_mean = es / (double)out->served;
     out->raw_mean = rs / (double)out->served;
   }
+  return RP_OK;
+}
+
+rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, double* cell_out) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (steps == 0) return fail(RP_ERR_INVALID, "steps must be positive");
+  rp_status st = prepare(e);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  // snapshot of everything a step reads: state buffers, control block, host
+  // counters (the lists are rebuilt from the state)
+  const size_t db = dyn_bytes(e->cap), sb = stat_bytes(e->cap);
+  void *sdyn = nullptr, *sstat = nullptr;
+  HIP_TRY(hipMalloc(&sdyn, db));
+  HIP_TRY(hipMalloc(&sstat, sb));
+  const int cur0 = e->cur;
+  HIP_TRY(hipMemcpy(sdyn, e->d_dyn_mem[cur0], db, hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(sstat, e->d_stat_mem[0], sb, hipMemcpyDeviceToDevice));
+  Ctl c0{};
+  HIP_TRY(hipMemcpy(&c0, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  const uint64_t steps_done0 = e->steps_done, step_count0 = e->step_count, launch0 = e->launch_no,
+                 label0 = e->label_host, chain0 = e->chain_no;
+  const bool at_rest0 = e->at_rest, timing0 = e->timing;
+  const double skin0 = e->prm.skin, cell0 = e->prm.cell_size;
+  auto restore = [&]() -> rp_status {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->cur = cur0;
+    HIP_TRY(hipMemcpy(e->d_dyn_mem[cur0], sdyn, db, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(e->d_stat_mem[0], sstat, sb, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(e->d_ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice));
+    e->steps_done = steps_done0;
+    e->step_count = step_count0;
+    e->launch_no = launch0;
+    e->label_host = label0;
+    e->chain_no = chain0;
+    e->at_rest = at_rest0;
+    e->geometry_dirty = true;  // lists and grid are rebuilt from the restored state
+    return RP_OK;
+  };
+  // candidates: skin as a fraction of the largest radius (never below the
+  // motion bound the automatic skin uses), automatic or half-cutoff cells
+  const double vmin = 4.0 * (double)kRebuildInterval * e->v_max * e->prm.dt;
+  const double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
+  const double fr[] = {0.2, 0.1, 0.05, 0.02, 0.01, 0.005};
+  const double cut = e->kp.margin_factor * 2.0 * rmax;
+  double best_t = HUGE_VAL, best_skin = skin0, best_cell = cell0;
+  e->timing = false;
+  for (double f : fr) {
+    const double skin = std::max(f * rmax, vmin);
+    for (int cm = 0; cm < 2; ++cm) {
+      const double cell = cm == 0 ? 0.0 : 0.5 * (cut + skin);
+      if ((st = restore())) break;
+      e->prm.skin = skin;
+      e->prm.cell_size = cell;
+      if ((st = rp_engine_step(e, 2))) break;
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((st = rp_engine_step(e, steps))) break;
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (t < best_t) {
+        best_t = t;
+        best_skin = skin;
+        best_cell = cell;
+      }
+    }
+    if (st) break;
+  }
+  e->timing = timing0;
+  rp_status rs = restore();
+  e->prm.skin = st ? skin0 : best_skin;
+  e->prm.cell_size = st ? cell0 : best_cell;
+  (void)hipFree(sdyn);
+  (void)hipFree(sstat);
+  if (st) return st;
+  if (rs) return rs;
+  if (skin_out) *skin_out = best_skin;
+  if (cell_out) *cell_out = best_cell;
   return RP_OK;
 }
 

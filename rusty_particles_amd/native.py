@@ -61,6 +61,7 @@ def lib() -> C.CDLL:
                                          C.POINTER(abi.RpMaterial), _dp, _dp, _dp, _dp, _dp, _u8p]
     L.rp_engine_instance_counts.argtypes = [_eng, C.POINTER(C.c_size_t)]
     L.rp_engine_set_graph.argtypes = [_eng, C.c_int]
+    L.rp_engine_autotune.argtypes = [_eng, C.c_uint32, _dp, _dp]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
     L.rp_last_error.argtypes = []
@@ -70,7 +71,8 @@ def lib() -> C.CDLL:
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
-                 "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph"):
+                 "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph",
+                 "rp_engine_autotune"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -231,6 +233,13 @@ class Engine:
         _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
                                        C.byref(tot)))
         return offs, js[: tot.value]
+
+    def autotune(self, steps: int = 20) -> tuple[float, float]:
+        """Pick the fastest Verlet skin and cell size for the current state
+        (the state itself is restored); returns (skin, cell_size), 0 = automatic."""
+        skin, cell = C.c_double(0.0), C.c_double(0.0)
+        _check(self._L.rp_engine_autotune(self._h, int(steps), C.byref(skin), C.byref(cell)))
+        return skin.value, cell.value
 
     def set_graph(self, on: bool) -> None:
         """Replay captured 20-launch step loops (default on)."""

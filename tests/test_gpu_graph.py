@@ -63,3 +63,23 @@ def test_graph_with_surrogate():
     a, _ = _run(p, sc, True, [3, 95], solver="surrogate", net=net)
     b, _ = _run(p, sc, False, [3, 95], solver="surrogate", net=net)
     _same(a, b)
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner restores the state it tried candidates on, and the
+    skin / cell size it picks change no result."""
+    sc = dense_scene(counts=(12, 12, 12), seed=23, vel=0.2)
+    p = params(1e-5, (-0.08,) * 3, (0.08,) * 3)
+    v9, n3 = scenes.box_mesh((-0.07,) * 3, (0.07,) * 3)
+    e = native.Engine(copy_params(p))
+    e.upload(sc["x"], sc["radius"], sc["mass"], v=sc["v"], w=sc["w"])
+    e.add_mesh(v9, n3)
+    e.step(13)
+    before = e.download()
+    skin, cell = e.autotune(10)
+    assert skin > 0.0
+    _same(before, e.download())
+    e.step(120)
+    ref, _ = _run(p, sc, True, [13, 120], [(v9, n3)])
+    _same(ref, e.download())
+    assert e.stats()["steps"] == 133
