@@ -108,6 +108,9 @@ def run_batch(args) -> int:
         e.set_graph(graph)
         e.step(args.warmup)
         e.synchronize()
+        if args.autotune:
+            sk, ce = e.autotune(args.autotune)
+            log(f"batched autotune: skin {sk:.3g} m, cell {ce:.3g} m")
         t0 = time.perf_counter()
         e.step(args.steps)
         e.synchronize()
@@ -218,6 +221,8 @@ def main() -> int:
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--autotune", type=int, default=20,
+                    help="steps per candidate of the engine's skin / cell-size autotuner before timing (0 = off)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -271,6 +276,12 @@ def main() -> int:
         eng.step(args.warmup)
         eng.synchronize()
         log(f"[{form}] warmup {args.warmup} steps done")
+        tuned = None
+        if args.autotune:
+            t0 = time.perf_counter()
+            tuned = eng.autotune(args.autotune)
+            log(f"[{form}] autotune: skin {tuned[0]:.3g} m, cell {tuned[1]:.3g} m (0 = automatic), "
+                f"{time.perf_counter() - t0:.1f}s")
 
         # timed region: whole job, K steps, one enqueue, synchronised on both sides
         eng.synchronize()
@@ -293,7 +304,8 @@ def main() -> int:
         achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
                "achieved_gbs": achieved, "contacts_per_step": st["contacts"],
-               "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st}
+               "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st,
+               "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None}
         if tm["surrogate"][1]:
             res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
         log(f"[{form}] force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
@@ -364,6 +376,7 @@ def main() -> int:
         "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
         "jkr_failures_per_step": st["jkr_failures"],
+        "autotuned_skin_m": R["autotuned_skin_m"], "autotuned_cell_m": R["autotuned_cell_m"],
     }
     if len(results) > 1 or main_form != "plain":
         out["forms"] = {f: {k: v for k, v in res.items() if k not in ("stats", "wall")} for f, res in results.items()}
