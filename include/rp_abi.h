@@ -237,13 +237,16 @@ rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
 rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
-/* Autotuner (no counterpart in the reference): times `steps` steps of the
- * current state for each candidate Verlet skin (0.2 .. 0.005 of the largest
- * radius, never below the automatic skin's motion bound) and broad-phase
- * cell (automatic or half the largest cutoff + skin), keeps the fastest in
- * the engine's parameters and returns it; the particle state is restored,
- * so the simulation continues as if the call had not been made (skin and
- * cell size change only which candidates the lists hold, never a result). */
+/* Autotuner (no counterpart in the reference): from the current state,
+ * times each candidate Verlet skin (the current one, then 0.2 .. 0.005 of
+ * the largest radius, never below the automatic skin's motion bound) and
+ * broad-phase cell (automatic or half the largest cutoff + skin) over chunks
+ * of `steps` steps until it has rebuilt its lists 3 times (at most 10
+ * chunks), keeps the fastest in the engine's parameters (the current one
+ * unless beaten by 3 %) and returns it (0 = automatic).  The particle state is
+ * restored, so the simulation continues as if the call had not been made
+ * (skin and cell size change only which candidates the lists hold, never a
+ * result). */
 rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, double* cell_out);
 
 /* Step loop in HIP graphs (on by default; no counterpart in the reference):

@@ -2629,34 +2629,48 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
     e->geometry_dirty = true;  // lists and grid are rebuilt from the restored state
     return RP_OK;
   };
-  // candidates: skin as a fraction of the largest radius (never below the
-  // motion bound the automatic skin uses), automatic or half-cutoff cells
+  // candidates: the current setting first, then skins as a fraction of the
+  // largest radius (never below the motion bound the automatic skin uses)
+  // with automatic or half-cutoff cells.  Each runs in chunks of `steps`
+  // until it has rebuilt its lists 3 times or run 10 chunks, so a small skin
+  // pays for its frequent rebuilds; another setting must beat the current one
+  // by 3 % to replace it.
   const double vmin = 4.0 * (double)kRebuildInterval * e->v_max * e->prm.dt;
   const double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
   const double fr[] = {0.2, 0.1, 0.05, 0.02, 0.01, 0.005};
   const double cut = e->kp.margin_factor * 2.0 * rmax;
-  double best_t = HUGE_VAL, best_skin = skin0, best_cell = cell0;
-  e->timing = false;
+  std::vector<std::pair<double, double>> cand{{skin0, cell0}};
   for (double f : fr) {
     const double skin = std::max(f * rmax, vmin);
-    for (int cm = 0; cm < 2; ++cm) {
-      const double cell = cm == 0 ? 0.0 : 0.5 * (cut + skin);
-      if ((st = restore())) break;
-      e->prm.skin = skin;
-      e->prm.cell_size = cell;
-      if ((st = rp_engine_step(e, 2))) break;
-      HIP_TRY(hipStreamSynchronize(e->stream));
+    cand.emplace_back(skin, 0.0);
+    cand.emplace_back(skin, 0.5 * (cut + skin));
+  }
+  double best_t = HUGE_VAL, best_skin = skin0, best_cell = cell0;
+  e->timing = false;
+  for (size_t q = 0; q < cand.size() && st == RP_OK; ++q) {
+    if ((st = restore())) break;
+    e->prm.skin = cand[q].first;
+    e->prm.cell_size = cand[q].second;
+    if ((st = rp_engine_step(e, 2))) break;
+    if ((st = read_ctl(e))) break;
+    const unsigned long long r0 = e->host_ctl.rebuilds;
+    double t = 0.0;
+    uint64_t done = 0;
+    for (int chunk = 0; chunk < 10; ++chunk) {
       const auto t0 = std::chrono::steady_clock::now();
       if ((st = rp_engine_step(e, steps))) break;
-      HIP_TRY(hipStreamSynchronize(e->stream));
-      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (t < best_t) {
-        best_t = t;
-        best_skin = skin;
-        best_cell = cell;
-      }
+      if ((st = read_ctl(e))) break;  // synchronises
+      t += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      done += steps;
+      if (e->host_ctl.rebuilds - r0 >= 3) break;
     }
     if (st) break;
+    const double per = t / (double)done * (q == 0 ? 0.97 : 1.0);
+    if (per < best_t) {
+      best_t = per;
+      best_skin = cand[q].first;
+      best_cell = cand[q].second;
+    }
   }
   e->timing = timing0;
   rp_status rs = restore();
