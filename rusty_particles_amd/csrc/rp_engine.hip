@@ -936,42 +936,6 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // Contacts of particle i: its neighbour list (loads issued kNbrBatch at a
 // time, so a wave waits for memory a few times per particle instead of once
 // per neighbour) and the triangles binned to its cell.
-template <int NORMAL, int TANG, bool RECORD, int JS>
-__device__ inline void gather(const KParams& P, const MaterialConsts& mc, double far_pp, double far_pw, const Dyn& S,
-                              const Stat& T, const Own& o, int i, int cnt, const int* nbr, const Tri* tris,
-                              uint32_t t0, uint32_t t1, const uint32_t* tri_idx, Acc& acc, unsigned int& nwall,
-                              Ctl* ctl, unsigned long long* prow, int* paircount, const SurrDev& sd) {
-  const int cap = P.cap;
-  for (int base = 0; base < cnt; base += kNbrBatch) {
-    int jj[kNbrBatch];
-    dv3 xq[kNbrBatch];
-    double rq[kNbrBatch];
-#pragma unroll
-    for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
-#pragma unroll
-    for (int q = 0; q < kNbrBatch; ++q) {
-      const int j = jj[q] < 0 ? i : jj[q];
-      ldxr(S.xr, j, xq[q], rq[q]);
-    }
-    int hit = 0;
-#pragma unroll
-    for (int q = 0; q < kNbrBatch; ++q) {
-      const dv3 d = pair_delta(P, o.x, xq[q]);
-      if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
-    }
-    while (hit) {
-      const int q = __ffs(hit) - 1;
-      hit &= hit - 1;
-      const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
-                                   : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-      contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
-    }
-  }
-  if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
-  for (uint32_t q = t0; q < t1; ++q)
-    contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
-}
-
 // Gather force kernel + integration epilogue, one thread per particle slot.
 // Reads Dyn buffer S and writes buffer D (so no thread can read a neighbour's
 // next-step state); the host swaps the two between steps.  All own-state loads are issued before the neighbour loop, and
@@ -1054,14 +1018,40 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
         acc.srow = sd.base[i];
         if (acc.srow >= 0) acc.rmask = sd.rowmask[i];
       }
-      if constexpr (BATCH) {
-        const InstConsts& ic = P.itab[T.inst[i]];
-        gather<NORMAL, TANG, RECORD, JS>(P, ic.mc, ic.jkr_far_pp, ic.jkr_far_pw, S, T, o, i, cnt, nbr, tris, t0, t1,
-                                         tri_idx, acc, nwall, ctl, prow, paircount, sd);
-      } else {
-        gather<NORMAL, TANG, RECORD, JS>(P, P.mc, P.jkr_far_pp, P.jkr_far_pw, S, T, o, i, cnt, nbr, tris, t0, t1,
-                                         tri_idx, acc, nwall, ctl, prow, paircount, sd);
+      const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
+      const MaterialConsts& mc = BATCH ? ic->mc : P.mc;
+      const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
+      const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
+      for (int base = 0; base < cnt; base += kNbrBatch) {
+        int jj[kNbrBatch];
+        dv3 xq[kNbrBatch];
+        double rq[kNbrBatch];
+    #pragma unroll
+        for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
+    #pragma unroll
+        for (int q = 0; q < kNbrBatch; ++q) {
+          const int j = jj[q] < 0 ? i : jj[q];
+          ldxr(S.xr, j, xq[q], rq[q]);
+        }
+        int hit = 0;
+    #pragma unroll
+        for (int q = 0; q < kNbrBatch; ++q) {
+          const dv3 d = pair_delta(P, o.x, xq[q]);
+          if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
+        }
+        while (hit) {
+          const int q = __ffs(hit) - 1;
+          hit &= hit - 1;
+          const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
+                                       : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
+          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
+        }
       }
+      if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
+      for (uint32_t q = t0; q < t1; ++q)
+        contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
+
+
 
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
