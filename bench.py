@@ -336,10 +336,12 @@ def main() -> int:
     R = results[main_form]
     value, wall, st, avg_s, achieved = R["value"], R["wall"], R["stats"], R["force_kernel_avg_us"] * 1e-6, \
         R["achieved_gbs"]
-    if args.calibrate_copy:
-        ms = native.probe_copy(256 * 1024 * 1024)
-        log(f"copy calibration: k_copy_probe read 268435456 B + wrote 268435456 B in {ms:.3f} ms "
-            f"({2 * 268435456 / ms / 1e6:.0f} GB/s)")
+    # what a plain streaming copy reaches on this GPU (context for the roofline;
+    # with --calibrate-copy also the reference point of a counter profile)
+    nbytes = (1 << 30) if not args.calibrate_copy else 256 * 1024 * 1024
+    ms = native.probe_copy(nbytes)
+    copy_gbs = 2 * nbytes / ms / 1e6
+    log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{args.workload}_force_traffic.json")
     if os.path.exists(tfile):
@@ -375,6 +377,7 @@ def main() -> int:
                    "particles": n, "triangles": ntri, "form": main_form},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "contacts_per_step": st["contacts"],
         "wall_contacts_per_step": st["wall_contacts"],

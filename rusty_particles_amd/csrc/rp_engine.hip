@@ -64,6 +64,10 @@ rp_status fail(rp_status code, const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
+#ifndef RP_FORCE_BLOCK
+#define RP_FORCE_BLOCK 256
+#endif
+constexpr int kForceBlock = RP_FORCE_BLOCK;  // workgroup of the force kernel
 constexpr uint8_t FLAG_FIXED = 1;
 constexpr uint8_t FLAG_DEAD = 2;
 constexpr int kScanItems = 16;                   // items per thread in the scan
@@ -947,7 +951,7 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
 template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
+__global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
                                                   const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
@@ -1096,7 +1100,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
   // atomic, issued by a wave only when one of its particles needs it
   const unsigned int c1 = wave_sum_u32(acc.ncont), c2 = wave_sum_u32(nwall), c3 = wave_sum_u32(acc.njkr);
   const bool anyo = __any(acc.overlap), anyf = __any(far);
-  __shared__ unsigned int red[kBlock / 64][4];
+  __shared__ unsigned int red[kForceBlock / 64][4];
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[wv][0] = c1;
@@ -1108,7 +1112,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
   __syncthreads();
   if (threadIdx.x == 0) {
     BlockStats b{0u, 0u, 0u, 0u};
-    for (int q = 0; q < kBlock / 64; ++q) {
+    for (int q = 0; q < kForceBlock / 64; ++q) {
       b.contacts += red[q][0];
       b.wall_contacts += red[q][1];
       b.jkr_failures += red[q][2];
@@ -1127,12 +1131,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Ctl* ctl, int flag_slot, unsig
       es += __shfl_down(es, off, 64);
       rs += __shfl_down(rs, off, 64);
     }
-    __shared__ SurrStats sred[kBlock / 64];
+    __shared__ SurrStats sred[kForceBlock / 64];
     if ((threadIdx.x & 63) == 0) sred[wv] = SurrStats{s1, s2, em, rm, es, rs};
     __syncthreads();
     if (threadIdx.x == 0) {
       SurrStats b = sred[0];
-      for (int q = 1; q < kBlock / 64; ++q) {
+      for (int q = 1; q < kForceBlock / 64; ++q) {
         b.served += sred[q].served;
         b.fallback += sred[q].fallback;
         b.err_max = fmaxf(b.err_max, sred[q].err_max);
@@ -1265,10 +1269,15 @@ __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long 
 
 // Plain streaming copy of known size: reference point for hardware traffic
 // counters (FETCH_SIZE / WRITE_SIZE) measured in the same profile.
+// 16-byte loads and stores, grid-stride: what a streaming kernel can reach.
 __global__ void __launch_bounds__(kBlock) k_copy_probe(const double* __restrict__ src, double* __restrict__ dst,
                                                        size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+  const size_t n2 = n / 2;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+    d2[i] = s2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
 // Moves the label base forward by `by` steps: offsets are relative to it.
@@ -1365,7 +1374,7 @@ struct rp_engine {
   float* d_surr_x = nullptr;   // [surr_rows_cap x 4]
   float* d_surr_y = nullptr;   // [surr_rows_cap x 2]
   int surr_rows_cap = 0;
-  SurrStats* d_sstats = nullptr;  // [cap / kBlock + 1]
+  SurrStats* d_sstats = nullptr;  // [cap / kForceBlock + 1]
   bool sstats_valid = false;     // the last force launch wrote d_sstats
 };
 
@@ -1881,8 +1890,8 @@ void launch_surrogate(rp_engine* e) {
 
 template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
 void launch_force_t(rp_engine* e, int flag_slot) {
-  int g = (int)((e->cap + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, flag_slot,
+  int g = (int)((e->cap + kForceBlock - 1) / kForceBlock);
+  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
                      label_off(e), e->kp,
                      e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                      e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
@@ -2100,13 +2109,13 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   HIP_TRY(hipMalloc(&e->d_key, (size_t)cap * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_tmp, (size_t)cap * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * sizeof(int)));
-  HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
-  HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kBlock + 1) * sizeof(BlockStats)));
+  HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
+  HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMalloc(&e->d_surr_base, (size_t)cap * sizeof(int)));
   HIP_TRY(hipMalloc(&e->d_surr_rowmask, (size_t)cap * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&e->d_surr_nmask, (size_t)cap * sizeof(unsigned long long)));
-  HIP_TRY(hipMalloc(&e->d_sstats, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
-  HIP_TRY(hipMemset(e->d_sstats, 0, ((size_t)cap / kBlock + 1) * sizeof(SurrStats)));
+  HIP_TRY(hipMalloc(&e->d_sstats, ((size_t)cap / kForceBlock + 1) * sizeof(SurrStats)));
+  HIP_TRY(hipMemset(e->d_sstats, 0, ((size_t)cap / kForceBlock + 1) * sizeof(SurrStats)));
   rp_status st = alloc_nbr(e, 32);
   if (st) return st;
   // host staging in the device layout: Dyn = xr records + 7 vec3 fields as
@@ -2718,13 +2727,18 @@ rp_status rp_probe_copy(size_t bytes, double* ms) {
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, nullptr));
-  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, nullptr, a, b, n);
-  HIP_TRY(hipEventRecord(e1, nullptr));
-  HIP_TRY(hipEventSynchronize(e1));
-  float t = 0.f;
-  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-  *ms = t;
+  // the fastest of 5 launches (the first pays for first-touch effects)
+  float best = 0.f;
+  for (int rep = 0; rep < 5; ++rep) {
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(kBlock), 0, nullptr, a, b, n);
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (rep == 0 || t < best) best = t;
+  }
+  *ms = best;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIP_TRY(hipFree(a));
