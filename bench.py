@@ -77,6 +77,16 @@ def build_workload(name: str, n: int | None):
     raise SystemExit(f"unknown workload {name}")
 
 
+def traffic_of(workload: str):
+    """HBM bytes per launch of the workload's force kernel from its committed
+    counter profile (FETCH_SIZE / WRITE_SIZE scaled by the copy probe), or None."""
+    tfile = os.path.join(ROOT, "profiles", f"{workload}_force_traffic.json")
+    if not os.path.exists(tfile):
+        return None
+    with open(tfile) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
 def run_batch(args) -> int:
     """Batched ensemble against one engine per instance."""
     import numpy as np
@@ -172,7 +182,7 @@ def run_batch(args) -> int:
         "config": {"workload": f"batch: {len(counts)} instances, {n} spheres in all, sJKR+Coulomb, material sweep, "
                                f"dt={p.dt:g}, FP64", "particles": n, "instances": len(counts), "form": "batched"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of("batch")},
         "cpu_baseline": cpu,
         "contacts_per_step": st["contacts"], "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
@@ -342,11 +352,7 @@ def main() -> int:
     ms = native.probe_copy(nbytes)
     copy_gbs = 2 * nbytes / ms / 1e6
     log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{args.workload}_force_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    traffic = traffic_of(args.workload)
 
     cpu = None
     if not args.no_cpu:
