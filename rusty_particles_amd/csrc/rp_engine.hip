@@ -114,6 +114,7 @@ struct Stat {
 struct InstConsts {
   MaterialConsts mc;
   double jkr_far_pp, jkr_far_pw;
+  double sjkr_zero_pp;
 };
 
 // Device-resident control block: slot count and rare flags.  Per-step
@@ -166,6 +167,7 @@ struct KParams {
   // reference grid (grid.rs:12-22, cell 1.0 m at simulation.rs:66)
   double ref_origin[3];
   double ref_cs;
+  double ref_inv_cs;  // 1 / ref_cs (rounded): fast path of ref_coord_fast
   int ref_dims[3];
   // fine grid
   double go[3];      // origin
@@ -184,6 +186,10 @@ struct KParams {
   // JKR: every contact with overlap below this (less 1e-8 |overlap|) has no
   // root (jkr_no_root holds for any R* up to r_max / 2, resp. r_max for walls)
   double jkr_far_pp, jkr_far_pw;
+  // sJKR: every particle-particle overlap below this has zero force
+  // (below the cut-off -sqrt(3 pi^2 gamma^2 R* / E*^2) of any R* up to r_max / 2;
+  // 0 when gamma <= 0, where only overlap > 0 has force)
+  double sjkr_zero_pp;
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
   int pair_cap;      // recorded pair entries per particle
@@ -261,6 +267,31 @@ __device__ inline int ref_mask(int ci, int cj, const KParams& P) {
     if (nn == cj) msk |= 1 << (o + 1);
   }
   return msk;
+}
+
+// The same cell without a division where that is exact: q = (x - origin) *
+// RN(1/cs) lies within 4e-16 |q| of the rounded quotient, so when q is more
+// than 1e-9 from an integer (and |q| < 1e6) both have the same floor.
+template <int K>
+__device__ inline int ref_coord_fast(double x, const KParams& P) {
+  const double a = x - P.ref_origin[K];
+  const double q = a * P.ref_inv_cs;
+  const double f = floor(q);
+  const double fr = q - f;
+  if (fabs(q) < 1.0e6 && fr > 1.0e-9 && fr < 1.0 - 1.0e-9) return (int)f;
+  return sat_i32(floor(a / P.ref_cs));
+}
+
+// ref_mask by the difference cj - ci when ci lies inside the grid (cj always
+// does): offset o reaches ci + o, wrapped once on a periodic axis.
+template <int K>
+__device__ inline int ref_mask_fast(int ci, int cj, const KParams& P) {
+  const int d = P.ref_dims[K];
+  if (ci < 0 || ci >= d) return ref_mask<K>(ci, cj, P);
+  const int diff = cj - ci;
+  int m = (diff == -1 ? 1 : 0) | (diff == 0 ? 2 : 0) | (diff == 1 ? 4 : 0);
+  if (P.periodic[K]) m |= (diff == d - 1 ? 1 : 0) | (diff == 1 - d ? 4 : 0);
+  return m;
 }
 
 // XCD-aware block index: consecutive logical blocks land on the same XCD
@@ -958,6 +989,12 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
                                                   unsigned long long* pairbuf, int* paircount, SurrDev sd,
                                                   SurrStats* sstats) {
   const int n = ctl->n;
+  // JKR past its no-root bound (shortcut and surrogate solvers) and sJKR past
+  // its cut-off have zero normal force; with Coulomb or the spring law (G* = 0)
+  // such a contact adds nothing and is only counted.  Mindlin propagates its
+  // NaN for negative overlaps, and recording needs every pair in order.
+  constexpr bool kSplit = !RECORD && TANG != RP_TANGENTIAL_MINDLIN &&
+                          ((NORMAL == RP_NORMAL_JKR && JS >= 1) || NORMAL == RP_NORMAL_SJKR);
   // the rows of this step have been read by the MLP kernel; the next step's
   // collect kernel starts from zero
   if (JS >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1026,29 +1063,92 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       const MaterialConsts& mc = BATCH ? ic->mc : P.mc;
       const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
       const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
-      for (int base = 0; base < cnt; base += kNbrBatch) {
-        int jj[kNbrBatch];
-        dv3 xq[kNbrBatch];
-        double rq[kNbrBatch];
-    #pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
-    #pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) {
-          const int j = jj[q] < 0 ? i : jj[q];
-          ldxr(S.xr, j, xq[q], rq[q]);
+      if constexpr (kSplit) {
+        // Two passes.  First, the list entries below 64: a contact whose
+        // overlap is past the model's zero-force threshold adds exactly zero
+        // (see contact_pp), so it is only counted (reference-grid
+        // multiplicity without divisions); the others are marked.  Then the
+        // marked entries and the entries from 64 on get the full contact, in
+        // list order, so the force sums keep the reference's order.
+        const double zthr = NORMAL == RP_NORMAL_JKR ? far_pp
+                                                    : (BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp);
+        const bool zon = NORMAL == RP_NORMAL_SJKR || mc.pp.gamma > 0.0;
+        const int ka = min(cnt, 64);
+        unsigned long long defer = 0ull;
+        for (int base = 0; base < ka; base += kNbrBatch) {
+          int jj[kNbrBatch];
+          dv3 xq[kNbrBatch];
+          double rq[kNbrBatch];
+#pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < ka) ? nbr[(base + q) * cap + i] : -1;
+#pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) {
+            const int j = jj[q] < 0 ? i : jj[q];
+            ldxr(S.xr, j, xq[q], rq[q]);
+          }
+#pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) {
+            const dv3 d = pair_delta(P, o.x, xq[q]);
+            const double dsq = dot(d, d);
+            const double rsum = o.r + rq[q];
+            if (!(jj[q] >= 0 && in_contact(P, dsq, rsum))) continue;
+            const double ov = rsum - sqrt(dsq);
+            const bool zero = zon && (NORMAL == RP_NORMAL_JKR ? ov < zthr - 1e-8 * fabs(ov) : ov < zthr);
+            if (!zero) {
+              defer |= 1ull << (base + q);
+              continue;
+            }
+            const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
+                      cj2 = ref_coord_fast<2>(xq[q].z, P);
+            if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
+                cj2 >= P.ref_dims[2])
+              continue;
+            const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(o.rc0, cj0, P)) *
+                                                     __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
+                                                     __popc(ref_mask_fast<2>(o.rc2, cj2, P)));
+            acc.ncont += mult;
+            if (NORMAL == RP_NORMAL_JKR) acc.njkr += mult;
+          }
         }
-        int hit = 0;
-    #pragma unroll
-        for (int q = 0; q < kNbrBatch; ++q) {
-          const dv3 d = pair_delta(P, o.x, xq[q]);
-          if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
+        while (defer) {
+          const int k = __ffsll((long long)defer) - 1;
+          defer &= defer - 1ull;
+          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, nbr[k * cap + i], k, acc, ctl, prow, sd);
         }
-        while (hit) {
-          const int q = __ffs(hit) - 1;
-          hit &= hit - 1;
-          const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
-                                       : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
+        for (int k = ka; k < cnt; ++k) {
+          const int j = nbr[k * cap + i];
+          dv3 xj;
+          double rj;
+          ldxr(S.xr, j, xj, rj);
+          const dv3 d = pair_delta(P, o.x, xj);
+          if (in_contact(P, dot(d, d), o.r + rj))
+            contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, k, acc, ctl, prow, sd);
+        }
+      } else {
+        for (int base = 0; base < cnt; base += kNbrBatch) {
+          int jj[kNbrBatch];
+          dv3 xq[kNbrBatch];
+          double rq[kNbrBatch];
+      #pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < cnt) ? nbr[(base + q) * cap + i] : -1;
+      #pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) {
+            const int j = jj[q] < 0 ? i : jj[q];
+            ldxr(S.xr, j, xq[q], rq[q]);
+          }
+          int hit = 0;
+      #pragma unroll
+          for (int q = 0; q < kNbrBatch; ++q) {
+            const dv3 d = pair_delta(P, o.x, xq[q]);
+            if (jj[q] >= 0 && in_contact(P, dot(d, d), o.r + rq[q])) hit |= 1 << q;
+          }
+          while (hit) {
+            const int q = __ffs(hit) - 1;
+            hit &= hit - 1;
+            const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
+                                         : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
+            contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
+          }
         }
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
@@ -1409,6 +1509,15 @@ double jkr_far(const PairConsts& c, double rs) {
   return (kJkrDMin * t - 1e-3 * rs - 1e-8 * (rs + t)) * (1.0 + 1e-12);
 }
 
+// sJKR's force is zero below dc = -sqrt(3 pi^2 gamma^2 R* / E*^2) (physics.rs:
+// 256-262), which falls with R*: the value at the largest R*, widened by
+// 1e-9 for rounding, bounds every contact's.  With gamma <= 0 the Hertz
+// branch has force only for overlap > 0.
+double sjkr_zero(const PairConsts& c, double rs) {
+  if (!(c.gamma > 0.0)) return 0.0;
+  return -std::sqrt(c.sterm * rs / c.estar_sq) * (1.0 + 1e-9);
+}
+
 rp_status validate(const rp_params* p) {
   if (!p) return fail(RP_ERR_INVALID, "params is NULL");
   if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(RP_ERR_INVALID, "dt must be positive and finite");
@@ -1447,6 +1556,7 @@ rp_status compute_geometry(rp_engine* e) {
   K.mc = material_consts(p.particle, p.wall);
   K.margin_factor = (p.normal_model == RP_NORMAL_JKR || p.normal_model == RP_NORMAL_SJKR) ? 1.5 : 1.0;
   K.ref_cs = p.ref_cell_size;
+  K.ref_inv_cs = 1.0 / p.ref_cell_size;
   for (int k = 0; k < 3; ++k) K.ref_dims[k] = sat_i32(std::ceil(K.bsize[k] / K.ref_cs)) + 1;
 
   // fine grid
@@ -1457,6 +1567,7 @@ rp_status compute_geometry(rp_engine* e) {
   // threshold of the largest possible R* bounds every contact's from below
   K.jkr_far_pp = jkr_far(K.mc.pp, 0.5 * rmax);
   K.jkr_far_pw = jkr_far(K.mc.pw, rmax);
+  K.sjkr_zero_pp = sjkr_zero(K.mc.pp, 0.5 * rmax);
   // per-instance constants of a batched engine
   if (e->ninst > 1) {
     std::vector<InstConsts> tab((size_t)e->ninst);
@@ -1464,6 +1575,7 @@ rp_status compute_geometry(rp_engine* e) {
       tab[q].mc = material_consts(e->inst_particle[q], e->inst_wall[q]);
       tab[q].jkr_far_pp = jkr_far(tab[q].mc.pp, 0.5 * rmax);
       tab[q].jkr_far_pw = jkr_far(tab[q].mc.pw, rmax);
+      tab[q].sjkr_zero_pp = sjkr_zero(tab[q].mc.pp, 0.5 * rmax);
     }
     free_ptr(e->d_itab);
     e->d_itab = nullptr;
