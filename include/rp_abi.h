@@ -204,14 +204,17 @@ rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3,
  * axes, models, gravity, meshes) but have their own particle and wall
  * materials, stepped together by every rp_engine_step -- one set of kernel
  * launches over all instances' particles.  counts[k] particles of instance k
- * follow those of instance k-1 in the arrays; wall may be NULL (the
- * engine's wall material for all).  Particles of different instances never
+ * follow those of instance k-1 in the arrays (the state arrays as in
+ * rp_engine_upload_full; any but x3, radius and mass may be NULL); wall may
+ * be NULL (the engine's wall material for all).  Particles of different instances never
  * interact.  Each instance evolves exactly as a single engine given that
  * instance alone.  Downloads list the instances in order, each in API
  * order.  Not available with pair recording or the JKR surrogate. */
 rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* counts,
                                  const rp_material* particle, const rp_material* wall,
                                  const double* x3, const double* v3, const double* w3,
+                                 const double* a3, const double* tau3, const double* rx3,
+                                 const double* rv3, const double* rw3,
                                  const double* radius, const double* mass, const uint8_t* fixed);
 
 /* Alive particles per instance (ninst entries; 1 for a single engine). */

@@ -2319,7 +2319,9 @@ rp_status rp_engine_upload_full(rp_engine* e, size_t n, const double* x3, const 
 
 rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* counts, const rp_material* particle,
                                  const rp_material* wall, const double* x3, const double* v3, const double* w3,
-                                 const double* radius, const double* mass, const uint8_t* fixed) {
+                                 const double* a3, const double* tau3, const double* rx3, const double* rv3,
+                                 const double* rw3, const double* radius, const double* mass,
+                                 const uint8_t* fixed) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
   if (ninst < 1 || ninst > 65535) return fail(RP_ERR_INVALID, "ninst must lie in [1, 65535]");
   if (!counts || !particle) return fail(RP_ERR_INVALID, "counts and particle materials are required");
@@ -2337,8 +2339,7 @@ rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* coun
   e->inst_wall.assign((size_t)ninst, e->prm.wall);
   if (wall) e->inst_wall.assign(wall, wall + ninst);
   e->inst_count.assign(counts, counts + ninst);
-  return upload_impl(e, n, x3, v3, w3, nullptr, nullptr, nullptr, nullptr, nullptr, radius, mass, fixed,
-                     inst.data());
+  return upload_impl(e, n, x3, v3, w3, a3, tau3, rx3, rv3, rw3, radius, mass, fixed, inst.data());
 }
 
 rp_status rp_engine_instance_counts(rp_engine* e, size_t* counts) {

@@ -59,7 +59,8 @@ def lib() -> C.CDLL:
     L.rp_engine_set_jkr_solver.argtypes = [_eng, C.c_int32, C.POINTER(abi.RpJkrEnvelope), _fp, C.c_int32]
     L.rp_engine_jkr_stats.argtypes = [_eng, C.POINTER(abi.RpJkrStats)]
     L.rp_engine_upload_batch.argtypes = [_eng, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(abi.RpMaterial),
-                                         C.POINTER(abi.RpMaterial), _dp, _dp, _dp, _dp, _dp, _u8p]
+                                         C.POINTER(abi.RpMaterial), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                         _dp, _u8p]
     L.rp_engine_instance_counts.argtypes = [_eng, C.POINTER(C.c_size_t)]
     L.rp_engine_set_graph.argtypes = [_eng, C.c_int]
     L.rp_engine_autotune.argtypes = [_eng, C.c_uint32, _dp, _dp]
@@ -168,7 +169,7 @@ class Engine:
         self._ninst = 1
 
     def upload_batch(self, counts, materials, x, radius, mass, fixed=None, v=None, w=None,
-                     wall_materials=None) -> None:
+                     wall_materials=None, a=None, tau=None, rx=None, rv=None, rw=None) -> None:
         """Independent instances in one engine: counts[k] particles of
         instance k (arrays concatenated in instance order), each with its own
         particle material (abi.RpMaterial) and optionally wall material."""
@@ -179,12 +180,12 @@ class Engine:
         r = np.ascontiguousarray(np.asarray(radius, dtype=np.float64).reshape(n))
         m = np.ascontiguousarray(np.asarray(mass, dtype=np.float64).reshape(n))
         f = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, dtype=np.uint8).reshape(n))
-        vs, ws = _vec(v, n), _vec(w, n)
+        arrs = [_vec(q, n) for q in (v, w, a, tau, rx, rv, rw)]
         cnt = (C.c_size_t * k)(*counts)
         pm = (abi.RpMaterial * k)(*materials)
         wm = None if wall_materials is None else (abi.RpMaterial * k)(*wall_materials)
-        _check(self._L.rp_engine_upload_batch(self._h, k, cnt, pm, wm, _ptr(xs), _ptr(vs), _ptr(ws), _ptr(r),
-                                              _ptr(m), _ptr(f, C.c_uint8)))
+        _check(self._L.rp_engine_upload_batch(self._h, k, cnt, pm, wm, _ptr(xs), *[_ptr(q) for q in arrs],
+                                              _ptr(r), _ptr(m), _ptr(f, C.c_uint8)))
         self._ninst = k
 
     def instance_counts(self) -> np.ndarray:

@@ -63,6 +63,7 @@ class Simulation:
         self._meshes: list[tuple[np.ndarray, np.ndarray]] = []
         self._engine: native.Engine | None = None
         self._record = False
+        self._jkr = ("exact", None, False)  # solver, surrogate, check
 
     # ---- construction (src/lib.rs:57-80) ---------------------------------------
     @staticmethod
@@ -118,6 +119,11 @@ class Simulation:
             self._steps(steps)
 
     def _steps(self, n: int) -> None:
+        eng = self._prepare_engine()
+        eng.step(n)
+        self._host_valid = False
+
+    def _prepare_engine(self) -> native.Engine:
         eng = self._ensure_engine()
         if self._params_dirty:
             eng.set_params(self._params)
@@ -127,8 +133,7 @@ class Simulation:
             eng.upload(h["x"], h["radius"], h["mass"], h["fixed"], v=h["v"], w=h["w"], a=h["a"], tau=h["tau"],
                        rx=h["rx"], rv=h["rv"], rw=h["rw"])
             self._engine_valid = True
-        eng.step(n)
-        self._host_valid = False
+        return eng
 
     # ---- materials, models, boundaries (src/lib.rs:98-180) --------------------------
     def set_particle_material(self, youngs_modulus, poissons_ratio, density, friction_coefficient,
@@ -169,6 +174,34 @@ class Simulation:
         self._params.cell_size = float(cell_size)
         self._params.skin = float(skin)
         self._params_dirty = True
+
+    def set_jkr_solver(self, solver="exact", surrogate=None, check=False):
+        """Extension: 'exact' (the reference's Newton solve, default),
+        'shortcut' (the same results; contacts that cannot have a root skip
+        the iterations) or 'surrogate' (a trained surrogate.JkrSurrogate,
+        FP64-polished, exact fallback).  check=True also runs the exact solve
+        for every served contact and records the force errors (jkr_stats)."""
+        if solver not in abi.JKR_SOLVERS:
+            raise ValueError("Invalid JKR solver")
+        if solver == "surrogate" and surrogate is None:
+            raise ValueError("the surrogate solver needs a trained network")
+        self._jkr = (solver, surrogate, bool(check))
+        if self._engine is not None:
+            self._engine.set_jkr_solver(solver, surrogate, check=check)
+
+    def jkr_stats(self) -> dict:
+        """Extension: surrogate counters of the last step (served, fallback, errors)."""
+        if self._engine is None:
+            return {}
+        return self._engine.jkr_stats()
+
+    def autotune(self, steps=20):
+        """Extension: picks the fastest Verlet skin and cell size for the current
+        state (the state is unchanged; results never depend on them)."""
+        self._prepare_engine()
+        skin, cell = self._engine.autotune(steps)
+        self._params.skin, self._params.cell_size = skin, cell
+        return skin, cell
 
     def enable_gpu(self):
         """src/lib.rs:146-148: RuntimeError when no device can be used."""
@@ -274,6 +307,8 @@ class Simulation:
             for v9, n3 in self._meshes:
                 eng.add_mesh(v9, n3)
             eng.set_record(self._record)
+            if self._jkr[0] != "exact":
+                eng.set_jkr_solver(*self._jkr[:2], check=self._jkr[2])
         return self._engine
 
     def _sync_host(self) -> None:
@@ -282,3 +317,70 @@ class Simulation:
         st = self._engine.download()
         self._h = {k: st[k] for k in (*_FIELDS3, "radius", "mass", "fixed")}
         self._host_valid = True
+
+
+class Ensemble:
+    """Extension (no counterpart in the reference): steps several Simulation
+    objects together on one batched engine -- one set of kernel launches per
+    step over all of them.  The simulations must agree on everything but
+    their particles and materials (dt, bounds, periodic axes, contact models,
+    gravity, meshes, engine tuning); each evolves exactly as it would alone.
+    After run()/step() every member's getters return its new state."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        if not self.sims:
+            raise ValueError("an ensemble needs at least one simulation")
+        p0, m0 = self.sims[0]._params, self.sims[0]._meshes
+        for s in self.sims[1:]:
+            p = s._params
+            same = (p.dt == p0.dt and list(p.bounds_min) == list(p0.bounds_min)
+                    and list(p.bounds_max) == list(p0.bounds_max) and list(p.periodic) == list(p0.periodic)
+                    and p.normal_model == p0.normal_model and p.tangential_model == p0.tangential_model
+                    and list(p.gravity) == list(p0.gravity) and p.ref_cell_size == p0.ref_cell_size
+                    and p.cell_size == p0.cell_size and p.skin == p0.skin)
+            if not same:
+                raise ValueError("ensemble members must share dt, bounds, periodic axes, models, gravity and tuning")
+            if len(s._meshes) != len(m0) or any(not (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]))
+                                                for a, b in zip(s._meshes, m0)):
+                raise ValueError("ensemble members must share their meshes")
+        self._engine: native.Engine | None = None
+        self._dt32 = self.sims[0]._dt32
+
+    def step(self):
+        self._steps(1)
+
+    def run(self, duration):
+        steps = int(math.ceil(float(np.float32(duration) / self._dt32)))
+        if steps > 0:
+            self._steps(steps)
+
+    def _steps(self, n: int) -> None:
+        if self._engine is None:
+            try:
+                eng = native.Engine(abi.RpParams.from_buffer_copy(self.sims[0]._params))
+            except native.EngineError as e:
+                raise RuntimeError(str(e)) from e
+            for s in self.sims:
+                s._sync_host()
+            hs = [s._h for s in self.sims]
+            cat = lambda k: np.concatenate([h[k] for h in hs])
+            eng.upload_batch([h["x"].shape[0] for h in hs], [s._params.particle for s in self.sims], cat("x"),
+                             cat("radius"), cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"), a=cat("a"),
+                             tau=cat("tau"), rx=cat("rx"), rv=cat("rv"), rw=cat("rw"),
+                             wall_materials=[s._params.wall for s in self.sims])
+            for v9, n3 in self.sims[0]._meshes:
+                eng.add_mesh(v9, n3)
+            if self.sims[0]._jkr[0] == "shortcut":
+                eng.set_jkr_solver("shortcut")
+            self._engine = eng
+        self._engine.step(n)
+        # members see their new state through their own getters
+        counts = self._engine.instance_counts()
+        st = self._engine.download()
+        o = 0
+        for s, c in zip(self.sims, counts):
+            s._h = {k: np.array(st[k][o:o + c]) for k in (*_FIELDS3, "radius", "mass", "fixed")}
+            s._host_valid = True
+            s._engine_valid = False
+            o += int(c)
