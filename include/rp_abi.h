@@ -209,7 +209,8 @@ rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3,
  * be NULL (the engine's wall material for all).  Particles of different instances never
  * interact.  Each instance evolves exactly as a single engine given that
  * instance alone.  Downloads list the instances in order, each in API
- * order.  Not available with pair recording or the JKR surrogate. */
+ * order.  Not available with pair recording.  The JKR surrogate serves the
+ * instances whose E* and gamma lie in its envelope. */
 rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* counts,
                                  const rp_material* particle, const rp_material* wall,
                                  const double* x3, const double* v3, const double* w3,

@@ -752,6 +752,7 @@ struct Acc {
   // entries that have rows, the entries near the served set (next step's
   // collect visits only these), and counters
   int srow;
+  int sok;  // the particle's material pair lies in the surrogate's envelope
   unsigned long long rmask, nmask;
   unsigned int sserved, sfall;
   float emax, rmax;
@@ -778,11 +779,11 @@ __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rst
     return zero3();
   }
   if (JS >= 2) {
-    if (k < 64 && sd.pp_ok && rstar >= sd.r_lo && rstar <= sd.r_hi && s >= kSurrSLo * rstar &&
+    if (k < 64 && acc.sok && rstar >= sd.r_lo && rstar <= sd.r_hi && s >= kSurrSLo * rstar &&
         s <= kSurrSHi * rstar && ov >= sd.md_lo * t && ov <= sd.md_hi * t)
       acc.nmask |= 1ull << k;
     double d = 0.0;
-    if (surr_route(ov, rstar, s, t, sd, k, d)) {
+    if (surr_route(ov, rstar, s, t, sd, k, d, acc.sok)) {
       const unsigned long long bit = 1ull << k;  // k < 64 here
       const int row = (acc.rmask & bit) ? acc.srow + __popcll(acc.rmask & (bit - 1ull)) : -1;
       if (row >= 0 && row < sd.rows_cap) {
@@ -1026,6 +1027,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
   acc.npair = 0;
   acc.overlap = false;
   acc.srow = -1;
+  acc.sok = 0;
   acc.rmask = 0ull;
   acc.nmask = 0ull;
   acc.sserved = 0;
@@ -1058,6 +1060,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       if (JS >= 2) {
         acc.srow = sd.base[i];
         if (acc.srow >= 0) acc.rmask = sd.rowmask[i];
+        acc.sok = BATCH ? (int)sd.inst[T.inst[i]].x : sd.pp_ok;
       }
       const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
       const MaterialConsts& mc = BATCH ? ic->mc : P.mc;
@@ -1259,6 +1262,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
 // reference-grid visibility, overlap, R*), so both kernels agree on which
 // contacts consume a feature row.  Returns the row's scaled features.
 __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o, int j, int k, const SurrDev& sd,
+                                    const PairConsts& c, double far, float f_lne, float f_lng, int ok,
                                     float4* feat) {
   dv3 xj;
   double rj;
@@ -1275,19 +1279,18 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
   if (mult == 0) return false;
   const double overlap = rsum - sqrt(dsq);
   const double rstar = (o.r * rj) / (o.r + rj);
-  const PairConsts& c = P.mc.pp;
   if (!(c.gamma > 0.0)) return false;
-  if (overlap < P.jkr_far_pp - 1e-8 * fabs(overlap)) return false;
+  if (overlap < far - 1e-8 * fabs(overlap)) return false;
   const double s = cbrt(rstar * rstar * c.jc);
   const double t = s * s / rstar;
   if (overlap < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(overlap))) return false;
   double d = 0.0;
-  if (!surr_route(overlap, rstar, s, t, sd, k, d)) return false;
+  if (!surr_route(overlap, rstar, s, t, sd, k, d, ok)) return false;
   if (feat) {
     feat->x = scale_in((float)sqrt(d - kJkrDMin), sd.sc.in_lo[0], sd.sc.in_hi[0]);
     feat->y = scale_in((float)log(rstar), sd.sc.in_lo[1], sd.sc.in_hi[1]);
-    feat->z = sd.f_lne;
-    feat->w = sd.f_lng;
+    feat->z = f_lne;
+    feat->w = f_lng;
   }
   return true;
 }
@@ -1309,7 +1312,21 @@ __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long 
   const int lane = threadIdx.x & 63;
   unsigned long long mask = 0ull;
   Own o;
+  // the particle's material pair (per instance in a batched engine)
+  PairConsts pc = P.mc.pp;
+  double far = P.jkr_far_pp;
+  float f_lne = sd.f_lne, f_lng = sd.f_lng;
+  int ok = sd.pp_ok;
   if (i < n && !(T.flags[i] & FLAG_DEAD)) {
+    if (P.ninst > 1) {
+      const int q = T.inst[i];
+      pc = P.itab[q].mc.pp;
+      far = P.itab[q].jkr_far_pp;
+      const float4 f = sd.inst[q];
+      ok = (int)f.x;
+      f_lne = f.y;
+      f_lng = f.z;
+    }
     const int kmax = min(nbr_count[i], 64);
     unsigned long long visit = full ? ~0ull : sd.nmask[i];
     if (kmax < 64) visit &= (1ull << kmax) - 1ull;
@@ -1321,7 +1338,7 @@ __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long 
       while (visit) {
         const int k = __ffsll((long long)visit) - 1;
         visit &= visit - 1ull;
-        if (collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, nullptr)) mask |= 1ull << k;
+        if (collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, pc, far, f_lne, f_lng, ok, nullptr)) mask |= 1ull << k;
       }
     }
   }
@@ -1362,7 +1379,7 @@ __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long 
     const int k = __ffsll((long long)mask) - 1;
     mask &= mask - 1ull;
     float4 f;
-    (void)collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, &f);
+    (void)collect_pair(P, S, o, nbr[k * P.cap + i], k, sd, pc, far, f_lne, f_lng, ok, &f);
     rows[r++] = f;
   }
 }
@@ -1475,6 +1492,8 @@ struct rp_engine {
   float* d_surr_y = nullptr;   // [surr_rows_cap x 2]
   int surr_rows_cap = 0;
   SurrStats* d_sstats = nullptr;  // [cap / kForceBlock + 1]
+  float4* d_surr_inst = nullptr;  // [ninst] batched engines: per-instance envelope data
+  bool surr_inst_valid = false;
   bool sstats_valid = false;     // the last force launch wrote d_sstats
 };
 
@@ -1964,6 +1983,25 @@ SurrDev surr_dev(rp_engine* e) {
   sd.sc = make_scale(v);
   sd.f_lne = scale_in((float)std::log(pp.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
   sd.f_lng = scale_in((float)std::log(std::max(pp.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
+  if (e->ninst > 1) {
+    if (!e->surr_inst_valid) {
+      std::vector<float4> tab((size_t)e->ninst);
+      for (int q = 0; q < e->ninst; ++q) {
+        const PairConsts c = material_consts(e->inst_particle[q], e->inst_wall[q]).pp;
+        tab[q].x = inside(c.estar, v.estar_lo, v.estar_hi) && inside(c.gamma, v.gamma_lo, v.gamma_hi) ? 1.f : 0.f;
+        tab[q].y = scale_in((float)std::log(c.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
+        tab[q].z = scale_in((float)std::log(std::max(c.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
+        tab[q].w = 0.f;
+      }
+      free_ptr(e->d_surr_inst);
+      e->d_surr_inst = nullptr;
+      if (hipMalloc(&e->d_surr_inst, tab.size() * sizeof(float4)) == hipSuccess &&
+          hipMemcpy(e->d_surr_inst, tab.data(), tab.size() * sizeof(float4), hipMemcpyHostToDevice) == hipSuccess)
+        e->surr_inst_valid = true;
+    }
+    sd.inst = e->d_surr_inst;
+    if (!e->surr_inst_valid) sd.pp_ok = 0;
+  }
   sd.base = e->d_surr_base;
   sd.rowmask = e->d_surr_rowmask;
   sd.nmask = e->d_surr_nmask;
@@ -2017,13 +2055,18 @@ void launch_force_t(rp_engine* e, int flag_slot) {
 // JKR solver variants are compiled for the JKR model only; a recording
 // launch uses the exact solve with the shortcut (no surrogate rows).
 // Batched engines (several instances) use the per-instance material table;
-// they have no pair recording and no surrogate.
+// they have no pair recording.
 template <int NORMAL, int TANG, bool ADV, bool REC>
 void launch_force_j(rp_engine* e, int flag_slot) {
   if constexpr (!REC) {
     if (e->ninst > 1) {
       if constexpr (NORMAL == RP_NORMAL_JKR) {
-        if (e->jkr_solver != RP_JKR_EXACT) {
+        if (e->jkr_solver == RP_JKR_SURROGATE) {
+          if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, false, 3, true>(e, flag_slot);
+          else launch_force_t<NORMAL, TANG, ADV, false, 2, true>(e, flag_slot);
+          return;
+        }
+        if (e->jkr_solver == RP_JKR_SHORTCUT) {
           launch_force_t<NORMAL, TANG, ADV, false, 1, true>(e, flag_slot);
           return;
         }
@@ -2326,8 +2369,6 @@ rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* coun
   if (ninst < 1 || ninst > 65535) return fail(RP_ERR_INVALID, "ninst must lie in [1, 65535]");
   if (!counts || !particle) return fail(RP_ERR_INVALID, "counts and particle materials are required");
   if (e->record && ninst > 1) return fail(RP_ERR_STATE, "pair recording is not available for batched engines");
-  if (e->jkr_solver == RP_JKR_SURROGATE && ninst > 1)
-    return fail(RP_ERR_STATE, "the JKR surrogate is not available for batched engines");
   size_t n = 0;
   for (int q = 0; q < ninst; ++q) n += counts[q];
   std::vector<uint16_t> inst(n);
@@ -2339,6 +2380,7 @@ rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* coun
   e->inst_wall.assign((size_t)ninst, e->prm.wall);
   if (wall) e->inst_wall.assign(wall, wall + ninst);
   e->inst_count.assign(counts, counts + ninst);
+  e->surr_inst_valid = false;
   return upload_impl(e, n, x3, v3, w3, a3, tau3, rx3, rv3, rw3, radius, mass, fixed, inst.data());
 }
 
@@ -2654,7 +2696,6 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
   if (solver < RP_JKR_EXACT || solver > RP_JKR_SURROGATE) return fail(RP_ERR_INVALID, "unknown JKR solver");
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (solver == RP_JKR_SURROGATE) {
-    if (e->ninst > 1) return fail(RP_ERR_STATE, "the JKR surrogate is not available for batched engines");
     if (!env || !params) return fail(RP_ERR_INVALID, "the surrogate needs an envelope and parameters");
     const char* why = "";
     if (envelope_check(*env, &why) != RP_OK) return fail(RP_ERR_INVALID, why);
@@ -2669,6 +2710,7 @@ rp_status rp_engine_set_jkr_solver(rp_engine* e, int32_t solver, const rp_jkr_en
   }
   e->jkr_solver = solver;
   e->surr_check = check ? 1 : 0;
+  e->surr_inst_valid = false;
   e->gen++;
   if (e->d_ctl) {  // near masks of an earlier surrogate period are stale: the next collect scans everything
     const unsigned long long zero = 0ull;
@@ -2874,6 +2916,7 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_tri_idx);
   free_ptr(e->d_ctl);
   free_ptr(e->d_itab);
+  free_ptr(e->d_surr_inst);
   free_ptr(e->d_surr_x);
   free_ptr(e->d_surr_y);
   mlp_free(e->surr_net);

@@ -82,6 +82,7 @@ struct SurrDev {
   int check;          // also run the exact solve and record the force errors
   MlpScale sc;
   float f_lne, f_lng; // scaled ln E*, ln gamma of particle-particle pairs
+  const float4* inst; // batched engines: per instance (served 1/0, scaled ln E*, scaled ln gamma, -)
   int* base;          // [slots] first feature row of each particle, -1 = none
   unsigned long long* rowmask;  // [slots] neighbour-list entries that have a row (in row order)
   unsigned long long* nmask;    // [slots] entries near the served set: the next collect visits only these
@@ -103,10 +104,11 @@ struct SurrStats {
 
 // True when the network serves a contact: d = delta / t in the served range
 // and s / R* inside [kSurrSLo, kSurrSHi]; `k` is the contact's position in
-// the particle's neighbour list (a 64-bit mask records the served ones).
+// the particle's neighbour list (a 64-bit mask records the served ones), `ok`
+// says whether the pair's E* and gamma lie in the envelope.
 __host__ __device__ inline bool surr_route(double ov, double rstar, double s, double t, const SurrDev& sd, int k,
-                                           double& d) {
-  if (k >= 64 || !sd.pp_ok || !(rstar >= sd.r_lo && rstar <= sd.r_hi)) return false;
+                                           double& d, int ok) {
+  if (k >= 64 || !ok || !(rstar >= sd.r_lo && rstar <= sd.r_hi)) return false;
   if (!(s >= kSurrSLo * rstar && s <= kSurrSHi * rstar)) return false;
   d = ov / t;
   return (d >= sd.d_lo && d <= 0.0) || (d >= kSurrGapHi && d <= sd.d_hi);
