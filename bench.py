@@ -29,6 +29,11 @@ jittered lattices in a 0.2 m box STL column) stepped by one batched engine
 value = total particle-steps/s.  The same instances stepped one engine at a
 time (each with its own launches) are timed beside it.
 
+--workload jkr-batch: 64 periodic JKR packings (N_k ~ U{5000..40000}, 1.5M
+spheres, r lognormal median 50 um, surface energy ~U[0.02, 0.08] J/m^2) in
+one batched engine, once per JKR solver (exact, shortcut, surrogate); value
+= the surrogate+batched form.
+
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
 HIP events on the engine's stream), and a CPU baseline: the oracle (a C++
@@ -88,65 +93,119 @@ def traffic_of(workload: str):
 
 
 def run_batch(args) -> int:
-    """Batched ensemble against one engine per instance."""
+    """Batched ensembles: --workload batch (sJKR material sweep, plain form)
+    and --workload jkr-batch (JKR surface-energy sweep, one line per JKR
+    solver, the surrogate last); each against one engine per instance."""
     import numpy as np
 
     from rusty_particles_amd import abi, native, scenes
     t0 = time.perf_counter()
-    p, counts, mats, x, r, m, meshes = scenes.batch_ensemble_workload(args.instances)
+    jkr = args.workload == "jkr-batch"
+    if jkr:
+        p, counts, mats, x, r, m, v = scenes.jkr_ensemble_workload(args.instances or 64)
+        meshes = []
+        forms = args.jkr_forms.split(",")
+        desc = "JKR+Coulomb periodic packings, surface-energy sweep"
+    else:
+        p, counts, mats, x, r, m, meshes = scenes.batch_ensemble_workload(args.instances or 256)
+        v = None
+        forms = ["plain"]
+        desc = "sJKR+Coulomb, material sweep"
     n = int(sum(counts))
     offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-    log(f"workload batch: {len(counts)} instances, {n} particles, built in {time.perf_counter() - t0:.1f}s")
+    log(f"workload {args.workload}: {len(counts)} instances, {n} particles, built in {time.perf_counter() - t0:.1f}s")
 
-    def engine(lo, hi, batch=True):
+    net = None
+    surr_info = {}
+    if "surrogate" in forms:
+        from rusty_particles_amd import surrogate
+        rs = r[:, None] * r[None, :200] / (r[:, None] + r[None, :200])
+        gam = [q.surface_energy for q in mats]
+        e_star = 1.0 / (2 * (1 - p.particle.poissons_ratio ** 2) / p.particle.youngs_modulus)
+        env = surrogate.envelope(-0.40, 30.0, (0.9 * rs.min(), 1.1 * rs.max()), e_star, (min(gam), max(gam)))
+        t0 = time.perf_counter()
+        net = surrogate.train(env, nsamples=args.train_samples, epochs=args.train_epochs, batch=4096, lr=3e-3)
+        surr_info["train_s"] = time.perf_counter() - t0
+        log(f"surrogate trained in {surr_info['train_s']:.1f}s")
+
+    def engine(lo, hi, form, batch=True):
         q = abi.RpParams.from_buffer_copy(p)
         sl = slice(int(offs[lo]), int(offs[hi]))
+        vv = None if v is None else v[sl]
         if batch:
             e = native.Engine(q)
-            e.upload_batch(counts[lo:hi], mats[lo:hi], x[sl], r[sl], m[sl])
+            e.upload_batch(counts[lo:hi], mats[lo:hi], x[sl], r[sl], m[sl], v=vv)
         else:
             q.particle = mats[lo]
             e = native.Engine(q)
-            e.upload(x[sl], r[sl], m[sl])
+            e.upload(x[sl], r[sl], m[sl], v=vv)
         for v9, n3 in meshes:
             e.add_mesh(v9, n3)
+        if form != "plain":
+            e.set_jkr_solver(form, net)
         return e
 
-    res = {}
-    for graph in (True, False):
-        e = engine(0, len(counts))
-        e.set_graph(graph)
+    def timed(e, steps):
         e.step(args.warmup)
         e.synchronize()
         if args.autotune:
             sk, ce = e.autotune(args.autotune)
-            log(f"batched autotune: skin {sk:.3g} m, cell {ce:.3g} m")
+            log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m")
         t0 = time.perf_counter()
-        e.step(args.steps)
+        e.step(steps)
         e.synchronize()
-        wall = time.perf_counter() - t0
-        res[graph] = (n * args.steps / wall, wall, e)
-        log(f"batched, graph {'on' if graph else 'off'}: {args.steps} steps in {wall:.3f}s, "
-            f"{n * args.steps / wall:.4g} particle-steps/s")
-        if not graph:
-            e.close()
-    value, wall, eng = res[True]
-    st = eng.stats()
-    eng.set_timing(True)
-    eng.step(args.timing_steps)
-    tm = eng.timing()
-    eng.set_timing(False)
-    f_ms, f_cnt = tm["force"]
-    avg_s = f_ms / max(f_cnt, 1) * 1e-3
-    achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
-    log(f"force kernel avg {avg_s * 1e6:.1f} us ({achieved:.0f} GB/s algorithmic), contacts/step {st['contacts']}")
-    eng.close()
+        return time.perf_counter() - t0
+
+    results = {}
+    for form in forms:
+        e = engine(0, len(counts), form)
+        wall = timed(e, args.steps)
+        value = n * args.steps / wall
+        st = e.stats()
+        e.set_timing(True)
+        e.step(args.timing_steps)
+        tm = e.timing()
+        e.set_timing(False)
+        avg_s = tm["force"][0] / max(tm["force"][1], 1) * 1e-3
+        res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
+               "achieved_gbs": FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9, "contacts_per_step": st["contacts"],
+               "wall": wall, "stats": st}
+        if tm["surrogate"][1]:
+            res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
+        log(f"[{form}] batched: {args.steps} steps in {wall:.3f}s, {value:.4g} particle-steps/s, force kernel "
+            f"{avg_s * 1e6:.1f} us")
+        if form == "surrogate":
+            e.set_jkr_solver("surrogate", net, check=True)
+            errs = []
+            for _ in range(5):
+                e.step(1)
+                errs.append(e.jkr_stats())
+            surr_info.update({
+                "served_per_step": float(np.mean([q["served"] for q in errs])),
+                "fallback_per_step": float(np.mean([q["fallback"] for q in errs])),
+                "force_rel_err_max": max(q["err_max"] for q in errs),
+                "force_rel_err_mean": float(np.mean([q["err_mean"] for q in errs])),
+            })
+            log(f"[surrogate] check: {surr_info}")
+        e.close()
+        results[form] = res
+    main = forms[-1]
+    R = results[main]
+
+    graph_off = None
+    if not jkr:
+        e = engine(0, len(counts), main)
+        e.set_graph(False)
+        wall_off = timed(e, args.steps)
+        graph_off = n * args.steps / wall_off
+        log(f"batched, graph off: {graph_off:.4g} particle-steps/s")
+        e.close()
 
     # the same instances, one engine each, run back to back (graph on)
     sep_steps = max(args.steps // 4, 20)
     total = 0.0
     for k in range(len(counts)):
-        e = engine(k, k + 1, batch=False)
+        e = engine(k, k + 1, main, batch=False)
         e.step(args.warmup)
         e.synchronize()
         t0 = time.perf_counter()
@@ -167,28 +226,34 @@ def run_batch(args) -> int:
             q = abi.RpParams.from_buffer_copy(p)
             q.particle = mats[k]
             cell = 2.0 * float(r.max()) * 1.5
-            val, sec = cpu_baseline(q, x[sl], r[sl], m[sl], None, meshes, args.cpu_steps, threads, cell)
+            val, sec = cpu_baseline(q, x[sl], r[sl], m[sl], None if v is None else v[sl], meshes, args.cpu_steps,
+                                    threads, cell)
             csec += sec
             cwork += counts[k] * args.cpu_steps
         cpu = {"value": cwork / csec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"instances {sample} ({sum(counts[k] for k in sample)} particles), {args.cpu_steps} "
-                         f"step(s) each after 1 warm-up step; {csec:.1f}s"}
+                         f"step(s) each after 1 warm-up step, reference's JKR solve; {csec:.1f}s"}
         log(f"cpu baseline {cwork / csec:.4g} particle-steps/s")
     out = {
         "metric": "particle-steps/s on 1 GPU, plain vs surrogate+batched at 1e-3 force err; HBM GB/s",
-        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": R["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batch: {len(counts)} instances, {n} spheres in all, sJKR+Coulomb, material sweep, "
-                               f"dt={p.dt:g}, FP64", "particles": n, "instances": len(counts), "form": "batched"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of("batch")},
+        "config": {"workload": f"{args.workload}: {len(counts)} instances, {n} spheres in all, {desc}, dt={p.dt:g}, "
+                               f"FP64", "particles": n, "instances": len(counts),
+                   "form": "batched" if main == "plain" else f"{main}+batched"},
+        "roofline": {"bound": "hbm", "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)},
         "cpu_baseline": cpu,
-        "contacts_per_step": st["contacts"], "wall_contacts_per_step": st["wall_contacts"],
-        "force_kernel_avg_us": avg_s * 1e6,
-        "graph_off_value": res[False][0],
+        "contacts_per_step": R["contacts_per_step"], "force_kernel_avg_us": R["force_kernel_avg_us"],
         "one_engine_per_instance_value": sep_value,
     }
+    if graph_off is not None:
+        out["graph_off_value"] = graph_off
+    if len(results) > 1:
+        out["forms"] = {f: {k: q for k, q in res.items() if k not in ("stats", "wall")} for f, res in results.items()}
+    if surr_info:
+        out["surrogate"] = surr_info
     print(json.dumps(out), flush=True)
     return 0
 
@@ -230,7 +295,7 @@ def main() -> int:
                     help="JKR solvers to time on the jkr workload (the last one is the reported value)")
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
-    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
     ap.add_argument("--autotune", type=int, default=None,
                     help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
                          "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
@@ -251,7 +316,7 @@ def main() -> int:
     from rusty_particles_amd import abi, native
     if native.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
-    if args.workload == "batch":
+    if args.workload in ("batch", "jkr-batch"):
         return run_batch(args)
 
     t0 = time.perf_counter()

@@ -281,3 +281,38 @@ def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int 
     p.particle = abi.RpMaterial(1e7, 0.3, 2500.0, 0.5, 0.5, 0.0)
     v9, n3 = box_mesh((0.0, 0.0, 0.0), (side, top, side))
     return p, [int(c) for c in counts], mats, x, r, m, [(v9, n3)]
+
+
+def jkr_ensemble_workload(ninst: int = 64, seed: int = 0xD3E50006, nmin: int = 5000, nmax: int = 40000,
+                          phi_max: float = 0.35):
+    """Ensemble of `ninst` periodic JKR packings for a surface-energy sweep:
+    N_k ~ U{nmin..nmax} spheres (r lognormal, median 50 um, sigma 0.3,
+    rho 1500), gamma_k ~ U[0.02, 0.08] J/m^2 (E = 1e8, nu = 0.3, mu = 0.3),
+    JKR + Coulomb, dt = 1e-8 s, velocities ~N(0, 0.05) m/s.  All instances
+    share one periodic cube, sized so that the largest N_k fills it to
+    `phi_max` (smaller instances are looser); each is placed by random
+    sequential addition, largest spheres first.  Returns (params, counts,
+    materials, x, r, m, v)."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(nmin, nmax + 1, ninst)
+    radii = [50e-6 * np.exp(0.3 * rng.standard_normal(int(c))) for c in counts]
+    vmax = max(4.0 / 3.0 * np.pi * float(np.sum(r ** 3)) for r in radii)
+    L = (vmax / phi_max) ** (1.0 / 3.0)
+    xs, rs, vs, mats = [], [], [], []
+    for k, rad in enumerate(radii):
+        order = np.argsort(-rad, kind="stable")
+        x, r = rsa_radii(rad[order], (0.0, 0.0, 0.0), (L, L, L), (1, 1, 1), seed + 31 * k)
+        perm = np.random.default_rng(seed + 31 * k + 1).permutation(r.size)
+        xs.append(x[perm])
+        rs.append(r[perm])
+        vs.append(np.random.default_rng(seed + 31 * k + 2).normal(0.0, 0.05, (r.size, 3)))
+        mats.append(abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, float(rng.uniform(0.02, 0.08))))
+    x, r, v = np.concatenate(xs), np.concatenate(rs), np.concatenate(vs)
+    m = sphere_mass(r, 1500.0)
+    p = abi.default_params(1e-8, (0.0, 0.0, 0.0), (L, L, L))
+    for k in range(3):
+        p.periodic[k] = 1
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, 0.05)
+    return p, [int(c) for c in counts], mats, x, r, m, v
