@@ -215,6 +215,31 @@ __device__ inline void stxr(double* xr, int i, dv3 x, double r) {
   q.w = r;
   reinterpret_cast<double4*>(xr)[i] = q;
 }
+#ifndef RP_STREAM_HINTS
+#define RP_STREAM_HINTS 2
+#endif
+// Streaming hints for the force kernel's own-particle state: it is read once
+// and written once per step, so non-temporal accesses keep the caches for
+// the neighbours' position records, which many threads gather.
+// 0 = none, 1 = non-temporal stores, 2 = also non-temporal loads of the planes.
+__device__ inline dv3 ld3s(const double* p, int i, int cap) {
+  if (RP_STREAM_HINTS >= 2)
+    return mk(__builtin_nontemporal_load(p + i), __builtin_nontemporal_load(p + i + cap),
+              __builtin_nontemporal_load(p + i + 2 * cap));
+  return mk(p[i], p[i + cap], p[i + 2 * cap]);
+}
+__device__ inline void st3s(double* p, int i, int cap, dv3 v) {
+  if (RP_STREAM_HINTS >= 1) {
+    __builtin_nontemporal_store(v.x, p + i);
+    __builtin_nontemporal_store(v.y, p + i + cap);
+    __builtin_nontemporal_store(v.z, p + i + 2 * cap);
+  } else {
+    p[i] = v.x;
+    p[i + cap] = v.y;
+    p[i + 2 * cap] = v.z;
+  }
+}
+
 __device__ inline void st3(double* p, int i, int cap, dv3 v) {
   p[i] = v.x;
   p[i + cap] = v.y;
@@ -1041,8 +1066,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
     const uint8_t flags = T.flags[i];
     Own o;
     ldxr(S.xr, i, o.x, o.r);
-    o.v = ld3(S.v, i, cap);
-    o.w = ld3(S.w, i, cap);
+    o.v = ld3s(S.v, i, cap);
+    o.w = ld3s(S.w, i, cap);
     o.m = T.m[i];
     const int cnt = nbr_count[i];
     if (flags & FLAG_DEAD) {
@@ -1163,7 +1188,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       // apply (simulation.rs:533-536) and second half kick (:543-561)
       const dv3 a = mk(P.g[0] + acc.f.x / o.m, P.g[1] + acc.f.y / o.m, P.g[2] + acc.f.z / o.m);
       const dv3 tau = mk(0.0 + acc.t.x, 0.0 + acc.t.y, 0.0 + acc.t.z);
-      dv3 rx = ld3(S.rx, i, cap), rv = ld3(S.rv, i, cap), rw = ld3(S.rw, i, cap);
+      dv3 rx = ld3s(S.rx, i, cap), rv = ld3s(S.rv, i, cap), rw = ld3s(S.rw, i, cap);
       dv3 v = o.v, w = o.w;
       const dv3 alpha = angular_acc(tau, o.r, o.m);
       if (!(flags & FLAG_FIXED)) {
@@ -1190,11 +1215,11 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       }
       if (alive) {
         stxr(D.xr, i, x, o.r);
-        st3(D.v, i, cap, v);
-        st3(D.w, i, cap, w);
-        st3(D.rx, i, cap, rx);
-        st3(D.rv, i, cap, rv);
-        st3(D.rw, i, cap, rw);
+        st3s(D.v, i, cap, v);
+        st3s(D.w, i, cap, w);
+        st3s(D.rx, i, cap, rx);
+        st3s(D.rv, i, cap, rv);
+        st3s(D.rw, i, cap, rw);
       }
     }
   }
