@@ -275,7 +275,7 @@ rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count);
 
 /* Diagnostics, no counterpart in the reference: copies `bytes` bytes
  * between two fresh device buffers with a plain streaming kernel (16-byte
- * accesses) and returns the fastest of 5 launches.  A counter profile of the same process uses it to scale
+ * non-temporal accesses) and returns the fastest of 5 launches.  A counter profile of the same process uses it to scale
  * FETCH_SIZE / WRITE_SIZE by a kernel of known traffic. */
 rp_status rp_probe_copy(size_t bytes, double* ms);
 
