@@ -142,7 +142,6 @@ struct Ctl {
   // rebuilds + 1 when the force kernel last wrote the surrogate's near masks
   // (a rebuild re-orders the lists, so the next collect scans every entry)
   unsigned long long surr_mask_epoch;
-  unsigned surr_rows[kSurrRanges];  // JKR surrogate feature rows claimed by the current step, per range
   unsigned surr_rows_max;  // most rows any range was asked for in one step (sizes the row buffers)
 };
 
@@ -1026,8 +1025,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
   if (JS >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
     unsigned mx = 0u;
     for (int r = 0; r < kSurrRanges; ++r) {
-      mx = max(mx, sd.counter[r]);
-      sd.counter[r] = 0u;
+      mx = max(mx, sd.counter[r * kSurrCounterStride]);
+      sd.counter[r * kSurrCounterStride] = 0u;
     }
     ctl->surr_rows_max = max(ctl->surr_rows_max, mx);
   }
@@ -1388,7 +1387,7 @@ __global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long 
       tot += t;
     }
     const int range = blockIdx.x % kSurrRanges;
-    bbase = tot ? (unsigned)range * (unsigned)(sd.rows_cap / kSurrRanges) + atomicAdd(&sd.counter[range], tot) : 0u;
+    bbase = tot ? (unsigned)range * (unsigned)(sd.rows_cap / kSurrRanges) + atomicAdd(&sd.counter[range * kSurrCounterStride], tot) : 0u;
   }
   __syncthreads();
   if (i >= n) return;
@@ -1520,6 +1519,7 @@ struct rp_engine {
   int surr_rows_cap = 0;
   SurrStats* d_sstats = nullptr;  // [cap / kForceBlock + 1]
   float4* d_surr_inst = nullptr;  // [ninst] batched engines: per-instance envelope data
+  unsigned* d_surr_counters = nullptr;  // [kSurrRanges * kSurrCounterStride] row claims
   bool surr_inst_valid = false;
   bool sstats_valid = false;     // the last force launch wrote d_sstats
 };
@@ -2036,7 +2036,7 @@ SurrDev surr_dev(rp_engine* e) {
   sd.md_hi = v.d_hi + kSurrMaskMargin;
   sd.x = e->d_surr_x;
   sd.y = e->d_surr_y;
-  sd.counter = e->d_ctl->surr_rows;
+  sd.counter = e->d_surr_counters;
   sd.rows_cap = e->surr_rows_cap;
   return sd;
 }
@@ -2294,6 +2294,9 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMalloc(&e->d_surr_base, (size_t)cap * sizeof(int)));
+  if (!e->d_surr_counters)
+    HIP_TRY(hipMalloc(&e->d_surr_counters, (size_t)kSurrRanges * kSurrCounterStride * sizeof(unsigned)));
+  HIP_TRY(hipMemset(e->d_surr_counters, 0, (size_t)kSurrRanges * kSurrCounterStride * sizeof(unsigned)));
   HIP_TRY(hipMalloc(&e->d_surr_rowmask, (size_t)cap * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&e->d_surr_nmask, (size_t)cap * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&e->d_sstats, ((size_t)cap / kForceBlock + 1) * sizeof(SurrStats)));
@@ -2944,6 +2947,7 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_ctl);
   free_ptr(e->d_itab);
   free_ptr(e->d_surr_inst);
+  free_ptr(e->d_surr_counters);
   free_ptr(e->d_surr_x);
   free_ptr(e->d_surr_y);
   mlp_free(e->surr_net);

@@ -180,7 +180,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __re
   mlp_tile(net, x, y, rows, blockIdx.x * 4 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
 }
 
-// Row counts read on the device: range r of [0, cap) holds counter[r] rows.
+// Row counts read on the device: range r of [0, cap) holds
+// counter[r * kSurrCounterStride] rows.
 // A fixed grid walks 64-row groups (4 tiles, one per wave); a group lies in
 // one range (ranges are multiples of 64 rows), so the skip test is the same
 // for every wave of a workgroup and all meet the barriers.
@@ -191,7 +192,7 @@ __global__ void __launch_bounds__(256) k_mlp_counted(MlpDev net, const float* __
   for (int g = blockIdx.x; g * 64 < cap; g += gridDim.x) {
     const int r = (g * 64) / range_rows;
     const int first = r * range_rows;
-    const int rows = (int)min(counter[r], (unsigned)range_rows);
+    const int rows = (int)min(counter[r * kSurrCounterStride], (unsigned)range_rows);
     if (g * 64 - first >= rows) continue;
     mlp_tile(net, x + (size_t)first * kMlpIn, y + (size_t)first * kMlpOut, rows, (g * 64 - first) / 16 + (threadIdx.x >> 6),
              hs[threadIdx.x >> 6]);

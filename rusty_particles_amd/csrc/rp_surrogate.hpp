@@ -52,6 +52,9 @@ constexpr double kSurrSLo = 1e-3, kSurrSHi = 0.9;  // s / R* range served
 // (workgroup b from range b % kSurrRanges), so the claiming atomics spread
 // over as many addresses.
 constexpr int kSurrRanges = 16;
+// The range counters sit 256 bytes apart (different cache lines and memory
+// channels), so claims on different ranges do not contend.
+constexpr int kSurrCounterStride = 64;
 
 // Scaling of inputs and outputs (the training envelope).
 struct MlpScale {
@@ -89,7 +92,7 @@ struct SurrDev {
   double md_lo, md_hi;  // d range of nmask (the served range widened by a margin)
   float* x;           // [rows_cap x 4] scaled features
   float* y;           // [rows_cap x 2] scaled predictions
-  unsigned* counter;  // [kSurrRanges] rows claimed in this step, per range
+  unsigned* counter;  // rows claimed in this step: range r at counter[r * kSurrCounterStride]
   int rows_cap;       // rows in all; range r holds rows [r R, r R + R), R = rows_cap / kSurrRanges
 };
 
@@ -150,7 +153,8 @@ void mlp_free(MlpHost& m);
 void mlp_infer(const MlpDev& net, const float* x, float* y, int rows, hipStream_t stream);
 
 // The same on the claimed rows of each of the kSurrRanges ranges of
-// [0, cap): counter[r] rows from r cap / kSurrRanges (counts read on the device).
+// [0, cap): counter[r * kSurrCounterStride] rows from r cap / kSurrRanges
+// (counts read on the device).
 void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsigned* counter, int cap,
                        hipStream_t stream);
 
