@@ -220,19 +220,20 @@ def run_batch(args) -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         sample = list(range(0, len(counts), max(len(counts) // 4, 1)))[:4]
-        csec, cwork = 0.0, 0
+        csec, cwork, csteps = 0.0, 0, []
         for k in sample:
             sl = slice(int(offs[k]), int(offs[k + 1]))
             q = abi.RpParams.from_buffer_copy(p)
             q.particle = mats[k]
             cell = 2.0 * float(r.max()) * 1.5
-            val, sec = cpu_baseline(q, x[sl], r[sl], m[sl], None if v is None else v[sl], meshes, args.cpu_steps,
-                                    threads, cell)
+            val, sec, ns = cpu_baseline(q, x[sl], r[sl], m[sl], None if v is None else v[sl], meshes,
+                                        args.cpu_steps, threads, cell, args.cpu_seconds / len(sample))
             csec += sec
-            cwork += counts[k] * args.cpu_steps
+            cwork += counts[k] * ns
+            csteps.append(ns)
         cpu = {"value": cwork / csec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"instances {sample} ({sum(counts[k] for k in sample)} particles), {args.cpu_steps} "
-                         f"step(s) each after 1 warm-up step, reference's JKR solve; {csec:.1f}s"}
+               "sample": f"instances {sample} ({sum(counts[k] for k in sample)} particles), {csteps} "
+                         f"steps after 1 warm-up step each, reference's JKR solve; {csec:.1f}s"}
         log(f"cpu baseline {cwork / csec:.4g} particle-steps/s")
     out = {
         "metric": "particle-steps/s on 1 GPU, plain vs surrogate+batched at 1e-3 force err; HBM GB/s",
@@ -258,9 +259,12 @@ def run_batch(args) -> int:
     return 0
 
 
-def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float):
+def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
-    largest contact cutoff (the cell-size heuristic of simulation.rs:64)."""
+    largest contact cutoff (the cell-size heuristic of simulation.rs:64).
+    Times at least `steps` steps, more when the warm-up step says that
+    `seconds` of CPU time hold more (at most 10,000).  Returns
+    (particle-steps/s, seconds, steps timed)."""
     import numpy as np
 
     import oracle
@@ -273,11 +277,15 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float):
         o.add_mesh_arrays(v9, n3)
     o.set_cell_size(cell)
     o.set_record(False)
+    t0 = time.perf_counter()
     o.step(1)  # warm-up (allocations, first touch)
+    warm = time.perf_counter() - t0
+    if seconds > 0.0:
+        steps = max(steps, min(10_000, int(seconds / max(warm, 1e-6))))
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return x.shape[0] * steps / dt, dt
+    return x.shape[0] * steps / dt, dt, steps
 
 
 def main() -> int:
@@ -287,7 +295,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="hopper")
     ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="least CPU-baseline steps")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU-baseline sample length: as many steps as fit, going by the warm-up step")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
@@ -423,10 +433,10 @@ def main() -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cell = 2.0 * float(r.max()) * (1.5 if p.normal_model in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 1.0)
-        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_steps} step(s) of the same {n}-particle workload")
-        cval, csec = cpu_baseline(p, x, r, m, v, meshes, args.cpu_steps, threads, cell)
+        log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:g}s of the same {n}-particle workload")
+        cval, csec, csteps = cpu_baseline(p, x, r, m, v, meshes, args.cpu_steps, threads, cell, args.cpu_seconds)
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} step(s) of the full {n}-particle workload after 1 warm-up step, "
+               "sample": f"{csteps} step(s) of the full {n}-particle workload after 1 warm-up step, "
                          f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff), brute-force "
                          f"{ntri}-triangle wall loop as in simulation.rs:465-523; {csec:.1f}s"}
         log(f"cpu baseline {cval:.4g} particle-steps/s ({csec:.1f}s)")

@@ -580,6 +580,23 @@ static void normal_law(int model, double overlap, const double* n, const double*
   if (conv) *conv = info.converged;
 }
 
+// The reference's JKR solve (physics.rs:132-219) in FP64 on n contacts:
+// contact radius, iterations of its Newton loop (physics.rs:171-189; the
+// iteration that stops on a small step counts) and whether the residual check
+// passed.  E is the effective modulus E*.
+extern "C" void rpo_jkr_solve(size_t n, const double* overlap, const double* E, const double* rstar,
+                              const double* gamma, double* a_out, int* iterations, int* converged) {
+  const rpo::V3<double> nn = rpo::v3<double>(0.0, 0.0, 1.0);
+  for (size_t k = 0; k < n; ++k) {
+    rpo::JkrInfo info{0, 1};
+    double a = 0.0;
+    (void)rpo::jkr_force<double>(overlap[k], nn, nn, E[k], rstar[k], gamma[k], &info, &a);
+    a_out[k] = a;
+    iterations[k] = info.iterations;
+    converged[k] = info.converged;
+  }
+}
+
 extern "C" void rpo_normal_law(int bits, int model, double overlap, const double* n, const double* vrel, double E,
                     double rstar, double mstar, double rest, double gamma, double* out, double* a_out,
                     int* converged) {

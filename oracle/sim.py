@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
     L.rpo_max_threads.restype = C.c_int
     L.rpo_normal_law.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                  C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
+    L.rpo_jkr_solve.argtypes = [C.c_size_t, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.rpo_coulomb.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double, _dp]
     L.rpo_lsc.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
     L.rpo_triangle.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp]
@@ -286,6 +287,20 @@ def normal_law(bits, model, overlap, n, vrel, estar, rstar, mstar=1.0, rest=0.5,
     L.rpo_normal_law(bits, model, overlap, _p(nn), _p(vv), estar, rstar, mstar, rest, gamma, _p(out), C.byref(a),
                      C.byref(conv))
     return out, a.value, conv.value
+
+
+def jkr_solve(overlap, estar, rstar, gamma):
+    """The reference's JKR Newton solve in FP64 on arrays of contacts:
+    (contact radius, Newton iterations, converged)."""
+    L = lib()
+    arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), np.shape(overlap)), dtype=np.float64)
+            for v in (overlap, estar, rstar, gamma)]
+    n = arrs[0].size
+    a = np.zeros(n)
+    its = np.zeros(n, np.int32)
+    conv = np.zeros(n, np.int32)
+    L.rpo_jkr_solve(n, *[_p(v) for v in arrs], _p(a), _p(its, C.c_int), _p(conv, C.c_int))
+    return a, its, conv.astype(bool)
 
 
 def coulomb_law(bits, fn, vrel, n, mu):

@@ -327,6 +327,14 @@ __device__ inline int xcd_block(int b, int nb) {
   return x * q + min(x, r) + k;
 }
 
+// Orders the LDS accesses of one wavefront's lanes around this point (each
+// wave works in its own LDS slice, so no workgroup barrier is needed).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline unsigned int wave_sum_u32(unsigned int v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
@@ -1061,14 +1069,131 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
   acc.esum = 0.0;
   acc.rsum = 0.0;
   bool far = false;
+  uint8_t flags = 0;
+  Own o{};
+  int cnt = 0;
   if (i < n) {
-    const uint8_t flags = T.flags[i];
-    Own o;
+    flags = T.flags[i];
     ldxr(S.xr, i, o.x, o.r);
     o.v = ld3s(S.v, i, cap);
     o.w = ld3s(S.w, i, cap);
     o.m = T.m[i];
-    const int cnt = nbr_count[i];
+    cnt = nbr_count[i];
+  }
+  const bool live = i < n && !(flags & FLAG_DEAD);
+  if (live) {
+    o.rc0 = ref_coord<0>(o.x.x, P);
+    o.rc1 = ref_coord<1>(o.x.y, P);
+    o.rc2 = ref_coord<2>(o.x.z, P);
+  }
+  unsigned long long defer = 0ull;
+  if constexpr (kSplit) {
+    // Zero-force pass over the list entries below 64, shared by the wave:
+    // a contact whose overlap is past the model's zero-force threshold adds
+    // exactly zero (see contact_pp), so it is only counted (reference-grid
+    // multiplicity); the others are marked for the ordered pass below.
+    // The wave's entries are dealt to its 64 lanes in turn, so the pass
+    // takes (sum of the wave's list lengths) / 64 rounds instead of the
+    // longest list; counts and marks go through LDS atomics (integers, so
+    // the order does not matter).
+    constexpr int kW = kForceBlock / 64;
+    __shared__ uint8_t s_owner[kW][64 * 64];     // lane owning each dealt entry
+    __shared__ int s_excl[kW][64];               // first dealt entry of each lane
+    __shared__ double s_own[kW][5][64];          // x, y, z, r, zero-force threshold
+    __shared__ int s_rc[kW][3][64];              // reference-grid cell
+    __shared__ unsigned int s_cnt[kW][2][64];    // contacts, JKR failures
+    __shared__ unsigned long long s_def[kW][64]; // entries for the ordered pass
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // threshold below which the force is zero; -inf: no zero-force contacts
+    double zthr = -HUGE_VAL;
+    if (live) {
+      const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
+      if (NORMAL == RP_NORMAL_JKR) {
+        const double g = BATCH ? ic->mc.pp.gamma : P.mc.pp.gamma;
+        if (g > 0.0) zthr = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
+      } else {
+        zthr = BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp;
+      }
+    }
+    const int ka = live ? min(cnt, 64) : 0;
+    int incl = ka;
+    for (int sh = 1; sh < 64; sh <<= 1) {
+      const int v = __shfl_up(incl, sh, 64);
+      if (lane >= sh) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int ex = incl - ka;
+    s_excl[wv][lane] = ex;
+    s_own[wv][0][lane] = o.x.x;
+    s_own[wv][1][lane] = o.x.y;
+    s_own[wv][2][lane] = o.x.z;
+    s_own[wv][3][lane] = o.r;
+    s_own[wv][4][lane] = zthr;
+    s_rc[wv][0][lane] = o.rc0;
+    s_rc[wv][1][lane] = o.rc1;
+    s_rc[wv][2][lane] = o.rc2;
+    s_cnt[wv][0][lane] = 0u;
+    s_cnt[wv][1][lane] = 0u;
+    s_def[wv][lane] = 0ull;
+    for (int t = 0; t < ka; ++t) s_owner[wv][ex + t] = (uint8_t)lane;
+    wave_sync();
+    const int ib = i - lane;  // slot of the wave's lane 0
+    for (int f0 = lane; f0 < total; f0 += 2 * 64) {
+      const int f1 = f0 + 64;
+      const bool h1 = f1 < total;
+      const int p0 = s_owner[wv][f0];
+      const int p1 = h1 ? (int)s_owner[wv][f1] : p0;
+      const int k0 = f0 - s_excl[wv][p0];
+      const int k1 = h1 ? f1 - s_excl[wv][p1] : k0;
+      const int j0 = nbr[k0 * cap + ib + p0];
+      const int j1 = h1 ? nbr[k1 * cap + ib + p1] : j0;
+      dv3 xq[2];
+      double rq[2];
+      ldxr(S.xr, j0, xq[0], rq[0]);
+      ldxr(S.xr, j1, xq[1], rq[1]);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (q == 1 && !h1) break;
+        const int p = q ? p1 : p0, k = q ? k1 : k0;
+        const dv3 xo = mk(s_own[wv][0][p], s_own[wv][1][p], s_own[wv][2][p]);
+        const double zt = s_own[wv][4][p];
+        const dv3 d = pair_delta(P, xo, xq[q]);
+        const double dsq = dot(d, d);
+        const double rsum = s_own[wv][3][p] + rq[q];
+        if (!in_contact(P, dsq, rsum)) continue;
+        // distance past (r_sum - threshold)(1 + 1e-6): the overlap is below
+        // the threshold by far more than rounding, so no square root is
+        // needed; nearer pairs take the exact test
+        const double tz = (rsum - zt) * (1.0 + 1e-6);
+        bool zero = dsq > tz * tz;
+        if (!zero) {
+          const double ov = rsum - sqrt(dsq);
+          zero = NORMAL == RP_NORMAL_JKR ? ov < zt - 1e-8 * fabs(ov) : ov < zt;
+        }
+        if (!zero) {
+          atomicOr(&s_def[wv][p], 1ull << k);
+          continue;
+        }
+        const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
+                  cj2 = ref_coord_fast<2>(xq[q].z, P);
+        if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
+            cj2 >= P.ref_dims[2])
+          continue;
+        const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(s_rc[wv][0][p], cj0, P)) *
+                                                 __popc(ref_mask_fast<1>(s_rc[wv][1][p], cj1, P)) *
+                                                 __popc(ref_mask_fast<2>(s_rc[wv][2][p], cj2, P)));
+        if (mult) {
+          atomicAdd(&s_cnt[wv][0][p], mult);
+          if (NORMAL == RP_NORMAL_JKR) atomicAdd(&s_cnt[wv][1][p], mult);
+        }
+      }
+    }
+    wave_sync();
+    acc.ncont += s_cnt[wv][0][lane];
+    acc.njkr += s_cnt[wv][1][lane];
+    defer = s_def[wv][lane];
+  }
+  if (i < n) {
     if (flags & FLAG_DEAD) {
       const double qnan = __builtin_nan("");
       stxr(D.xr, i, mk(qnan, qnan, qnan), qnan);
@@ -1077,9 +1202,6 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
                 cz = fine_coord(o.x.z, 2, P, true);
       const uint32_t cell = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
       const uint32_t t0 = tri_start[cell], t1 = tri_start[cell + 1];
-      o.rc0 = ref_coord<0>(o.x.x, P);
-      o.rc1 = ref_coord<1>(o.x.y, P);
-      o.rc2 = ref_coord<2>(o.x.z, P);
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
       if (JS >= 2) {
         acc.srow = sd.base[i];
@@ -1091,58 +1213,14 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
       const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
       if constexpr (kSplit) {
-        // Two passes.  First, the list entries below 64: a contact whose
-        // overlap is past the model's zero-force threshold adds exactly zero
-        // (see contact_pp), so it is only counted (reference-grid
-        // multiplicity without divisions); the others are marked.  Then the
-        // marked entries and the entries from 64 on get the full contact, in
-        // list order, so the force sums keep the reference's order.
-        const double zthr = NORMAL == RP_NORMAL_JKR ? far_pp
-                                                    : (BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp);
-        const bool zon = NORMAL == RP_NORMAL_SJKR || mc.pp.gamma > 0.0;
-        const int ka = min(cnt, 64);
-        unsigned long long defer = 0ull;
-        for (int base = 0; base < ka; base += kNbrBatch) {
-          int jj[kNbrBatch];
-          dv3 xq[kNbrBatch];
-          double rq[kNbrBatch];
-#pragma unroll
-          for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < ka) ? nbr[(base + q) * cap + i] : -1;
-#pragma unroll
-          for (int q = 0; q < kNbrBatch; ++q) {
-            const int j = jj[q] < 0 ? i : jj[q];
-            ldxr(S.xr, j, xq[q], rq[q]);
-          }
-#pragma unroll
-          for (int q = 0; q < kNbrBatch; ++q) {
-            const dv3 d = pair_delta(P, o.x, xq[q]);
-            const double dsq = dot(d, d);
-            const double rsum = o.r + rq[q];
-            if (!(jj[q] >= 0 && in_contact(P, dsq, rsum))) continue;
-            const double ov = rsum - sqrt(dsq);
-            const bool zero = zon && (NORMAL == RP_NORMAL_JKR ? ov < zthr - 1e-8 * fabs(ov) : ov < zthr);
-            if (!zero) {
-              defer |= 1ull << (base + q);
-              continue;
-            }
-            const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
-                      cj2 = ref_coord_fast<2>(xq[q].z, P);
-            if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
-                cj2 >= P.ref_dims[2])
-              continue;
-            const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(o.rc0, cj0, P)) *
-                                                     __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
-                                                     __popc(ref_mask_fast<2>(o.rc2, cj2, P)));
-            acc.ncont += mult;
-            if (NORMAL == RP_NORMAL_JKR) acc.njkr += mult;
-          }
-        }
+        // the marked entries and the entries from 64 on get the full
+        // contact, in list order, so the force sums keep the reference's order
         while (defer) {
           const int k = __ffsll((long long)defer) - 1;
           defer &= defer - 1ull;
           contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, nbr[k * cap + i], k, acc, ctl, prow, sd);
         }
-        for (int k = ka; k < cnt; ++k) {
+        for (int k = 64; k < cnt; ++k) {
           const int j = nbr[k * cap + i];
           dv3 xj;
           double rj;

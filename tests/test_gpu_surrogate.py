@@ -161,3 +161,57 @@ def test_surrogate_trajectory_tracks_exact(net):
     st = su.jkr_stats()
     print(st)
     assert st["served"] > 0 and st["fallback"] <= st["served"] // 100
+
+
+# A wide envelope: stiff and weakly adhesive pairs down to s / R* = 1e-3,
+# where the reference's solve starts far from the root.
+WIDE = surrogate.envelope(-0.40, 30.0, (5e-6, 1.2e-4), (1e7, 1e10), (0.005, 0.1))
+
+
+def _newton_steps(d, a0, cap=20):
+    """FP64 Newton steps on alpha^2 - sqrt(alpha) = d from a0 (clamped as the
+    engine's polish clamps it) until the engine's residual test passes;
+    cap + 1 when it never does within cap steps."""
+    a = np.clip(a0, 0.45, 1e4)
+    steps = np.full(d.shape, cap + 1)
+    for k in range(cap + 1):
+        ok = (a > 0.45) & (np.abs(a * a - np.sqrt(a) - d) <= 1e-10 * (1.0 + np.abs(d)))
+        steps[ok & (steps > cap)] = k
+        a = a - (a * a - np.sqrt(a) - d) / (2 * a - 0.5 / np.sqrt(a))
+    return steps
+
+
+def test_network_gain_over_exact_solve():
+    """What the trained network gains over the solve without it, on 20,000
+    served contacts drawn over a wide envelope.  Without the network every
+    contact runs the reference's Newton loop (physics.rs:171-189) from
+    sqrt(R* delta) or R*/2, a data-dependent number of iterations (a
+    wavefront waits for its slowest lane).  With it, the engine runs a fixed
+    four FP64 Newton steps from the predicted radius: the test shows that
+    those four steps reach the root for the served contacts, that the
+    reference needs more iterations than that, and that an undertrained
+    network's start does not get there."""
+    net = surrogate.train(WIDE, nsamples=400_000, epochs=40, batch=2048, lr=3e-3, seed=5)
+    weak = surrogate.train(WIDE, nsamples=4_096, epochs=1, batch=2048, lr=3e-3, seed=5)
+    x, t, raw = surrogate.samples(WIDE, 20_000, seed=123)
+    delta, rstar, estar, gamma, a_lab, _, d, ok = raw.T
+    assert ok.min() == 1.0
+    s = (rstar ** 2 * 8 * np.pi * gamma / estar) ** (1 / 3)
+    a_ref, its_ref, conv = osim.jkr_solve(delta, estar, rstar, gamma)
+    assert conv.all() and np.allclose(a_ref, a_lab, rtol=1e-12, atol=0)
+    steps_net = _newton_steps(d, net.predict(d, rstar, estar, gamma)[:, 0])
+    steps_weak = _newton_steps(d, weak.predict(d, rstar, estar, gamma)[:, 0])
+    # a wavefront of 64 contacts waits for its slowest lane
+    wave_ref = its_ref[: its_ref.size // 64 * 64].reshape(-1, 64).max(1)
+    neg = d <= 0.0
+    print("\nNewton iterations per served contact (20,000 contacts, s/R* %.1e..%.2f):" % ((s / rstar).min(),
+                                                                                        (s / rstar).max()))
+    print("  reference solve:    mean %.2f  (d<=0: %.2f, d>0: %.2f)  max %d  per 64-lane wave mean %.2f"
+          % (its_ref.mean(), its_ref[neg].mean(), its_ref[~neg].mean(), its_ref.max(), wave_ref.mean()))
+    print("  trained network:    steps to the root mean %.2f  max %d  within 4: %.4f"
+          % (steps_net.mean(), steps_net.max(), np.mean(steps_net <= 4)))
+    print("  undertrained:       within 4: %.4f" % np.mean(steps_weak <= 4))
+    assert np.mean(steps_net <= 4) >= 0.999      # the polish is exact for the served contacts
+    assert its_ref.mean() >= 1.5 * steps_net.mean()
+    assert wave_ref.mean() > 4.0                 # fixed four steps beat the wave's slowest lane
+    assert np.mean(steps_weak <= 4) < 0.9        # the gain comes from training
