@@ -101,13 +101,45 @@ void gemm(hipStream_t s, const uint16_t* A, int lda, const uint16_t* B, int ldb,
                      act);
 }
 
-// ---- fused inference: 16 rows per wave, weights from global (L2-resident) ----
+// ---- fused inference: 16 rows per wave ---------------------------------------
+// tanh x = 1 - 2 / (e^(2x) + 1) on the transcendental unit (v_exp_f32,
+// v_rcp_f32): absolute error below 1e-6, far under the BF16 rounding the
+// next layer applies to it.
+__device__ inline float tanh_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(fminf(x * 2.8853900817779268f, 64.0f));  // 2^(2x log2 e)
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+// One lane's B operands (weights in fragment order, see mlp_upload) and
+// biases, loaded once per workgroup and reused by every tile it runs.
+struct MlpLane {
+  bf16x4 w1[4], w2[16], w3[4];
+  float b1[4], b2[4], b3;
+};
+
+__device__ inline void mlp_lane_load(const MlpDev& net, MlpLane& L) {
+  const int lane = threadIdx.x & 63, li = lane & 15;
+  const uint2* w1 = reinterpret_cast<const uint2*>(net.w1);
+  const uint2* w2 = reinterpret_cast<const uint2*>(net.w2);
+  const uint2* w3 = reinterpret_cast<const uint2*>(net.w3);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    L.w1[t] = __builtin_bit_cast(bf16x4, w1[t * 64 + lane]);
+    L.w3[t] = __builtin_bit_cast(bf16x4, w3[t * 64 + lane]);
+    L.b1[t] = net.b1[t * 16 + li];
+    L.b2[t] = net.b2[t * 16 + li];
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) L.w2[t] = __builtin_bit_cast(bf16x4, w2[t * 64 + lane]);
+  L.b3 = net.b3[li];
+}
+
 // Layer outputs go through LDS to turn the accumulator layout (row = 4(l/16)+q)
 // into the A-operand layout (row = l % 16) of the next layer.
 // Rows [16 tile, 16 tile + 16) of one wave; every wave of the workgroup
 // calls it together (it contains workgroup barriers).  Waves past the last
 // row compute on zeros and store nothing.
-__device__ inline void mlp_tile(const MlpDev& net, const float* __restrict__ x, float* __restrict__ y, int rows,
+__device__ inline void mlp_tile(const MlpLane& L, const float* __restrict__ x, float* __restrict__ y, int rows,
                                 int tile, float (*h)[kMlpHid + 1]) {
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, kq = 4 * (lane >> 4);
@@ -122,13 +154,10 @@ __device__ inline void mlp_tile(const MlpDev& net, const float* __restrict__ x, 
   }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    bf16x4 b;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) b[q] = (short)net.w1[(kq + q) * kMlpHid + nt * 16 + li];
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, L.w1[nt], acc, 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanhf(acc[q] + net.b1[nt * 16 + li]);
+    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanh_fast(acc[q] + L.b1[nt]);
   }
   __syncthreads();
   // layer 2: [16 x 64] x W2 [64 x 64]
@@ -141,35 +170,28 @@ __device__ inline void mlp_tile(const MlpDev& net, const float* __restrict__ x, 
 #pragma unroll
     for (int q = 0; q < 4; ++q) a2[q] = (short)f2bf(h[li][kt * 16 + kq + q]);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      bf16x4 b;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) b[q] = (short)net.w2[(kt * 16 + kq + q) * kMlpHid + nt * 16 + li];
-      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b, acc2[nt], 0, 0, 0);
-    }
+    for (int nt = 0; nt < 4; ++nt)
+      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, L.w2[kt * 4 + nt], acc2[nt], 0, 0, 0);
   }
   __syncthreads();
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanhf(acc2[nt][q] + net.b2[nt * 16 + li]);
+    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanh_fast(acc2[nt][q] + L.b2[nt]);
   __syncthreads();
   // layer 3: [16 x 64] x W3 [64 x 16(2)]
   f32x4 acc3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int kt = 0; kt < 4; ++kt) {
-    bf16x4 a3, b;
+    bf16x4 a3;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a3[q] = (short)f2bf(h[li][kt * 16 + kq + q]);
-      b[q] = (short)net.w3[(kt * 16 + kq + q) * 16 + li];
-    }
-    acc3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, b, acc3, 0, 0, 0);
+    for (int q = 0; q < 4; ++q) a3[q] = (short)f2bf(h[li][kt * 16 + kq + q]);
+    acc3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, L.w3[kt], acc3, 0, 0, 0);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int r = r0 + kq + q;
-    if (li < kMlpOut && r < rows) y[(size_t)r * kMlpOut + li] = acc3[q] + net.b3[li];
+    if (li < kMlpOut && r < rows) y[(size_t)r * kMlpOut + li] = acc3[q] + L.b3;
   }
   __syncthreads();  // h is reused by the next tile
 }
@@ -177,7 +199,9 @@ __device__ inline void mlp_tile(const MlpDev& net, const float* __restrict__ x, 
 __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __restrict__ x,
                                                    float* __restrict__ y, int rows) {
   __shared__ float hs[4][16][kMlpHid + 1];
-  mlp_tile(net, x, y, rows, blockIdx.x * 4 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
+  MlpLane L;
+  mlp_lane_load(net, L);
+  mlp_tile(L, x, y, rows, blockIdx.x * 4 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
 }
 
 // Row counts read on the device: range r of [0, cap) holds
@@ -188,13 +212,15 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __re
 __global__ void __launch_bounds__(256) k_mlp_counted(MlpDev net, const float* __restrict__ x,
                                                      float* __restrict__ y, const unsigned* counter, int cap) {
   __shared__ float hs[4][16][kMlpHid + 1];
+  MlpLane L;
+  mlp_lane_load(net, L);
   const int range_rows = cap / kSurrRanges;
   for (int g = blockIdx.x; g * 64 < cap; g += gridDim.x) {
     const int r = (g * 64) / range_rows;
     const int first = r * range_rows;
     const int rows = (int)min(counter[r * kSurrCounterStride], (unsigned)range_rows);
     if (g * 64 - first >= rows) continue;
-    mlp_tile(net, x + (size_t)first * kMlpIn, y + (size_t)first * kMlpOut, rows, (g * 64 - first) / 16 + (threadIdx.x >> 6),
+    mlp_tile(L, x + (size_t)first * kMlpIn, y + (size_t)first * kMlpOut, rows, (g * 64 - first) / 16 + (threadIdx.x >> 6),
              hs[threadIdx.x >> 6]);
   }
 }
@@ -530,10 +556,24 @@ int mlp_upload(MlpHost& m, const float* params, hipStream_t stream) {
   if (!m.b && hipMalloc(&m.b, (64 + 64 + 16) * sizeof(float)) != hipSuccess) return RP_ERR_NOMEM;
   std::vector<float> pp;
   pad_params(params, pp);
+  // Fragment order: lane l of tile t holds the MFMA B operand
+  // W[k0 + 4 (l / 16) + q][n0 + l % 16], q = 0..3, at 4 (64 t + l) + q, so
+  // each lane loads its operand of a tile as one 8-byte word.
   std::vector<uint16_t> wb(16 * 64 + 64 * 64 + 64 * 16);
-  for (int i = 0; i < 16 * 64; ++i) wb[i] = f2bf(pp[kPW1 + i]);
-  for (int i = 0; i < 64 * 64; ++i) wb[16 * 64 + i] = f2bf(pp[kPW2 + i]);
-  for (int i = 0; i < 64 * 16; ++i) wb[16 * 64 + 64 * 64 + i] = f2bf(pp[kPW3 + i]);
+  uint16_t* f1 = wb.data();
+  uint16_t* f2 = f1 + 16 * 64;
+  uint16_t* f3 = f2 + 64 * 64;
+  for (int l = 0; l < 64; ++l) {
+    const int li = l % 16, kq = 4 * (l / 16);
+    for (int q = 0; q < 4; ++q) {
+      for (int nt = 0; nt < 4; ++nt) f1[4 * (64 * nt + l) + q] = f2bf(pp[kPW1 + (kq + q) * 64 + nt * 16 + li]);
+      for (int kt = 0; kt < 4; ++kt) {
+        for (int nt = 0; nt < 4; ++nt)
+          f2[4 * (64 * (kt * 4 + nt) + l) + q] = f2bf(pp[kPW2 + (kt * 16 + kq + q) * 64 + nt * 16 + li]);
+        f3[4 * (64 * kt + l) + q] = f2bf(pp[kPW3 + (kt * 16 + kq + q) * 16 + li]);
+      }
+    }
+  }
   std::vector<float> bb(64 + 64 + 16);
   for (int i = 0; i < 64; ++i) bb[i] = pp[kPB1 + i];
   for (int i = 0; i < 64; ++i) bb[64 + i] = pp[kPB2 + i];
