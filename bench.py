@@ -216,6 +216,13 @@ def run_batch(args) -> int:
     sep_value = n * sep_steps / total
     log(f"one engine per instance: {sep_steps} steps each, {total:.3f}s in all, {sep_value:.4g} particle-steps/s")
 
+    # what a plain streaming copy reaches on this GPU (with --calibrate-copy
+    # also the reference point of a counter profile)
+    nbytes = (1 << 30) if not args.calibrate_copy else 256 * 1024 * 1024
+    ms = native.probe_copy(nbytes)
+    copy_gbs = 2 * nbytes / ms / 1e6
+    log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -245,6 +252,7 @@ def run_batch(args) -> int:
                    "form": "batched" if main == "plain" else f"{main}+batched"},
         "roofline": {"bound": "hbm", "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)},
+        "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "contacts_per_step": R["contacts_per_step"], "force_kernel_avg_us": R["force_kernel_avg_us"],
         "one_engine_per_instance_value": sep_value,

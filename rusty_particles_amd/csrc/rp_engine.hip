@@ -1578,7 +1578,13 @@ struct rp_engine {
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  std::vector<std::pair<int, size_t>> ev_marks;  // (kind, index of start event); end = start + 1
+  struct Mark {
+    int kind;
+    size_t idx;      // start event; end event = idx + 1
+    uint64_t count;  // launches the mark covers
+  };
+  std::vector<Mark> ev_marks;
+  bool mark_tail = false;  // the last work enqueued is the end of the last mark
   // batched engine: instances, their materials, the device table
   int ninst = 1;
   std::vector<rp_material> inst_particle, inst_wall;
@@ -2057,20 +2063,32 @@ hipEvent_t next_event(rp_engine* e) {
 // Offset of the next launch's label from the device's label base.
 unsigned long long label_off(const rp_engine* e) { return (unsigned long long)(e->launch_no - e->label_host); }
 
+// Timing marks: an event pair around a launch of one kind.  Force launches
+// that follow each other with nothing else enqueued in between share one
+// pair (its end event is recorded again after each), so their average is
+// the run's time over its launches, without an event record between them.
 void mark_begin(rp_engine* e, int kind) {
   if (!e->timing) return;
+  if (kind == 0 && e->mark_tail && !e->ev_marks.empty() && e->ev_marks.back().kind == 0) {
+    e->ev_marks.back().count += 1;
+    return;
+  }
   size_t idx = e->ev_used;
   hipEvent_t a = next_event(e);
   hipEvent_t b = next_event(e);
   if (!a || !b) return;
   (void)hipEventRecord(a, e->stream);
-  e->ev_marks.emplace_back(kind, idx);
+  e->ev_marks.push_back(rp_engine::Mark{kind, idx, 1});
 }
 
 void mark_end(rp_engine* e) {
   if (!e->timing || e->ev_marks.empty()) return;
-  (void)hipEventRecord(e->ev_pool[e->ev_marks.back().second + 1], e->stream);
+  (void)hipEventRecord(e->ev_pool[e->ev_marks.back().idx + 1], e->stream);
+  e->mark_tail = true;
 }
+
+// Other work is about to be enqueued: the next mark starts afresh.
+void mark_break(rp_engine* e) { e->mark_tail = false; }
 
 // Device view of the surrogate state (null pointers when it is not in use).
 SurrDev surr_dev(rp_engine* e) {
@@ -2543,8 +2561,10 @@ rp_status rp_engine_set_record(rp_engine* e, int on) {
 rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
   if (nsteps == 0) return RP_OK;
+  mark_break(e);
   rp_status st = prepare(e);
   if (st) return st;
+  mark_break(e);
   const int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   uint64_t remaining = nsteps;
   for (int round = 0; remaining > 0; ++round) {
@@ -2588,6 +2608,7 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
       mark_end(e);
     }
     HIP_TRY(hipGetLastError());
+    mark_break(e);
     // One sync per call: count the launches that advanced the state.  Launches
     // stalled by a skin violation or a list overflow are issued again.
     HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
@@ -2959,6 +2980,7 @@ rp_status rp_engine_set_timing(rp_engine* e, int on) {
   e->timing = on != 0;
   e->ev_used = 0;
   e->ev_marks.clear();
+  e->mark_tail = false;
   return RP_OK;
 }
 
@@ -2971,12 +2993,13 @@ rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count) {
   }
   for (auto& mk : e->ev_marks) {
     float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, e->ev_pool[mk.second], e->ev_pool[mk.second + 1]));
-    ms[mk.first] += (double)t;
-    count[mk.first] += 1;
+    HIP_TRY(hipEventElapsedTime(&t, e->ev_pool[mk.idx], e->ev_pool[mk.idx + 1]));
+    ms[mk.kind] += (double)t;
+    count[mk.kind] += mk.count;
   }
   e->ev_used = 0;
   e->ev_marks.clear();
+  e->mark_tail = false;
   return RP_OK;
 }
 

@@ -83,3 +83,31 @@ def test_autotune_keeps_state_and_results():
     ref, _ = _run(p, sc, True, [13, 120], [(v9, n3)])
     _same(ref, e.download())
     assert e.stats()["steps"] == 133
+
+
+def test_timing_counts_every_force_launch():
+    """Timing mode: every force launch is counted once (consecutive launches
+    share one event pair), the rebuild chain at least at the call's start,
+    and the timed force work fits in the call's wall time."""
+    import time
+    sc = dense_scene(counts=(12, 12, 12), seed=23, vel=0.3)
+    p = params(1e-5, (-0.08,) * 3, (0.08,) * 3)
+    e = native.Engine(copy_params(p))
+    e.upload(sc["x"], sc["radius"], sc["mass"], sc.get("fixed"), v=sc.get("v"), w=sc.get("w"))
+    e.step(3)
+    e.set_timing(True)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.step(25)
+    e.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    tm = e.timing()
+    e.set_timing(False)
+    assert tm["force"][1] == 25
+    assert tm["rebuild"][1] >= 1
+    assert 0.0 < tm["force"][0] <= wall_ms
+    # timing does not change the state: the same 28 steps without it
+    f = native.Engine(copy_params(p))
+    f.upload(sc["x"], sc["radius"], sc["mass"], sc.get("fixed"), v=sc.get("v"), w=sc.get("w"))
+    f.step(28)
+    _same(e.download(), f.download())
