@@ -29,6 +29,7 @@
 // while a rebuild re-orders the particles.  Neighbour lists are column-major
 // (entry k of slot i at nbr[k*cap + i]).  No float atomics anywhere: results
 // are bitwise reproducible run to run.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1584,7 +1585,7 @@ struct rp_engine {
     uint64_t count;  // launches the mark covers
   };
   std::vector<Mark> ev_marks;
-  bool mark_tail = false;  // the last work enqueued is the end of the last mark
+  hipEvent_t ext_start = nullptr, ext_stop = nullptr;  // the next force launch's timing pair
   // batched engine: instances, their materials, the device table
   int ninst = 1;
   std::vector<rp_material> inst_particle, inst_wall;
@@ -2063,32 +2064,30 @@ hipEvent_t next_event(rp_engine* e) {
 // Offset of the next launch's label from the device's label base.
 unsigned long long label_off(const rp_engine* e) { return (unsigned long long)(e->launch_no - e->label_host); }
 
-// Timing marks: an event pair around a launch of one kind.  Force launches
-// that follow each other with nothing else enqueued in between share one
-// pair (its end event is recorded again after each), so their average is
-// the run's time over its launches, without an event record between them.
+// Timing marks: an event pair around the launches of one kind.  A force
+// launch (kind 0) is timed by its own dispatch (hipExtLaunchKernel records
+// the pair at the kernel's start and end), so its time is the kernel's
+// execution alone, without the gaps that separate event records add.
 void mark_begin(rp_engine* e, int kind) {
   if (!e->timing) return;
-  if (kind == 0 && e->mark_tail && !e->ev_marks.empty() && e->ev_marks.back().kind == 0) {
-    e->ev_marks.back().count += 1;
-    return;
-  }
   size_t idx = e->ev_used;
   hipEvent_t a = next_event(e);
   hipEvent_t b = next_event(e);
   if (!a || !b) return;
-  (void)hipEventRecord(a, e->stream);
+  if (kind == 0) {
+    e->ext_start = a;
+    e->ext_stop = b;
+  } else {
+    (void)hipEventRecord(a, e->stream);
+  }
   e->ev_marks.push_back(rp_engine::Mark{kind, idx, 1});
 }
 
 void mark_end(rp_engine* e) {
   if (!e->timing || e->ev_marks.empty()) return;
+  if (e->ev_marks.back().kind == 0) return;  // recorded by the launch itself
   (void)hipEventRecord(e->ev_pool[e->ev_marks.back().idx + 1], e->stream);
-  e->mark_tail = true;
 }
-
-// Other work is about to be enqueued: the next mark starts afresh.
-void mark_break(rp_engine* e) { e->mark_tail = false; }
 
 // Device view of the surrogate state (null pointers when it is not in use).
 SurrDev surr_dev(rp_engine* e) {
@@ -2164,11 +2163,22 @@ void launch_surrogate(rp_engine* e) {
 template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
 void launch_force_t(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kForceBlock - 1) / kForceBlock);
-  hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
-                     label_off(e), e->kp,
-                     e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
-                     e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
-                     JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+  if (e->ext_start) {
+    // timing mode: the dispatch records the kernel's start and end
+    hipExtLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream,
+                          e->ext_start, e->ext_stop, 0u, e->d_ctl, flag_slot, label_off(e), e->kp,
+                          e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], (const int*)e->d_nbr_count,
+                          (const int*)e->d_nbr, (const Tri*)e->d_tris, (const uint32_t*)e->d_tri_start,
+                          (const uint32_t*)e->d_tri_idx, (const float*)e->d_xb, e->d_bstats, e->d_pairbuf,
+                          e->d_paircount, JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+    e->ext_start = e->ext_stop = nullptr;
+  } else {
+    hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
+                       label_off(e), e->kp,
+                       e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
+                       e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
+                       JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+  }
   e->force_grid = g;
   e->sstats_valid = JS >= 2;
   e->cur ^= 1;
@@ -2561,10 +2571,8 @@ rp_status rp_engine_set_record(rp_engine* e, int on) {
 rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
   if (nsteps == 0) return RP_OK;
-  mark_break(e);
   rp_status st = prepare(e);
   if (st) return st;
-  mark_break(e);
   const int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   uint64_t remaining = nsteps;
   for (int round = 0; remaining > 0; ++round) {
@@ -2608,7 +2616,6 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
       mark_end(e);
     }
     HIP_TRY(hipGetLastError());
-    mark_break(e);
     // One sync per call: count the launches that advanced the state.  Launches
     // stalled by a skin violation or a list overflow are issued again.
     HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
@@ -2980,7 +2987,7 @@ rp_status rp_engine_set_timing(rp_engine* e, int on) {
   e->timing = on != 0;
   e->ev_used = 0;
   e->ev_marks.clear();
-  e->mark_tail = false;
+  e->ext_start = e->ext_stop = nullptr;
   return RP_OK;
 }
 
@@ -2999,7 +3006,7 @@ rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count) {
   }
   e->ev_used = 0;
   e->ev_marks.clear();
-  e->mark_tail = false;
+  e->ext_start = e->ext_stop = nullptr;
   return RP_OK;
 }
 

@@ -86,9 +86,10 @@ def test_autotune_keeps_state_and_results():
 
 
 def test_timing_counts_every_force_launch():
-    """Timing mode: every force launch is counted once (consecutive launches
-    share one event pair), the rebuild chain at least at the call's start,
-    and the timed force work fits in the call's wall time."""
+    """Timing mode: every force launch is counted once (its dispatch records
+    its own start and end), the rebuild chain at least at the call's start,
+    the timed force work fits in the call's wall time, and the state is the
+    one an untimed run reaches."""
     import time
     sc = dense_scene(counts=(12, 12, 12), seed=23, vel=0.3)
     p = params(1e-5, (-0.08,) * 3, (0.08,) * 3)
