@@ -56,9 +56,11 @@ def run_oracle(p, sc, meshes=(), steps=1, bits=64, record=True):
     return o
 
 
-def run_engine(p, sc, meshes=(), steps=1, record=True, chunks=None):
+def run_engine(p, sc, meshes=(), steps=1, record=True, chunks=None, solver=None):
     from rusty_particles_amd import native
     e = native.Engine(copy_params(p))
+    if solver:
+        e.set_jkr_solver(solver)
     e.upload(sc["x"], sc["radius"], sc["mass"], sc.get("fixed"), v=sc.get("v"), w=sc.get("w"), a=sc.get("a"),
              tau=sc.get("tau"))
     for v9, n3 in meshes:
@@ -67,7 +69,7 @@ def run_engine(p, sc, meshes=(), steps=1, record=True, chunks=None):
     if chunks:
         for c in chunks:
             e.step(c)
-    else:
+    elif steps:
         e.step(steps)
     return e
 
