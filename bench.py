@@ -79,6 +79,9 @@ def build_workload(name: str, n: int | None):
     if name == "jkr":
         p, x, r, m, v = scenes.jkr_periodic_workload(n or 2_000_000)
         return p, x, r, m, v, [], "JKR+Coulomb periodic packing"
+    if name == "jkr-dense":
+        p, x, r, m, v = scenes.jkr_dense_workload(n or 2_000_000)
+        return p, x, r, m, v, [], "JKR+Coulomb compressed FCC crystal"
     raise SystemExit(f"unknown workload {name}")
 
 

@@ -1948,6 +1948,10 @@ rp_status alloc_cells(rp_engine* e) {
 }
 
 rp_status alloc_nbr(rp_engine* e, int nbr_cap) {
+  // the kernels address list entry k of slot i as k * cap + i in 32 bits
+  if ((size_t)nbr_cap * (size_t)e->cap >= ((size_t)1 << 31))
+    return fail(RP_ERR_NOMEM, "neighbour lists would exceed 2^31 entries (" + std::to_string(nbr_cap) +
+                                  " per particle x " + std::to_string(e->cap) + " slots)");
   e->gen++;
   free_ptr(e->d_nbr);
   e->d_nbr = nullptr;

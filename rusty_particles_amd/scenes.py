@@ -241,6 +241,40 @@ def jkr_periodic_workload(n: int = 2_000_000, seed: int = 0xD3E50003, phi: float
     return p, x, r, m, v
 
 
+def jkr_dense_workload(n: int = 2_000_000, seed: int = 0xD3E50007, spread: float = 0.02, gap: float = -1e-6):
+    """A compressed adhesive crystal: the JKR material of the 2M-particle
+    packing (E=1e8, nu=0.3, gamma 0.05, rho 1500, Coulomb mu=0.3) with radii
+    50 um x U[1 - spread, 1 + spread] on a periodic FCC lattice whose
+    nearest-neighbour distance is 100 um + `gap` (default: 1 um closer), so
+    each sphere has 12 adhesive contacts, most of them with a JKR root
+    (overlaps of -1 um ... +3 um, i.e. d = delta / t of about -4 ... 13), and
+    6 second neighbours inside the 1.5 r_sum cut-off without one; jitter
+    +-0.2 um, Gaussian velocities (std 0.05 m/s per component), dt=1e-8 s."""
+    rng = np.random.default_rng(seed)
+    m_cells = max(1, int(round((n / 4.0) ** (1.0 / 3.0))))
+    a = 100e-6 + gap
+    side = a * np.sqrt(2.0)  # cubic FCC cell
+    base = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    g = np.arange(m_cells)
+    cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    x = ((cells[:, None, :] + base[None, :, :]).reshape(-1, 3) + 0.25) * side
+    x += rng.uniform(-0.2e-6, 0.2e-6, x.shape)
+    L = m_cells * side
+    nn = x.shape[0]
+    perm = rng.permutation(nn)
+    x = x[perm]
+    r = 50e-6 * rng.uniform(1.0 - spread, 1.0 + spread, nn)
+    m = sphere_mass(r, 1500.0)
+    v = rng.normal(0.0, 0.05, (nn, 3))
+    p = abi.default_params(1e-8, (0.0, 0.0, 0.0), (L, L, L))
+    for k in range(3):
+        p.periodic[k] = 1
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, 0.05)
+    return p, x, r, m, v
+
+
 def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int = 2000, nmax: int = 50000):
     """Ensemble of `ninst` independent simulations for a material sweep:
     N_k ~ U{nmin..nmax} spheres, r ~ U[1, 3] mm, rho 2500, per-instance
