@@ -805,6 +805,8 @@ __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rst
     *failed = 1;
     return zero3();
   }
+  // a compressed contact always has a root (the bound below is negative)
+  if (JS == 1 && ov >= 0.0) return jkr(ov, n, c, rstar, failed);
   const double s = cbrt(rstar * rstar * c.jc);
   const double t = s * s / rstar;
   if (ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov))) {
