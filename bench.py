@@ -207,7 +207,7 @@ def run_batch(args) -> int:
     # the same instances, one engine each, run back to back (graph on)
     sep_steps = max(args.steps // 4, 20)
     total = 0.0
-    for k in range(len(counts)):
+    for k in range(0 if args.no_compare else len(counts)):
         e = engine(k, k + 1, main, batch=False)
         e.step(args.warmup)
         e.synchronize()
@@ -216,8 +216,9 @@ def run_batch(args) -> int:
         e.synchronize()
         total += time.perf_counter() - t0
         e.close()
-    sep_value = n * sep_steps / total
-    log(f"one engine per instance: {sep_steps} steps each, {total:.3f}s in all, {sep_value:.4g} particle-steps/s")
+    sep_value = n * sep_steps / total if total > 0.0 else None
+    if sep_value:
+        log(f"one engine per instance: {sep_steps} steps each, {total:.3f}s in all, {sep_value:.4g} particle-steps/s")
 
     # what a plain streaming copy reaches on this GPU (with --calibrate-copy
     # also the reference point of a counter profile)
@@ -311,6 +312,8 @@ def main() -> int:
                     help="CPU-baseline sample length: as many steps as fit, going by the warm-up step")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch workloads: skip the one-engine-per-instance comparison")
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--jkr-forms", default="exact,shortcut,surrogate",
                     help="JKR solvers to time on the jkr workload (the last one is the reported value)")

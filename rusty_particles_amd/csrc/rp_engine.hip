@@ -805,46 +805,49 @@ __device__ inline dv3 jkr_fast(double ov, dv3 n, const PairConsts& c, double rst
     *failed = 1;
     return zero3();
   }
-  // a compressed contact always has a root (the bound below is negative)
-  if (JS == 1 && ov >= 0.0) return jkr(ov, n, c, rstar, failed);
-  const double s = cbrt(rstar * rstar * c.jc);
-  const double t = s * s / rstar;
-  if (ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov))) {
-    *failed = 1;
-    return zero3();
-  }
-  if (JS >= 2) {
-    if (k < 64 && acc.sok && rstar >= sd.r_lo && rstar <= sd.r_hi && s >= kSurrSLo * rstar &&
-        s <= kSurrSHi * rstar && ov >= sd.md_lo * t && ov <= sd.md_hi * t)
-      acc.nmask |= 1ull << k;
-    double d = 0.0;
-    if (surr_route(ov, rstar, s, t, sd, k, d, acc.sok)) {
-      const unsigned long long bit = 1ull << k;  // k < 64 here
-      const int row = (acc.rmask & bit) ? acc.srow + __popcll(acc.rmask & (bit - 1ull)) : -1;
-      if (row >= 0 && row < sd.rows_cap) {
-        const float y0 = sd.y[2 * (size_t)row], y1 = sd.y[2 * (size_t)row + 1];
-        double al = (double)unscale_out(y0, sd.sc.out_lo[0], sd.sc.out_hi[0]);
-        if (polish_alpha(d, al)) {
-          const dv3 f = jkr_finish(al * s, ov, n, c, rstar, failed);
-          ++acc.sserved;
-          if (JS == 3) {
-            int ef = 0;
-            const dv3 fe = jkr(ov, n, c, rstar, &ef);
-            const double fpull = c.jpull * rstar;  // -(2/3) pi gamma R*
-            const double scale = fmax(len(fe), -fpull);
-            const double err = len(sub(f, fe)) / scale;
-            double fraw = (double)unscale_out(y1, sd.sc.out_lo[1], sd.sc.out_hi[1]) * (kPi * c.gamma * rstar);
-            if (fraw < fpull) fraw = 0.0;
-            const double raw = fabs(fraw - dot(fe, n)) / scale;
-            acc.emax = fmaxf(acc.emax, (float)err);
-            acc.rmax = fmaxf(acc.rmax, (float)raw);
-            acc.esum += err;
-            acc.rsum += raw;
+  // A compressed contact always has a root (the bound below is negative), so
+  // the shortcut solver skips the cube root there.  Every path ends in the
+  // one jkr() call below, so a wavefront runs the Newton loop once.
+  if (JS >= 2 || ov < 0.0) {
+    const double s = cbrt(rstar * rstar * c.jc);
+    const double t = s * s / rstar;
+    if (ov < kJkrDMin * t - 1e-3 * rstar - 1e-8 * (rstar + t + fabs(ov))) {
+      *failed = 1;
+      return zero3();
+    }
+    if (JS >= 2) {
+      if (k < 64 && acc.sok && rstar >= sd.r_lo && rstar <= sd.r_hi && s >= kSurrSLo * rstar &&
+          s <= kSurrSHi * rstar && ov >= sd.md_lo * t && ov <= sd.md_hi * t)
+        acc.nmask |= 1ull << k;
+      double d = 0.0;
+      if (surr_route(ov, rstar, s, t, sd, k, d, acc.sok)) {
+        const unsigned long long bit = 1ull << k;  // k < 64 here
+        const int row = (acc.rmask & bit) ? acc.srow + __popcll(acc.rmask & (bit - 1ull)) : -1;
+        if (row >= 0 && row < sd.rows_cap) {
+          const float y0 = sd.y[2 * (size_t)row], y1 = sd.y[2 * (size_t)row + 1];
+          double al = (double)unscale_out(y0, sd.sc.out_lo[0], sd.sc.out_hi[0]);
+          if (polish_alpha(d, al)) {
+            const dv3 f = jkr_finish(al * s, ov, n, c, rstar, failed);
+            ++acc.sserved;
+            if (JS == 3) {
+              int ef = 0;
+              const dv3 fe = jkr(ov, n, c, rstar, &ef);
+              const double fpull = c.jpull * rstar;  // -(2/3) pi gamma R*
+              const double scale = fmax(len(fe), -fpull);
+              const double err = len(sub(f, fe)) / scale;
+              double fraw = (double)unscale_out(y1, sd.sc.out_lo[1], sd.sc.out_hi[1]) * (kPi * c.gamma * rstar);
+              if (fraw < fpull) fraw = 0.0;
+              const double raw = fabs(fraw - dot(fe, n)) / scale;
+              acc.emax = fmaxf(acc.emax, (float)err);
+              acc.rmax = fmaxf(acc.rmax, (float)raw);
+              acc.esum += err;
+              acc.rsum += raw;
+            }
+            return f;
           }
-          return f;
         }
+        ++acc.sfall;
       }
-      ++acc.sfall;
     }
   }
   return jkr(ov, n, c, rstar, failed);
@@ -1534,6 +1537,11 @@ struct rp_engine {
   double v_max = 0.0;  // fastest uploaded particle (sizes the automatic skin)
   uint64_t step_count = 0;
   bool have_particles = false;
+  // a skin picked by the autotuner is widened when it turns out too thin
+  // for later motion (see skin_guard)
+  bool skin_tuned = false;
+  uint64_t guard_steps = 0;
+  unsigned long long guard_r0 = 0;
   bool geometry_dirty = true;
   bool record = false;
   KParams kp{};
@@ -2374,6 +2382,7 @@ rp_status rp_engine_set_params(rp_engine* e, const rp_params* params) {
   rp_status st = validate(params);
   if (st) return st;
   e->prm = *params;
+  e->skin_tuned = false;
   e->geometry_dirty = true;
   return RP_OK;
 }
@@ -2574,6 +2583,28 @@ rp_status rp_engine_set_record(rp_engine* e, int on) {
   return RP_OK;
 }
 
+constexpr uint64_t kGuardChunk = 200;  // steps between checks of a tuned skin
+
+// The autotuner measures the motion of its moment: particles released at
+// rest make a thin skin look best, and once they move it rebuilds the lists
+// at nearly every check.  After a tuned skin has run kGuardChunk steps or
+// more, if the lists were rebuilt at more than a quarter of the checks (one
+// every kRebuildInterval launches), the skin is doubled, up to the automatic
+// skin, and the lists are rebuilt with it.  Skin never changes a result.
+void skin_guard(rp_engine* e, uint64_t executed) {
+  if (!e->skin_tuned) return;
+  e->guard_steps += executed;
+  if (e->guard_steps < kGuardChunk) return;
+  const unsigned long long r = e->host_ctl.rebuilds - e->guard_r0;
+  const double auto_skin = std::max(0.1 * e->r_max, 4.0 * (double)kRebuildInterval * e->v_max * e->prm.dt);
+  if (r * 4ull * (unsigned long long)kRebuildInterval > e->guard_steps && e->prm.skin < auto_skin) {
+    e->prm.skin = std::min(2.0 * e->prm.skin, auto_skin);
+    e->geometry_dirty = true;
+  }
+  e->guard_steps = 0;
+  e->guard_r0 = e->host_ctl.rebuilds;
+}
+
 rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
   if (nsteps == 0) return RP_OK;
@@ -2581,8 +2612,9 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (st) return st;
   const int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   uint64_t remaining = nsteps;
-  for (int round = 0; remaining > 0; ++round) {
-    if (round > 64) return fail(RP_ERR_STATE, "time step made no progress after 64 neighbour-list rebuilds");
+  int stalled = 0;  // rounds that ended with stalled launches
+  while (remaining > 0) {
+    if (stalled > 64) return fail(RP_ERR_STATE, "time step made no progress after 64 neighbour-list rebuilds");
     if (e->record && (st = ensure_pairs(e))) return st;
     if (e->at_rest) {
       mark_begin(e, 2);
@@ -2590,10 +2622,14 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
                          e->kp, e->dyn[e->cur], e->stat[0], e->d_xb, label_off(e));
       mark_end(e);
     }
-    for (uint64_t s = 0; s < remaining; ++s) {
+    // A tuned skin is checked every kGuardChunk steps (skin_guard).
+    const uint64_t chunk = e->skin_tuned ? std::min<uint64_t>(remaining, kGuardChunk) : remaining;
+    const bool final_chunk = chunk == remaining;
+    for (uint64_t s = 0; s < chunk; ++s) {
       // Whole captured loops where they fit (aligned to the rebuild cadence,
       // never the call's last launch, which leaves the at-rest state).
-      if (e->use_graph && !e->timing && s > 0 && e->launch_no % kGraphSteps == 0 && remaining - s > kGraphSteps) {
+      if (e->use_graph && !e->timing && s > 0 && e->launch_no % kGraphSteps == 0 &&
+          (final_chunk ? chunk - s > kGraphSteps : chunk - s >= kGraphSteps)) {
         if ((st = replay_steps(e))) return st;
         s += kGraphSteps - 1;
         continue;
@@ -2606,7 +2642,7 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
         mark_end(e);
       }
       const int slot = (int)(e->chain_no & 1);
-      const bool last = (s + 1 == remaining);
+      const bool last = final_chunk && s + 1 == chunk;
       if (surr_active(e) && !(last && e->record)) {
         mark_begin(e, 3);
         launch_surrogate(e);
@@ -2622,7 +2658,8 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
       mark_end(e);
     }
     HIP_TRY(hipGetLastError());
-    // One sync per call: count the launches that advanced the state.  Launches
+    // One sync per round (the call, or a chunk of it while a tuned skin is
+    // checked): count the launches that advanced the state.  Launches
     // stalled by a skin violation or a list overflow are issued again.
     HIP_TRY(hipMemcpyAsync(&e->host_ctl, e->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -2637,7 +2674,14 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
     }
     e->at_rest = e->host_ctl.at_rest != 0;
     remaining -= executed;
+    skin_guard(e, executed);
     if (remaining == 0) break;
+    if (executed == chunk) {
+      // the chunk ran whole: go on (with the lists of a widened skin)
+      if (e->geometry_dirty && (st = prepare(e))) return st;
+      continue;
+    }
+    ++stalled;
     if (e->host_ctl.overflow) {
       // a rebuild found more candidates than the lists hold: grow them
       int want = (int)round_up((size_t)(e->host_ctl.max_count * 3 / 2 + 16), 8);
@@ -2894,6 +2938,7 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   if (steps == 0) return fail(RP_ERR_INVALID, "steps must be positive");
   rp_status st = prepare(e);
   if (st) return st;
+  e->skin_tuned = false;  // no guard while the candidates run
   HIP_TRY(hipStreamSynchronize(e->stream));
   // snapshot of everything a step reads: state buffers, control block, host
   // counters (the lists are rebuilt from the state)
@@ -2926,12 +2971,19 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
     return RP_OK;
   };
   // candidates: the current setting first, then skins as a fraction of the
-  // largest radius (never below the motion bound the automatic skin uses)
+  // largest radius (never below the motion bound the automatic skin uses,
+  // nor the fall bound below)
   // with automatic or half-cutoff cells.  Each runs in chunks of `steps`
   // until it has rebuilt its lists 3 times or run 10 chunks, so a small skin
   // pays for its frequent rebuilds; another setting must beat the current one
   // by 3 % to replace it.
-  const double vmin = 4.0 * (double)kRebuildInterval * e->v_max * e->prm.dt;
+  // Speeds grow after tuning under gravity: a skin must also hold the fall of
+  // a particle released at rest over 1,000 steps, |g| (1000 dt)^2 (twice the
+  // distance, as the rebuild check allows half the skin).
+  const double gmag = std::sqrt(e->prm.gravity[0] * e->prm.gravity[0] + e->prm.gravity[1] * e->prm.gravity[1] +
+                                e->prm.gravity[2] * e->prm.gravity[2]);
+  const double fall = gmag * (1000.0 * e->prm.dt) * (1000.0 * e->prm.dt);
+  const double vmin = std::max(4.0 * (double)kRebuildInterval * e->v_max * e->prm.dt, fall);
   const double rmax = e->r_max > 0.0 ? e->r_max : 1e-3;
   const double fr[] = {0.2, 0.1, 0.05, 0.02, 0.01, 0.005};
   const double cut = e->kp.margin_factor * 2.0 * rmax;
@@ -2972,6 +3024,11 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   rp_status rs = restore();
   e->prm.skin = st ? skin0 : best_skin;
   e->prm.cell_size = st ? cell0 : best_cell;
+  if (!st && !rs && best_skin != skin0) {
+    e->skin_tuned = true;
+    e->guard_steps = 0;
+    e->guard_r0 = c0.rebuilds;
+  }
   (void)hipFree(sdyn);
   (void)hipFree(sstat);
   if (st) return st;

@@ -112,3 +112,37 @@ def test_timing_counts_every_force_launch():
     f.upload(sc["x"], sc["radius"], sc["mass"], sc.get("fixed"), v=sc.get("v"), w=sc.get("w"))
     f.step(28)
     _same(e.download(), f.download())
+
+
+def test_tuned_skin_keeps_up_with_falling_particles():
+    """A lattice released at rest under gravity: at rest every skin looks
+    cheap, but the particles speed up after tuning.  The autotuner keeps the
+    skin above the fall over 1,000 steps and the engine widens a tuned skin
+    that rebuilds at more than a quarter of the checks, so the last 500
+    steps of a long call rebuild at fewer than a third of the checks -- and
+    the result equals an untuned run's bit for bit (skin and cell size never
+    change one)."""
+    p, counts, mats, x, r, m, meshes = scenes.batch_ensemble_workload(ninst=1, nmin=6000, nmax=6000)
+    p.particle = mats[0]
+    sc = dict(x=x, radius=r, mass=m, fixed=np.zeros(x.shape[0], np.uint8))
+
+    def make():
+        e = native.Engine(copy_params(p))
+        e.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"])
+        for v9, n3 in meshes:
+            e.add_mesh(v9, n3)
+        return e
+
+    tuned = make()
+    tuned.step(2)
+    skin, cell = tuned.autotune(20)
+    print(f"autotune: skin {skin:.3g} m, cell {cell:.3g} m (r_max {r.max():.3g} m)")
+    tuned.step(2998)
+    r0 = tuned.stats()["rebuilds"]
+    tuned.step(500)
+    late = tuned.stats()["rebuilds"] - r0
+    print(f"rebuilds in the last 500 steps: {late}")
+    assert late < 17
+    plain = make()
+    plain.step(3500)
+    _same(tuned.download(), plain.download())
