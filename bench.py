@@ -331,8 +331,8 @@ def main() -> int:
     if args.autotune is None:
         args.autotune = 0 if args.workload in ("hopper", "box") else 20
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not part of "
-                                   "this build (use --gpus 1)"}))
+        print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not "
+                                   "supported (use --gpus 1)"}))
         return 2
 
     import numpy as np
