@@ -153,7 +153,7 @@ def run_batch(args) -> int:
         e.synchronize()
         if args.autotune:
             sk, ce = e.autotune(args.autotune)
-            log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m")
+            log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m, tiling {e.tiling()}")
         t0 = time.perf_counter()
         e.step(steps)
         e.synchronize()
@@ -384,7 +384,7 @@ def main() -> int:
             t0 = time.perf_counter()
             tuned = eng.autotune(args.autotune)
             log(f"[{form}] autotune: skin {tuned[0]:.3g} m, cell {tuned[1]:.3g} m (0 = automatic), "
-                f"{time.perf_counter() - t0:.1f}s")
+                f"tiling {eng.tiling()}, {time.perf_counter() - t0:.1f}s")
 
         # timed region: whole job, K steps, one enqueue, synchronised on both sides
         eng.synchronize()
@@ -408,7 +408,8 @@ def main() -> int:
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
                "achieved_gbs": achieved, "contacts_per_step": st["contacts"],
                "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st,
-               "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None}
+               "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None,
+               "tiling": eng.tiling()}
         if tm["surrogate"][1]:
             res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
         log(f"[{form}] force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
@@ -478,7 +479,7 @@ def main() -> int:
         "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
         "jkr_failures_per_step": st["jkr_failures"],
-        "autotuned_skin_m": R["autotuned_skin_m"], "autotuned_cell_m": R["autotuned_cell_m"],
+        "autotuned_skin_m": R["autotuned_skin_m"], "autotuned_cell_m": R["autotuned_cell_m"], "tiling": R["tiling"],
     }
     if len(results) > 1 or main_form != "plain":
         out["forms"] = {f: {k: v for k, v in res.items() if k not in ("stats", "wall")} for f, res in results.items()}

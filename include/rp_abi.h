@@ -243,15 +243,29 @@ rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
 
 /* Autotuner (no counterpart in the reference): from the current state,
  * times each candidate Verlet skin (the current one, then 0.2 .. 0.005 of
- * the largest radius, never below the automatic skin's motion bound) and
- * broad-phase cell (automatic or half the largest cutoff + skin) over chunks
- * of `steps` steps until it has rebuilt its lists 3 times (at most 10
- * chunks), keeps the fastest in the engine's parameters (the current one
- * unless beaten by 3 %) and returns it (0 = automatic).  The particle state is
- * restored, so the simulation continues as if the call had not been made
- * (skin and cell size change only which candidates the lists hold, never a
- * result). */
+ * the largest radius, never below the automatic skin's motion bound nor the
+ * fall of a particle released at rest over 1,000 steps) and broad-phase cell
+ * (automatic or half the largest cutoff + skin) over chunks of `steps` steps
+ * until it has rebuilt its lists 3 times (at most 10 chunks), keeps the
+ * fastest in the engine's parameters (the current one unless beaten by 3 %)
+ * and returns it (0 = automatic).  Where the force kernel has a zero-force
+ * pass (JKR with the shortcut or surrogate, sJKR; Coulomb or spring
+ * tangential law) it then times the other tiling of that pass
+ * (rp_engine_set_tiling) with the chosen skin and cell and keeps the faster.
+ * The particle state is restored, so the simulation continues as if the call
+ * had not been made (skin, cell size and tiling never change a result).  A
+ * tuned skin is widened later, up to the automatic skin, when more than a
+ * quarter of the rebuild checks of 200 steps rebuilt the lists. */
 rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, double* cell_out);
+
+/* Tiling of the force kernel's zero-force pass (no counterpart in the
+ * reference): RP_TILING_LANE, one lane per particle walks its list;
+ * RP_TILING_WAVE (default), a wavefront deals the list entries of its 64
+ * particles over its lanes in turn.  Results are the same either way. */
+#define RP_TILING_LANE 0
+#define RP_TILING_WAVE 1
+rp_status rp_engine_set_tiling(rp_engine* e, int32_t tiling);
+rp_status rp_engine_get_tiling(rp_engine* e, int32_t* tiling);
 
 /* Step loop in HIP graphs (on by default; no counterpart in the reference):
  * rp_engine_step replays a captured loop of 20 launches (two rebuild-chain

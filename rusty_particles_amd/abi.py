@@ -72,6 +72,8 @@ class RpParams(C.Structure):
 RP_MLP_PARAMS = 4610
 RP_TIMING_KINDS = 4
 RP_JKR_EXACT, RP_JKR_SHORTCUT, RP_JKR_SURROGATE = 0, 1, 2
+RP_TILING_LANE, RP_TILING_WAVE = 0, 1
+TILINGS = {"lane": RP_TILING_LANE, "wave": RP_TILING_WAVE}
 JKR_SOLVERS = {"exact": RP_JKR_EXACT, "shortcut": RP_JKR_SHORTCUT, "surrogate": RP_JKR_SURROGATE}
 
 
@@ -165,6 +167,8 @@ EXPORTS = (
     "rp_engine_upload_batch",
     "rp_engine_set_graph",
     "rp_engine_autotune",
+    "rp_engine_set_tiling",
+    "rp_engine_get_tiling",
     "rp_engine_instance_counts",
     "rp_engine_jkr_stats",
     "rp_engine_destroy",

@@ -328,6 +328,16 @@ __device__ inline int xcd_block(int b, int nb) {
   return x * q + min(x, r) + k;
 }
 
+// JKR past its no-root bound (shortcut and surrogate solvers) and sJKR past
+// its cut-off have zero normal force; with Coulomb or the spring law (G* = 0)
+// such a contact adds nothing and is only counted, in a pass of its own.
+// Mindlin propagates its NaN for negative overlaps, and recording needs every
+// pair in order.
+constexpr bool split_pass(int normal, int tang, bool record, int js) {
+  return !record && tang != RP_TANGENTIAL_MINDLIN &&
+         ((normal == RP_NORMAL_JKR && js >= 1) || normal == RP_NORMAL_SJKR);
+}
+
 // Orders the LDS accesses of one wavefront's lanes around this point (each
 // wave works in its own LDS slice, so no workgroup barrier is needed).
 __device__ inline void wave_sync() {
@@ -1020,7 +1030,7 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
-template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH>
+template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH, bool DEAL>
 __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
@@ -1028,12 +1038,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
                                                   unsigned long long* pairbuf, int* paircount, SurrDev sd,
                                                   SurrStats* sstats) {
   const int n = ctl->n;
-  // JKR past its no-root bound (shortcut and surrogate solvers) and sJKR past
-  // its cut-off have zero normal force; with Coulomb or the spring law (G* = 0)
-  // such a contact adds nothing and is only counted.  Mindlin propagates its
-  // NaN for negative overlaps, and recording needs every pair in order.
-  constexpr bool kSplit = !RECORD && TANG != RP_TANGENTIAL_MINDLIN &&
-                          ((NORMAL == RP_NORMAL_JKR && JS >= 1) || NORMAL == RP_NORMAL_SJKR);
+  // zero-force pass (split_pass), dealt over the wave (DEAL) or per lane
+  constexpr bool kSplit = split_pass(NORMAL, TANG, RECORD, JS);
   // the rows of this step have been read by the MLP kernel; the next step's
   // collect kernel starts from zero
   if (JS >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1093,7 +1099,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
     o.rc2 = ref_coord<2>(o.x.z, P);
   }
   unsigned long long defer = 0ull;
-  if constexpr (kSplit) {
+  if constexpr (kSplit && DEAL) {
     // Zero-force pass over the list entries below 64, shared by the wave:
     // a contact whose overlap is past the model's zero-force threshold adds
     // exactly zero (see contact_pp), so it is only counted (reference-grid
@@ -1219,6 +1225,49 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
       const double far_pp = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
       const double far_pw = BATCH ? ic->jkr_far_pw : P.jkr_far_pw;
       if constexpr (kSplit) {
+        if constexpr (!DEAL) {
+          // The zero-force pass in one lane per particle (the tiling the
+          // autotuner may pick instead of the wave-dealt one above).
+          const double zthr = NORMAL == RP_NORMAL_JKR ? far_pp
+                                                      : (BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp);
+          const bool zon = NORMAL == RP_NORMAL_SJKR || mc.pp.gamma > 0.0;
+          const int ka = min(cnt, 64);
+          for (int base = 0; base < ka; base += kNbrBatch) {
+            int jj[kNbrBatch];
+            dv3 xq[kNbrBatch];
+            double rq[kNbrBatch];
+#pragma unroll
+            for (int q = 0; q < kNbrBatch; ++q) jj[q] = (base + q < ka) ? nbr[(base + q) * cap + i] : -1;
+#pragma unroll
+            for (int q = 0; q < kNbrBatch; ++q) {
+              const int j = jj[q] < 0 ? i : jj[q];
+              ldxr(S.xr, j, xq[q], rq[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < kNbrBatch; ++q) {
+              const dv3 d = pair_delta(P, o.x, xq[q]);
+              const double dsq = dot(d, d);
+              const double rsum = o.r + rq[q];
+              if (!(jj[q] >= 0 && in_contact(P, dsq, rsum))) continue;
+              const double ov = rsum - sqrt(dsq);
+              const bool zero = zon && (NORMAL == RP_NORMAL_JKR ? ov < zthr - 1e-8 * fabs(ov) : ov < zthr);
+              if (!zero) {
+                defer |= 1ull << (base + q);
+                continue;
+              }
+              const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
+                        cj2 = ref_coord_fast<2>(xq[q].z, P);
+              if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
+                  cj2 >= P.ref_dims[2])
+                continue;
+              const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(o.rc0, cj0, P)) *
+                                                       __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
+                                                       __popc(ref_mask_fast<2>(o.rc2, cj2, P)));
+              acc.ncont += mult;
+              if (NORMAL == RP_NORMAL_JKR) acc.njkr += mult;
+            }
+          }
+        }
         // the marked entries and the entries from 64 on get the full
         // contact, in list order, so the force sums keep the reference's order
         while (defer) {
@@ -1562,6 +1611,7 @@ struct rp_engine {
   uint64_t graph_gen[2] = {0, 0};
   uint64_t gen = 1;        // bumped by every change to what launches capture
   bool use_graph = true;
+  int tiling = RP_TILING_WAVE;  // zero-force pass of the force kernel
   uint64_t graph_replays = 0;
   uint64_t steps_done = 0;  // steps that advanced the state (device count at the last sync)
   bool at_rest = true;    // state is at rest (needs k_begin before the next force launch)
@@ -2174,12 +2224,12 @@ void launch_surrogate(rp_engine* e) {
   mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
 }
 
-template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
-void launch_force_t(rp_engine* e, int flag_slot) {
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH, bool DEAL>
+void launch_force_d(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kForceBlock - 1) / kForceBlock);
   if (e->ext_start) {
     // timing mode: the dispatch records the kernel's start and end
-    hipExtLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream,
+    hipExtLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL>), dim3(g), dim3(kForceBlock), 0, e->stream,
                           e->ext_start, e->ext_stop, 0u, e->d_ctl, flag_slot, label_off(e), e->kp,
                           e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], (const int*)e->d_nbr_count,
                           (const int*)e->d_nbr, (const Tri*)e->d_tris, (const uint32_t*)e->d_tri_start,
@@ -2187,7 +2237,7 @@ void launch_force_t(rp_engine* e, int flag_slot) {
                           e->d_paircount, JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
     e->ext_start = e->ext_stop = nullptr;
   } else {
-    hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
+    hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
                        label_off(e), e->kp,
                        e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
                        e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
@@ -2197,6 +2247,17 @@ void launch_force_t(rp_engine* e, int flag_slot) {
   e->sstats_valid = JS >= 2;
   e->cur ^= 1;
   e->launch_no++;
+}
+
+// Kernels with a zero-force pass come in both tilings of it.
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
+void launch_force_t(rp_engine* e, int flag_slot) {
+  if constexpr (split_pass(NORMAL, TANG, REC, JS)) {
+    if (e->tiling == RP_TILING_WAVE) launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, true>(e, flag_slot);
+    else launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false>(e, flag_slot);
+  } else {
+    launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false>(e, flag_slot);
+  }
 }
 
 // JKR solver variants are compiled for the JKR model only; a recording
@@ -3020,6 +3081,42 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
       best_cell = cand[q].second;
     }
   }
+  // the other tiling of the zero-force pass, where the force kernel has one,
+  // with the chosen skin and cell
+  const int tiling0 = e->tiling;
+  int best_tiling = tiling0;
+  const int js = e->kp.normal == RP_NORMAL_JKR ? (e->jkr_solver == RP_JKR_EXACT ? 0 : 1) : 0;
+  const bool has_split = split_pass(e->kp.normal, e->kp.tangential, false, js);
+  if (st == RP_OK && has_split) {
+    const int other = tiling0 == RP_TILING_WAVE ? RP_TILING_LANE : RP_TILING_WAVE;
+    double t_other = HUGE_VAL;
+    for (int pass = 0; pass < 2 && st == RP_OK; ++pass) {
+      // pass 0: the chosen setting again with the current tiling (same
+      // warm-up as the other), pass 1: the other tiling
+      if ((st = restore())) break;
+      e->prm.skin = best_skin;
+      e->prm.cell_size = best_cell;
+      (void)rp_engine_set_tiling(e, pass == 0 ? tiling0 : other);
+      if ((st = rp_engine_step(e, 2))) break;
+      if ((st = read_ctl(e))) break;
+      const unsigned long long r0 = e->host_ctl.rebuilds;
+      double t = 0.0;
+      uint64_t done = 0;
+      for (int chunk = 0; chunk < 10; ++chunk) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((st = rp_engine_step(e, steps))) break;
+        if ((st = read_ctl(e))) break;
+        t += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        done += steps;
+        if (e->host_ctl.rebuilds - r0 >= 3) break;
+      }
+      const double per = t / (double)std::max<uint64_t>(done, 1);
+      if (pass == 0) best_t = per;
+      else t_other = per;
+    }
+    if (st == RP_OK && t_other < 0.97 * best_t) best_tiling = other;
+  }
+  (void)rp_engine_set_tiling(e, st ? tiling0 : best_tiling);
   e->timing = timing0;
   rp_status rs = restore();
   e->prm.skin = st ? skin0 : best_skin;
@@ -3035,6 +3132,22 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   if (rs) return rs;
   if (skin_out) *skin_out = best_skin;
   if (cell_out) *cell_out = best_cell;
+  return RP_OK;
+}
+
+rp_status rp_engine_set_tiling(rp_engine* e, int32_t tiling) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (tiling != RP_TILING_LANE && tiling != RP_TILING_WAVE) return fail(RP_ERR_INVALID, "unknown tiling");
+  if (tiling != e->tiling) {
+    e->tiling = tiling;
+    e->gen++;  // captured step loops hold the other kernel
+  }
+  return RP_OK;
+}
+
+rp_status rp_engine_get_tiling(rp_engine* e, int32_t* tiling) {
+  if (!e || !tiling) return fail(RP_ERR_INVALID, "NULL argument");
+  *tiling = e->tiling;
   return RP_OK;
 }
 

@@ -63,6 +63,8 @@ def lib() -> C.CDLL:
                                          _dp, _u8p]
     L.rp_engine_instance_counts.argtypes = [_eng, C.POINTER(C.c_size_t)]
     L.rp_engine_set_graph.argtypes = [_eng, C.c_int]
+    L.rp_engine_set_tiling.argtypes = [_eng, C.c_int32]
+    L.rp_engine_get_tiling.argtypes = [_eng, C.POINTER(C.c_int32)]
     L.rp_engine_autotune.argtypes = [_eng, C.c_uint32, _dp, _dp]
     L.rp_engine_destroy.argtypes = [_eng]
     L.rp_engine_destroy.restype = None
@@ -74,7 +76,7 @@ def lib() -> C.CDLL:
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
                  "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph",
-                 "rp_engine_autotune"):
+                 "rp_engine_autotune", "rp_engine_set_tiling", "rp_engine_get_tiling"):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
@@ -237,11 +239,23 @@ class Engine:
         return offs, js[: tot.value]
 
     def autotune(self, steps: int = 20) -> tuple[float, float]:
-        """Pick the fastest Verlet skin and cell size for the current state
-        (the state itself is restored); returns (skin, cell_size), 0 = automatic."""
+        """Pick the fastest Verlet skin and cell size for the current state,
+        then the faster tiling of the force kernel's zero-force pass (the
+        state itself is restored); returns (skin, cell_size), 0 = automatic;
+        tiling() gives the tiling."""
         skin, cell = C.c_double(0.0), C.c_double(0.0)
         _check(self._L.rp_engine_autotune(self._h, int(steps), C.byref(skin), C.byref(cell)))
         return skin.value, cell.value
+
+    def set_tiling(self, tiling: str) -> None:
+        """'wave' (default): a wavefront deals its particles' list entries over
+        its lanes in the zero-force pass; 'lane': one lane per particle."""
+        _check(self._L.rp_engine_set_tiling(self._h, abi.TILINGS[tiling]))
+
+    def tiling(self) -> str:
+        t = C.c_int32(0)
+        _check(self._L.rp_engine_get_tiling(self._h, C.byref(t)))
+        return {v: k for k, v in abi.TILINGS.items()}[t.value]
 
     def set_graph(self, on: bool) -> None:
         """Replay captured 20-launch step loops (default on)."""

@@ -146,3 +146,36 @@ def test_tuned_skin_keeps_up_with_falling_particles():
     plain = make()
     plain.step(3500)
     _same(tuned.download(), plain.download())
+
+
+@pytest.mark.parametrize("normal,solver,batched", [(abi.RP_NORMAL_SJKR, None, False),
+                                                   (abi.RP_NORMAL_JKR, "shortcut", False),
+                                                   (abi.RP_NORMAL_SJKR, None, True)],
+                         ids=["sjkr", "jkr-shortcut", "sjkr-batched"])
+def test_tilings_of_the_zero_force_pass_agree(normal, solver, batched):
+    """One lane per particle and the wave-dealt zero-force pass give the same
+    state bit for bit (integer counts, forces added in list order)."""
+    sc = dense_scene(counts=(12, 12, 12), seed=31, vel=0.3)
+    p = params(1e-5, (-0.08,) * 3, (0.08,) * 3, normal=normal, gamma=0.05)
+    v9, n3 = scenes.box_mesh((-0.07,) * 3, (0.07,) * 3)
+    out = {}
+    for tiling in ("lane", "wave"):
+        e = native.Engine(copy_params(p))
+        if batched:
+            n = sc["x"].shape[0]
+            counts = [n // 3, n // 3, n - 2 * (n // 3)]
+            mats = [abi.RpMaterial(1e7, 0.3, 2500.0, 0.2 + 0.2 * k, 0.5, 0.02 * k) for k in range(3)]
+            e.upload_batch(counts, mats, sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+        else:
+            e.upload(sc["x"], sc["radius"], sc["mass"], sc.get("fixed"), v=sc.get("v"), w=sc.get("w"))
+        e.add_mesh(v9, n3)
+        if solver:
+            e.set_jkr_solver(solver)
+        e.set_tiling(tiling)
+        assert e.tiling() == tiling
+        for c in (7, 61, 52):
+            e.step(c)
+        out[tiling] = (e.download(), e.stats())
+    _same(out["lane"][0], out["wave"][0])
+    for k in ("contacts", "wall_contacts", "jkr_failures", "rebuilds"):
+        assert out["lane"][1][k] == out["wave"][1][k], k
