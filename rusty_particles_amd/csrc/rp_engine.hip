@@ -81,7 +81,7 @@ constexpr uint64_t kRebuildInterval = 10;        // steps between launches of th
 constexpr long long kNoStall = 0x7fffffffffffffffLL;  // no stalled step recorded
 constexpr uint64_t kGraphSteps = 2 * kRebuildInterval;  // launches per captured step loop (two rebuild chains,
                                                         // so buffer and request parities repeat)
-constexpr int kNbrBatch = 2;                     // neighbours whose loads the force kernel issues together
+constexpr int kNbrBatch = 4;                     // neighbours whose loads the force kernel issues together
 constexpr double kPi = 3.14159265358979323846;
 // Margin (in d = delta / t) of the surrogate's near masks: a contact whose d
 // changes by more than this in one step, into the served set, takes the exact
