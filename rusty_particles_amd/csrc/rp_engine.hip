@@ -81,6 +81,10 @@ constexpr uint64_t kRebuildInterval = 10;        // steps between launches of th
 constexpr long long kNoStall = 0x7fffffffffffffffLL;  // no stalled step recorded
 constexpr uint64_t kGraphSteps = 2 * kRebuildInterval;  // launches per captured step loop (two rebuild chains,
                                                         // so buffer and request parities repeat)
+#ifndef RP_DEAL_BATCH
+#define RP_DEAL_BATCH 4
+#endif
+constexpr int kDealBatch = RP_DEAL_BATCH;  // entries per lane and round of the dealt zero-force pass
 constexpr int kNbrBatch = 4;                     // neighbours whose loads the force kernel issues together
 constexpr double kPi = 3.14159265358979323846;
 // Margin (in d = delta / t) of the surrogate's near masks: a contact whose d
@@ -1153,23 +1157,27 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, 
     for (int t = 0; t < ka; ++t) s_owner[wv][ex + t] = (uint8_t)lane;
     wave_sync();
     const int ib = i - lane;  // slot of the wave's lane 0
-    for (int f0 = lane; f0 < total; f0 += 2 * 64) {
-      const int f1 = f0 + 64;
-      const bool h1 = f1 < total;
-      const int p0 = s_owner[wv][f0];
-      const int p1 = h1 ? (int)s_owner[wv][f1] : p0;
-      const int k0 = f0 - s_excl[wv][p0];
-      const int k1 = h1 ? f1 - s_excl[wv][p1] : k0;
-      const int j0 = nbr[k0 * cap + ib + p0];
-      const int j1 = h1 ? nbr[k1 * cap + ib + p1] : j0;
-      dv3 xq[2];
-      double rq[2];
-      ldxr(S.xr, j0, xq[0], rq[0]);
-      ldxr(S.xr, j1, xq[1], rq[1]);
+    // each lane takes kDealBatch entries per round, their loads issued together
+    for (int f0 = lane; f0 < total; f0 += kDealBatch * 64) {
+      int pq[kDealBatch], kq[kDealBatch], jq[kDealBatch];
+      bool hq[kDealBatch];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        if (q == 1 && !h1) break;
-        const int p = q ? p1 : p0, k = q ? k1 : k0;
+      for (int q = 0; q < kDealBatch; ++q) {
+        const int f = f0 + 64 * q;
+        hq[q] = f < total;
+        pq[q] = hq[q] ? (int)s_owner[wv][f] : pq[0];
+        kq[q] = hq[q] ? f - s_excl[wv][pq[q]] : kq[0];
+      }
+#pragma unroll
+      for (int q = 0; q < kDealBatch; ++q) jq[q] = hq[q] ? nbr[kq[q] * cap + ib + pq[q]] : jq[0];
+      dv3 xq[kDealBatch];
+      double rq[kDealBatch];
+#pragma unroll
+      for (int q = 0; q < kDealBatch; ++q) ldxr(S.xr, jq[q], xq[q], rq[q]);
+#pragma unroll
+      for (int q = 0; q < kDealBatch; ++q) {
+        if (!hq[q]) break;
+        const int p = pq[q], k = kq[q];
         const dv3 xo = mk(s_own[wv][0][p], s_own[wv][1][p], s_own[wv][2][p]);
         const double zt = s_own[wv][4][p];
         const dv3 d = pair_delta(P, xo, xq[q]);
