@@ -146,6 +146,8 @@ def run_batch(args) -> int:
             e.add_mesh(v9, n3)
         if form != "plain":
             e.set_jkr_solver(form, net)
+        if args.no_graph:
+            e.set_graph(False)
         return e
 
     def timed(e, steps):
@@ -312,6 +314,8 @@ def main() -> int:
                     help="CPU-baseline sample length: as many steps as fit, going by the warm-up step")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel on its own (no HIP-graph step loop), e.g. under a profiler")
     ap.add_argument("--no-compare", action="store_true",
                     help="batch workloads: skip the one-engine-per-instance comparison")
     ap.add_argument("--timing-steps", type=int, default=100)
@@ -371,6 +375,8 @@ def main() -> int:
         if eng is not None:
             eng.close()
         eng = native.Engine(abi.RpParams.from_buffer_copy(p))
+        if args.no_graph:
+            eng.set_graph(False)
         eng.upload(x, r, m, v=v)
         for v9, n3 in meshes:
             eng.add_mesh(v9, n3)
