@@ -72,6 +72,13 @@ constexpr int kBlock = 256;
 #define RP_FORCE_BLOCK 128
 #endif
 constexpr int kForceBlock = RP_FORCE_BLOCK;
+// Four waves per SIMD for every force kernel: the JKR and sJKR variants need
+// 140-150 VGPRs (three waves) unconstrained; capped at 128 they spill a few
+// registers and still run 5-12 % faster (batch 592 -> 532 us, JKR shortcut
+// 406 -> 361 us per launch); the hopper kernel fits in 128 anyway.
+#ifndef RP_FORCE_ATTR
+#define RP_FORCE_ATTR __attribute__((amdgpu_waves_per_eu(4)))
+#endif
 constexpr uint8_t FLAG_FIXED = 1;
 constexpr uint8_t FLAG_DEAD = 2;
 constexpr int kScanItems = 16;                   // items per thread in the scan
@@ -1038,7 +1045,7 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
 template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH, bool DEAL>
-__global__ void __launch_bounds__(kForceBlock) k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
+__global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
                                                   const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
