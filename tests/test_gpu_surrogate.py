@@ -148,6 +148,25 @@ def test_surrogate_forces_match_exact(net):
     assert ex.stats()["jkr_failures"] == su.stats()["jkr_failures"]
 
 
+def test_surrogate_tilings_agree(net):
+    """The surrogate form with the zero-force pass in one lane per particle
+    and dealt over the wavefront: same state bit for bit, same served rows."""
+    d = np.random.default_rng(13).uniform(-0.5, 20.0, 2000)
+    sc = jkr_dimers(d, vel=0.2, seed=13)
+    p = _jkr_params(sc)
+    out = {}
+    for tiling in ("lane", "wave"):
+        e = native.Engine(copy_params(p))
+        e.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+        e.set_jkr_solver("surrogate", net)
+        e.set_tiling(tiling)
+        e.step(57)
+        out[tiling] = (e.download(), e.jkr_stats())
+    for k in ("x", "v", "w", "a", "tau"):
+        assert np.array_equal(out["lane"][0][k], out["wave"][0][k]), k
+    assert out["lane"][1]["served"] == out["wave"][1]["served"] > 0
+
+
 def test_surrogate_trajectory_tracks_exact(net):
     """Dimers with approach velocities cross every regime during 300 steps;
     the surrogate run stays within 1e-9 of the particle radius."""
