@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -187,7 +188,10 @@ struct KParams {
   double go[3];      // origin
   double ginv[3];    // 1 / cell size
   int gn[3];         // cells per axis
-  int gstride[3];    // linear index strides
+  int gstride[3];    // row-major strides (host)
+  // cell number of fine coordinate c on axis k:
+  // (c >> gsh[k]) * ghi[k] + (c & (2^gsh[k] - 1)) * glo[k]  (see cell_linear)
+  int gsh[3], ghi[3], glo[3];
   unsigned int ncells;      // fine cells of one instance
   unsigned int ncells_all;  // ncells * ninst: sort buckets (instance-major)
   int ninst;
@@ -277,6 +281,20 @@ __device__ inline int fine_coord(double x, int k, const KParams& P, bool clamp_o
   if (c < 0) return 0;
   if (c >= n) return n - 1;
   return (int)c;
+}
+
+// Number of the fine cell (c0, c1, c2), coordinates inside the grid.  Row
+// major (gsh = 0, ghi = stride), or in "pencils": 8 x 8 cells of the two
+// fast axes, each pencil running the whole slow axis, pencils in row-major
+// order -- so a contiguous range of cells is a compact block of space, not a
+// full-width layer, and neighbour gathers stay in L2.  Any numbering gives
+// the same lists (they are sorted by stable id).
+template <int K>
+__host__ __device__ inline uint32_t cell_term(int c, const KParams& P) {
+  return (uint32_t)((c >> P.gsh[K]) * P.ghi[K] + (c & ((1 << P.gsh[K]) - 1)) * P.glo[K]);
+}
+__host__ __device__ inline uint32_t cell_linear(const KParams& P, int c0, int c1, int c2) {
+  return cell_term<0>(c0, P) + cell_term<1>(c1, P) + cell_term<2>(c2, P);
 }
 
 // Reference-grid cell coordinate (grid.rs:84-86): floor((x - origin)/cs) as i32.
@@ -389,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, D
       int cx = fine_coord(x.x, 0, P, false);
       int cy = fine_coord(x.y, 1, P, false);
       int cz = fine_coord(x.z, 2, P, false);
-      c = (uint32_t)T.inst[i] * P.ncells + (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+      c = (uint32_t)T.inst[i] * P.ncells + cell_linear(P, cx, cy, cz);
     }
     key[i] = c;
     atomicAdd(&cell_count[c], 1u);
@@ -568,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
         for (int cx = lo[0]; cx <= hi[0]; ++cx) {
           int wx = cx;
           if (P.periodic[0] && !all[0]) wx = (cx % P.gn[0] + P.gn[0]) % P.gn[0];
-          uint32_t c = ibase + (uint32_t)(wx * P.gstride[0] + wy * P.gstride[1] + wz * P.gstride[2]);
+          uint32_t c = ibase + cell_linear(P, wx, wy, wz);
           uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
           for (uint32_t j = s0; j < s1; ++j) {
             if ((int)j == i) continue;
@@ -1230,7 +1248,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
     } else {
       const int cx = fine_coord(o.x.x, 0, P, true), cy = fine_coord(o.x.y, 1, P, true),
                 cz = fine_coord(o.x.z, 2, P, true);
-      const uint32_t cell = (uint32_t)(cx * P.gstride[0] + cy * P.gstride[1] + cz * P.gstride[2]);
+      const uint32_t cell = cell_linear(P, cx, cy, cz);
       const uint32_t t0 = tri_start[cell], t1 = tri_start[cell + 1];
       unsigned long long* prow = RECORD ? pairbuf + (size_t)i * P.pair_cap : nullptr;
       if (JS >= 2) {
@@ -1844,8 +1862,39 @@ rp_status compute_geometry(rp_engine* e) {
     K.gstride[order[q]] = stride;
     stride *= K.gn[order[q]];
   }
-  K.ncells = (unsigned int)stride;
   K.ninst = std::max(e->ninst, 1);
+  // Pencil numbering (cell_linear) when a layer of the slow axis holds many
+  // particles (cubic boxes): there a contiguous run of cells in row-major
+  // order is a thin full-width slab whose neighbours do not fit in L2.
+  // RP_CELL_ORDER=row|pencil overrides the choice.
+  const int fa = order[0], ma = order[1], sa = order[2];
+  const double per_layer = (double)e->cap / ((double)K.ninst * (double)K.gn[sa]);
+  bool pencil = per_layer > 8192.0 && K.gn[fa] >= 16 && K.gn[ma] >= 16;
+  if (const char* env = std::getenv("RP_CELL_ORDER")) {
+    if (std::string(env) == "row") pencil = false;
+    if (std::string(env) == "pencil") pencil = true;
+  }
+  const double pf = std::ceil(K.gn[fa] / 8.0), pm = std::ceil(K.gn[ma] / 8.0);
+  if (pencil && pm * pf * K.gn[sa] * 64.0 * K.ninst > kMaxCells) pencil = false;
+  if (pencil) {
+    K.gsh[sa] = 0;
+    K.ghi[sa] = 64;
+    K.glo[sa] = 0;
+    K.gsh[fa] = 3;
+    K.ghi[fa] = K.gn[sa] * 64;
+    K.glo[fa] = 1;
+    K.gsh[ma] = 3;
+    K.ghi[ma] = (int)pf * K.gn[sa] * 64;
+    K.glo[ma] = 8;
+    K.ncells = (unsigned int)(pm * pf * K.gn[sa] * 64.0);
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      K.gsh[k] = 0;
+      K.ghi[k] = K.gstride[k];
+      K.glo[k] = 0;
+    }
+    K.ncells = (unsigned int)stride;
+  }
   K.ncells_all = K.ncells * (unsigned int)K.ninst;
   // build positions are stored as float: |x - float(x)| <= |x| 2^-24 per axis
   double xmax = 0.0;
@@ -1899,7 +1948,7 @@ rp_status upload_meshes_binned(rp_engine* e) {
     for (int z = c0[2]; z <= c1[2]; ++z)
       for (int y = c0[1]; y <= c1[1]; ++y)
         for (int x = c0[0]; x <= c1[0]; ++x) {
-          uint32_t c = (uint32_t)(x * K.gstride[0] + y * K.gstride[1] + z * K.gstride[2]);
+          uint32_t c = cell_linear(K, x, y, z);
           entries.emplace_back(c, (uint32_t)t);
         }
   }
