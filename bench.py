@@ -34,6 +34,10 @@ spheres, r lognormal median 50 um, surface energy ~U[0.02, 0.08] J/m^2) in
 one batched engine, once per JKR solver (exact, shortcut, surrogate); value
 = the surrogate+batched form.
 
+Timing: --steps steps per timed region, --repeats regions (default 3) back to
+back on the same run, value = the median region; contacts/s = the contacts
+of the last step times steps/s.
+
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
 HIP events on the engine's stream), and a CPU baseline: the oracle (a C++
@@ -151,20 +155,25 @@ def run_batch(args) -> int:
         return e
 
     def timed(e, steps):
+        """Median wall time of --repeats timed regions of `steps` steps."""
         e.step(args.warmup)
         e.synchronize()
         if args.autotune:
             sk, ce = e.autotune(args.autotune)
             log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m, tiling {e.tiling()}")
-        t0 = time.perf_counter()
-        e.step(steps)
-        e.synchronize()
-        return time.perf_counter() - t0
+        walls = []
+        for _ in range(max(args.repeats, 1)):
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.step(steps)
+            e.synchronize()
+            walls.append(time.perf_counter() - t0)
+        return float(np.median(walls)), walls
 
     results = {}
     for form in forms:
         e = engine(0, len(counts), form)
-        wall = timed(e, args.steps)
+        wall, walls = timed(e, args.steps)
         value = n * args.steps / wall
         st = e.stats()
         e.set_timing(True)
@@ -174,7 +183,8 @@ def run_batch(args) -> int:
         avg_s = tm["force"][0] / max(tm["force"][1], 1) * 1e-3
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
                "achieved_gbs": FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9, "contacts_per_step": st["contacts"],
-               "wall": wall, "stats": st}
+               "contacts_per_s": st["contacts"] * args.steps / wall,
+               "repeat_values": [n * args.steps / w for w in walls], "wall": wall, "stats": st}
         if tm["surrogate"][1]:
             res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
         log(f"[{form}] batched: {args.steps} steps in {wall:.3f}s, {value:.4g} particle-steps/s, force kernel "
@@ -201,7 +211,7 @@ def run_batch(args) -> int:
     if not jkr:
         e = engine(0, len(counts), main)
         e.set_graph(False)
-        wall_off = timed(e, args.steps)
+        wall_off, _ = timed(e, args.steps)
         graph_off = n * args.steps / wall_off
         log(f"batched, graph off: {graph_off:.4g} particle-steps/s")
         e.close()
@@ -260,7 +270,9 @@ def run_batch(args) -> int:
                      "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)},
         "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
-        "contacts_per_step": R["contacts_per_step"], "force_kernel_avg_us": R["force_kernel_avg_us"],
+        "repeats": len(R["repeat_values"]), "repeat_values": R["repeat_values"],
+        "contacts_per_step": R["contacts_per_step"], "contacts_per_s": R["contacts_per_s"],
+        "force_kernel_avg_us": R["force_kernel_avg_us"],
         "one_engine_per_instance_value": sep_value,
     }
     if graph_off is not None:
@@ -319,6 +331,8 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true",
                     help="batch workloads: skip the one-engine-per-instance comparison")
     ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed regions of --steps steps each, back to back on the same run; value = the median")
     ap.add_argument("--jkr-forms", default="exact,shortcut,surrogate",
                     help="JKR solvers to time on the jkr workload (the last one is the reported value)")
     ap.add_argument("--train-samples", type=int, default=1_000_000)
@@ -392,12 +406,16 @@ def main() -> int:
             log(f"[{form}] autotune: skin {tuned[0]:.3g} m, cell {tuned[1]:.3g} m (0 = automatic), "
                 f"tiling {eng.tiling()}, {time.perf_counter() - t0:.1f}s")
 
-        # timed region: whole job, K steps, one enqueue, synchronised on both sides
-        eng.synchronize()
-        t0 = time.perf_counter()
-        eng.step(args.steps)
-        eng.synchronize()
-        wall = time.perf_counter() - t0
+        # timed regions: whole job, K steps each, one enqueue, synchronised on
+        # both sides; value = the median region
+        walls = []
+        for _ in range(max(args.repeats, 1)):
+            eng.synchronize()
+            t0 = time.perf_counter()
+            eng.step(args.steps)
+            eng.synchronize()
+            walls.append(time.perf_counter() - t0)
+        wall = float(np.median(walls))
         st = eng.stats()
         value = n * args.steps / wall
         log(f"[{form}] timed {args.steps} steps: {wall:.3f}s, {value:.4g} particle-steps/s, rebuilds "
@@ -413,6 +431,8 @@ def main() -> int:
         achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
                "achieved_gbs": achieved, "contacts_per_step": st["contacts"],
+               "contacts_per_s": st["contacts"] * args.steps / wall,
+               "repeat_values": [n * args.steps / w for w in walls],
                "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st,
                "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None,
                "tiling": eng.tiling()}
@@ -481,7 +501,10 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
+        "repeats": len(R["repeat_values"]),
+        "repeat_values": R["repeat_values"],
         "contacts_per_step": st["contacts"],
+        "contacts_per_s": R["contacts_per_s"],
         "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
         "jkr_failures_per_step": st["jkr_failures"],

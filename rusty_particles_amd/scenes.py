@@ -140,6 +140,49 @@ def hopper_mesh(r_out: float, half_angle_deg: float, r_top: float, y0: float = 0
     return v9, n3
 
 
+def drum_mesh(radius: float = 0.25, length: float = 0.2, centre=(0.0, 0.0, 0.0), nseg: int = 128,
+              nlen: int = 49, ncap: int = 8):
+    """Closed cylindrical drum around an axis parallel to Z (horizontal: gravity
+    is -Y), normals pointing inward.  The side has nseg x nlen quads (2 triangles
+    each); each end cap has ncap rings, a fan of nseg triangles at the centre and
+    2*nseg per outer ring.  Triangles: 2*nseg*(nlen + 2*ncap - 1); the defaults
+    give the 16,384 of a 0.25 m x 0.2 m drum.  Side and caps share their rim
+    vertices, so every edge belongs to exactly two triangles."""
+    cx, cy, cz = centre
+    th = 2 * np.pi * np.arange(nseg + 1) / nseg
+    th[-1] = 0.0  # the seam closes on the same vertex
+    zs = cz - 0.5 * length + length * np.arange(nlen + 1) / nlen
+    rim = np.stack([cx + radius * np.cos(th), cy + radius * np.sin(th)], 1)
+    tris = []
+    for i in range(nlen):
+        for j in range(nseg):
+            p00 = (*rim[j], zs[i])
+            p01 = (*rim[j + 1], zs[i])
+            p10 = (*rim[j], zs[i + 1])
+            p11 = (*rim[j + 1], zs[i + 1])
+            tris.append(p00 + p10 + p11)
+            tris.append(p00 + p11 + p01)
+    rr = radius * np.arange(ncap + 1) / ncap
+    for z in (zs[0], zs[-1]):
+        ring = lambda k, j: (cx + rr[k] * np.cos(th[j]), cy + rr[k] * np.sin(th[j]), z)
+        for j in range(nseg):
+            tris.append((cx, cy, z) + ring(1, j) + ring(1, j + 1))
+        for k in range(1, ncap):
+            for j in range(nseg):
+                tris.append(ring(k, j) + ring(k + 1, j) + ring(k + 1, j + 1))
+                tris.append(ring(k, j) + ring(k + 1, j + 1) + ring(k, j + 1))
+    v9 = np.array(tris, dtype=np.float32)
+    n3 = _tri_normals(v9)
+    # inward: towards the axis on the side, towards the middle plane on the caps
+    cen = (v9[:, 0:3].astype(np.float64) + v9[:, 3:6] + v9[:, 6:9]) / 3.0
+    cap = np.abs(np.abs(cen[:, 2] - cz) - 0.5 * length) < 1e-6 * max(length, radius)
+    out = np.where(cap[:, None], np.stack([np.zeros(len(cen)), np.zeros(len(cen)), cen[:, 2] - cz], 1),
+                   np.stack([cen[:, 0] - cx, cen[:, 1] - cy, np.zeros(len(cen))], 1))
+    flip = (n3 * out).sum(1) > 0
+    n3[flip] *= -1
+    return v9, n3
+
+
 def write_stl_binary(path, v9, n3, header=b"rusty_particles_amd"):
     v9 = np.asarray(v9, dtype="<f4").reshape(-1, 9)
     n3 = np.asarray(n3, dtype="<f4").reshape(-1, 3)

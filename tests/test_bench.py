@@ -32,6 +32,9 @@ def test_json_line_contract():
         assert key in out, key
     assert out["n_gpus"] == 1 and out["steps"] == 40 and out["warmup"] == 5
     assert out["unit"] == "particle-steps/s" and out["value"] > 0 and out["higher_is_better"] is True
+    reps = out["repeat_values"]
+    assert out["repeats"] == len(reps) == 3 and sorted(reps)[1] == out["value"]
+    assert out["contacts_per_s"] >= 0 and out["contacts_per_step"] >= 0
     assert out["dtype"] == "f64" and out["data"] == "synthetic" and "workload" in out["config"]
     rf = out["roofline"]
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
