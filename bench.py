@@ -34,9 +34,18 @@ spheres, r lognormal median 50 um, surface energy ~U[0.02, 0.08] J/m^2) in
 one batched engine, once per JKR solver (exact, shortcut, surrogate); value
 = the surrogate+batched form.
 
-Timing: --steps steps per timed region, --repeats regions (default 3) back to
-back on the same run, value = the median region; contacts/s = the contacts
-of the last step times steps/s.
+--workload drum: the capacity run, 16,000,000 spheres (r ~ U[0.4, 0.8] mm,
+rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m (walls at
+rest, as the reference's walls are), JKR (surface energy 0.01 J/m^2) +
+Coulomb (mu 0.5), E=5e7, nu=0.3, e=0.5, dt=2e-6 s, 1,000 steps per timed
+region; a jittered-lattice bed filling 30 % of the drum.  Forms: shortcut,
+then surrogate (the value).  CPU baseline sample: the bed's first 20,000
+spheres in the same drum.
+
+Timing: --steps steps per timed region, --repeats regions (default 3), each
+on a fresh engine from the same initial state after the same warm-up (so
+each times the same simulated window), value = the median region;
+contacts/s = the contacts of the last step times steps/s.
 
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
@@ -86,6 +95,9 @@ def build_workload(name: str, n: int | None):
     if name == "jkr-dense":
         p, x, r, m, v = scenes.jkr_dense_workload(n or 2_000_000)
         return p, x, r, m, v, [], "JKR+Coulomb compressed FCC crystal"
+    if name == "drum":
+        p, x, r, m, meshes = scenes.drum_workload(n or 16_000_000)
+        return p, x, r, m, None, meshes, "JKR+Coulomb drum (walls at rest)"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -154,26 +166,31 @@ def run_batch(args) -> int:
             e.set_graph(False)
         return e
 
-    def timed(e, steps):
-        """Median wall time of --repeats timed regions of `steps` steps."""
-        e.step(args.warmup)
-        e.synchronize()
-        if args.autotune:
-            sk, ce = e.autotune(args.autotune)
-            log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m, tiling {e.tiling()}")
-        walls = []
+    def timed(make, steps):
+        """Median wall time of --repeats timed regions of `steps` steps, each
+        on a fresh engine from make() (the same initial state), warmed up and
+        autotuned first; returns (median, all regions, the last engine)."""
+        walls, e = [], None
         for _ in range(max(args.repeats, 1)):
+            if e is not None:
+                e.close()
+            e = make()
+            e.step(args.warmup)
+            e.synchronize()
+            if args.autotune:
+                sk, ce = e.autotune(args.autotune)
+                log(f"autotune: skin {sk:.3g} m, cell {ce:.3g} m, tiling {e.tiling()}")
             e.synchronize()
             t0 = time.perf_counter()
             e.step(steps)
             e.synchronize()
             walls.append(time.perf_counter() - t0)
-        return float(np.median(walls)), walls
+        return float(np.median(walls)), walls, e
 
     results = {}
     for form in forms:
-        e = engine(0, len(counts), form)
-        wall, walls = timed(e, args.steps)
+        wall, walls, e = timed(lambda: engine(0, len(counts), form), args.steps)
+        log(f"[{form}] {len(walls)} timed regions of {args.steps} steps: " + ", ".join(f"{w:.4f}s" for w in walls))
         value = n * args.steps / wall
         st = e.stats()
         e.set_timing(True)
@@ -209,9 +226,11 @@ def run_batch(args) -> int:
 
     graph_off = None
     if not jkr:
-        e = engine(0, len(counts), main)
-        e.set_graph(False)
-        wall_off, _ = timed(e, args.steps)
+        def no_graph_engine():
+            q = engine(0, len(counts), main)
+            q.set_graph(False)
+            return q
+        wall_off, _, e = timed(no_graph_engine, args.steps)
         graph_off = n * args.steps / wall_off
         log(f"batched, graph off: {graph_off:.4g} particle-steps/s")
         e.close()
@@ -317,7 +336,7 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, s
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=None, help="steps per timed region (5000; drum 1000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="hopper")
     ap.add_argument("--n", type=int, default=None)
@@ -332,9 +351,11 @@ def main() -> int:
                     help="batch workloads: skip the one-engine-per-instance comparison")
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--repeats", type=int, default=3,
-                    help="timed regions of --steps steps each, back to back on the same run; value = the median")
-    ap.add_argument("--jkr-forms", default="exact,shortcut,surrogate",
-                    help="JKR solvers to time on the jkr workload (the last one is the reported value)")
+                    help="timed regions of --steps steps, each on a fresh engine from the initial state; "
+                         "value = the median")
+    ap.add_argument("--jkr-forms", default=None,
+                    help="JKR solvers to time on the JKR workloads, the last one is the reported value "
+                         "(default exact,shortcut,surrogate; drum shortcut,surrogate)")
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
@@ -347,7 +368,11 @@ def main() -> int:
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.autotune is None:
-        args.autotune = 0 if args.workload in ("hopper", "box") else 20
+        args.autotune = 0 if args.workload in ("hopper", "box", "drum") else 20
+    if args.steps is None:
+        args.steps = 1000 if args.workload == "drum" else 5000
+    if args.jkr_forms is None:
+        args.jkr_forms = "shortcut,surrogate" if args.workload == "drum" else "exact,shortcut,surrogate"
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not "
                                    "supported (use --gpus 1)"}))
@@ -372,7 +397,8 @@ def main() -> int:
     surr_info = {}
     if "surrogate" in forms:
         from rusty_particles_amd import surrogate
-        rs = r[:, None] * r[None, :200] / (r[:, None] + r[None, :200])
+        rr = r[::max(1, n // 2_000_000)]  # at most ~2M radii span the envelope
+        rs = rr[:, None] * rr[None, :200] / (rr[:, None] + rr[None, :200])
         env = surrogate.envelope(-0.40, 30.0, (0.9 * rs.min(), 1.1 * rs.max()),
                                  1.0 / (2 * (1 - p.particle.poissons_ratio ** 2) / p.particle.youngs_modulus),
                                  p.particle.surface_energy)
@@ -383,38 +409,44 @@ def main() -> int:
         log(f"surrogate trained in {surr_info['train_s']:.1f}s, loss {net.epoch_loss[0]:.3g} -> "
             f"{net.epoch_loss[-1]:.3g}")
 
-    results = {}
-    eng = None
-    for form in forms:
-        if eng is not None:
-            eng.close()
-        eng = native.Engine(abi.RpParams.from_buffer_copy(p))
+    def fresh_engine(form):
+        """An engine on the initial state, warmed up (and autotuned)."""
+        e = native.Engine(abi.RpParams.from_buffer_copy(p))
         if args.no_graph:
-            eng.set_graph(False)
-        eng.upload(x, r, m, v=v)
+            e.set_graph(False)
+        e.upload(x, r, m, v=v)
         for v9, n3 in meshes:
-            eng.add_mesh(v9, n3)
+            e.add_mesh(v9, n3)
         if form != "plain":
-            eng.set_jkr_solver(form, net)
-        eng.step(args.warmup)
-        eng.synchronize()
-        log(f"[{form}] warmup {args.warmup} steps done")
+            e.set_jkr_solver(form, net)
+        e.step(args.warmup)
+        e.synchronize()
         tuned = None
         if args.autotune:
             t0 = time.perf_counter()
-            tuned = eng.autotune(args.autotune)
+            tuned = e.autotune(args.autotune)
             log(f"[{form}] autotune: skin {tuned[0]:.3g} m, cell {tuned[1]:.3g} m (0 = automatic), "
-                f"tiling {eng.tiling()}, {time.perf_counter() - t0:.1f}s")
+                f"tiling {e.tiling()}, {time.perf_counter() - t0:.1f}s")
+        return e, tuned
 
+    results = {}
+    eng = None
+    for form in forms:
         # timed regions: whole job, K steps each, one enqueue, synchronised on
-        # both sides; value = the median region
+        # both sides, each on a fresh engine from the same initial state (so
+        # every region times the same simulated window); value = the median
         walls = []
         for _ in range(max(args.repeats, 1)):
+            if eng is not None:
+                eng.close()
+            eng, tuned = fresh_engine(form)
             eng.synchronize()
             t0 = time.perf_counter()
             eng.step(args.steps)
             eng.synchronize()
             walls.append(time.perf_counter() - t0)
+        log(f"[{form}] {len(walls)} timed regions of {args.steps} steps after {args.warmup} warm-up steps: "
+            + ", ".join(f"{w:.4f}s" for w in walls))
         wall = float(np.median(walls))
         st = eng.stats()
         value = n * args.steps / wall
@@ -474,10 +506,15 @@ def main() -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cell = 2.0 * float(r.max()) * (1.5 if p.normal_model in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 1.0)
-        log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:g}s of the same {n}-particle workload")
-        cval, csec, csteps = cpu_baseline(p, x, r, m, v, meshes, args.cpu_steps, threads, cell, args.cpu_seconds)
+        # the drum's 16M spheres x 16,384 triangles (brute-force wall loop) take minutes per CPU step:
+        # its sample is the bed's first 20,000 sites (bottom layers) in the same drum
+        ns = n if args.workload != "drum" else min(n, 20_000)
+        what = f"the full {n}-particle workload" if ns == n else f"the first {ns} of the {n} particles"
+        log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:g}s of {what}")
+        cval, csec, csteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
+                                          args.cpu_steps, threads, cell, args.cpu_seconds)
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{csteps} step(s) of the full {n}-particle workload after 1 warm-up step, "
+               "sample": f"{csteps} step(s) of {what} after 1 warm-up step, "
                          f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff), brute-force "
                          f"{ntri}-triangle wall loop as in simulation.rs:465-523; {csec:.1f}s"}
         log(f"cpu baseline {cval:.4g} particle-steps/s ({csec:.1f}s)")

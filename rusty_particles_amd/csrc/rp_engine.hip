@@ -2425,7 +2425,11 @@ rp_status capture_steps(rp_engine* e, hipGraphExec_t* exec) {
 rp_status replay_steps(rp_engine* e) {
   const int slot = (int)(e->chain_no & 1);
   if (!e->graph_exec[slot] || e->graph_gen[slot] != e->gen) {
-    if (e->graph_exec[slot]) (void)hipGraphExecDestroy(e->graph_exec[slot]);
+    if (e->graph_exec[slot]) {
+      // an earlier replay of this executable may still run on the stream
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      (void)hipGraphExecDestroy(e->graph_exec[slot]);
+    }
     e->graph_exec[slot] = nullptr;
     rp_status st = capture_steps(e, &e->graph_exec[slot]);
     if (st) return st;

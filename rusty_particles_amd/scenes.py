@@ -318,6 +318,52 @@ def jkr_dense_workload(n: int = 2_000_000, seed: int = 0xD3E50007, spread: float
     return p, x, r, m, v
 
 
+def drum_workload(n: int = 16_000_000, seed: int = 0xD3E50005, fill: float = 0.30, radius: float = 0.25):
+    """Capacity run: n spheres (r ~ U[0.4, 0.8] mm, rho 2500) in the lower part
+    of a closed drum (radius 0.25 m, axis along Z, 16,384-triangle STL), JKR
+    normal (surface energy 0.01 J/m^2) + Coulomb (mu 0.5), E=5e7, nu=0.3,
+    e=0.5, dt=2e-6 s, gravity -Y, walls at rest.
+
+    The bed starts as a jittered cubic lattice (spacing 2.02 r_max, jitter
+    +-4 um, so no two spheres touch), filled layer by layer from the bottom
+    of the drum, at least r_max + half a spacing from the wall.  The lattice
+    holds solid fraction ~0.24, so a 0.2 m drum cannot take 16M spheres at
+    a 30 % fill; the drum's length is set so that the bed's lattice volume is
+    `fill` of the drum's (1.15 m at 16M).  Returns (params, x, r, m, meshes)."""
+    rng = np.random.default_rng(seed)
+    rmin, rmax = 0.4e-3, 0.8e-3
+    s = 2.02 * rmax
+    length = n * s ** 3 / (fill * np.pi * radius * radius)
+    margin = rmax + 0.5 * s
+    xs = np.arange(-radius + margin, radius - margin + 1e-12, s)
+    zs = np.arange(-0.5 * length + margin, 0.5 * length - margin + 1e-12, s)
+    pieces, count = [], 0
+    y = -radius + margin
+    while count < n:
+        if y > radius - margin:
+            raise RuntimeError(f"drum holds only {count} of {n} lattice sites")
+        xk = xs[xs * xs + y * y <= (radius - margin) ** 2]
+        if xk.size:
+            X, Z = np.meshgrid(xk, zs, indexing="xy")
+            layer = np.stack([X.ravel(), np.full(X.size, y), Z.ravel()], 1)
+            take = min(layer.shape[0], n - count)
+            pieces.append(layer[:take])
+            count += take
+        y += s
+    x = np.concatenate(pieces)
+    x += rng.uniform(-4e-6, 4e-6, x.shape)
+    r = rng.uniform(rmin, rmax, n)
+    m = sphere_mass(r, 2500.0)
+    v9, n3 = drum_mesh(radius, length)
+    pad = 0.005
+    p = abi.default_params(2e-6, (-radius - pad, -radius - pad, -0.5 * length - pad),
+                           (radius + pad, radius + pad, 0.5 * length + pad))
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(5e7, 0.3, 2500.0, 0.5, 0.5, 0.01)
+    return p, x, r, m, [(v9, n3)]
+
+
 def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int = 2000, nmax: int = 50000):
     """Ensemble of `ninst` independent simulations for a material sweep:
     N_k ~ U{nmin..nmax} spheres, r ~ U[1, 3] mm, rho 2500, per-instance
