@@ -175,6 +175,18 @@ rp_status rp_engine_add_mesh(rp_engine* e, size_t ntri, const float* v9,
  * src/lib.rs:87-91). */
 rp_status rp_engine_add_mesh_stl(rp_engine* e, const char* path);
 
+/* Surface motion of mesh `mesh` (0-based, in the order added): a rigid
+ * rotation with angular velocity omega[3] (rad/s) about the point origin[3].
+ * The triangles stay where they are; a wall contact's relative velocity
+ * subtracts omega x (c - origin) at the contact point c (the closest point on
+ * the triangle).  Exact for surfaces of revolution about that axis (drums,
+ * cylinders, cones), which the rotation maps onto themselves.  omega = 0 makes
+ * the mesh static again.  At most 4 meshes move at once; RP_ERR_INVALID for a
+ * fifth, an unknown mesh or non-finite values (the previous motion is kept).
+ * No counterpart in the reference, whose walls are at rest
+ * (src/simulation.rs:475-479): this is for rotating drums. */
+rp_status rp_engine_set_mesh_motion(rp_engine* e, int32_t mesh, const double* origin, const double* omega);
+
 /* Replaces GpuSimulation::step (src/gpu.rs:153-166): advances nsteps time
  * steps of Simulation::step_cpu semantics (src/simulation.rs:173-562).
  * Enqueues work on the engine's stream and returns without a host sync per

@@ -21,6 +21,7 @@
 
 #include <omp.h>
 
+#include <array>
 #include <cstdio>
 #include <memory>
 
@@ -121,6 +122,7 @@ class SimBase {
                          double* rw) const = 0;
   virtual size_t count() const = 0;
   virtual void add_mesh(size_t ntri, const float* v9, const float* n3) = 0;
+  virtual int set_mesh_motion(int mesh, const double* origin, const double* omega) = 0;
   virtual void step(uint64_t nsteps) = 0;
   virtual void set_cell_size(double cs) = 0;
   virtual void set_record(int on) = 0;
@@ -200,7 +202,21 @@ class Sim : public SimBase {
       tr.v2 = v3<R>(R(v9[9 * t + 6]), R(v9[9 * t + 7]), R(v9[9 * t + 8]));
       tr.n = v3<R>(R(n3[3 * t + 0]), R(n3[3 * t + 1]), R(n3[3 * t + 2]));
       tris_.push_back(tr);
+      tri_mesh_.push_back((int)motion_.size());
     }
+    motion_.push_back({0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+  }
+
+  // Extension, no reference counterpart (its walls are at rest,
+  // simulation.rs:475-479): mesh `mesh` has the surface velocity of the
+  // rigid rotation omega about origin; the triangles stay where they are.
+  int set_mesh_motion(int mesh, const double* origin, const double* omega) override {
+    if (mesh < 0 || (size_t)mesh >= motion_.size()) return -1;
+    for (int k = 0; k < 3; ++k) {
+      motion_[mesh][k] = origin[k];
+      motion_[mesh][3 + k] = omega[k];
+    }
+    return 0;
   }
 
   void set_record(int on) override { record_ = on != 0; }
@@ -454,13 +470,21 @@ class Sim : public SimBase {
           if (record_) pairs_[i].push_back(j);
         }
         // walls (simulation.rs:465-523)
-        for (const Triangle<R>& tr : tris_) {
+        for (size_t ti = 0; ti < tris_.size(); ++ti) {
+          const Triangle<R>& tr = tris_[ti];
           R pen;
           V3<R> nrm;
           if (!intersect_sphere(tr, p.x, p.r, &pen, &nrm)) continue;
           if (!flag && pen > R(0.2) * p.r) flag = 1;
           V3<R> ra = (-nrm) * p.r;
           V3<R> vrel = p.v + cross(p.w, ra);
+          const std::array<double, 6>& mo = motion_[tri_mesh_[ti]];
+          if (mo[3] != 0.0 || mo[4] != 0.0 || mo[5] != 0.0) {
+            // a moving wall's surface velocity at the contact (closest) point
+            const V3<R> cp = closest_point(tr, p.x);
+            const V3<R> org = v3<R>(R(mo[0]), R(mo[1]), R(mo[2]));
+            vrel = vrel - cross(v3<R>(R(mo[3]), R(mo[4]), R(mo[5])), cp - org);
+          }
           V3<R> fn = normal_force(pen, nrm, vrel, estar_pw, p.r, p.m, gamma_pw, true, &fails);
           R fnmag = length(fn);
           V3<R> ft = coulomb_friction(fnmag, vrel, nrm, pm.mu);
@@ -499,6 +523,8 @@ class Sim : public SimBase {
   Grid<R> grid_;
   std::vector<Particle<R>> ps_;
   std::vector<Triangle<R>> tris_;
+  std::vector<int> tri_mesh_;                  // mesh of each triangle
+  std::vector<std::array<double, 6>> motion_;  // per mesh: origin, omega
   std::vector<V3<R>> ftot_, ttot_;
   std::vector<R> fscale_;
   std::vector<std::vector<int64_t>> pairs_;
@@ -533,6 +559,9 @@ void rpo_get_state(void* s, double* x, double* v, double* w, double* a, double* 
 size_t rpo_count(void* s) { return static_cast<SimBase*>(s)->count(); }
 void rpo_add_mesh(void* s, size_t ntri, const float* v9, const float* n3) {
   static_cast<SimBase*>(s)->add_mesh(ntri, v9, n3);
+}
+int rpo_set_mesh_motion(void* s, int mesh, const double* origin, const double* omega) {
+  return static_cast<SimBase*>(s)->set_mesh_motion(mesh, origin, omega);
 }
 void rpo_step(void* s, uint64_t nsteps) { static_cast<SimBase*>(s)->step(nsteps); }
 void rpo_set_record(void* s, int on) { static_cast<SimBase*>(s)->set_record(on); }

@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
     L.rpo_count.argtypes = [C.c_void_p]
     L.rpo_count.restype = C.c_size_t
     L.rpo_add_mesh.argtypes = [C.c_void_p, C.c_size_t, _fp, _fp]
+    L.rpo_set_mesh_motion.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.rpo_set_mesh_motion.restype = C.c_int
     L.rpo_step.argtypes = [C.c_void_p, C.c_uint64]
     L.rpo_set_record.argtypes = [C.c_void_p, C.c_int]
     L.rpo_pair_total.argtypes = [C.c_void_p]
@@ -171,6 +173,14 @@ class OracleSimulation:
         v9 = np.ascontiguousarray(np.asarray(v9, dtype=np.float32).reshape(-1, 9))
         n3 = np.ascontiguousarray(np.asarray(n3, dtype=np.float32).reshape(-1, 3))
         self._L.rpo_add_mesh(self._h, v9.shape[0], _p(v9, C.c_float), _p(n3, C.c_float))
+
+    def set_mesh_motion(self, mesh, origin, omega):
+        """Surface velocity of mesh `mesh`: the rigid rotation omega about origin
+        (an extension; the reference's walls are at rest)."""
+        o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+        w = np.ascontiguousarray(omega, dtype=np.float64).reshape(3)
+        if self._L.rpo_set_mesh_motion(self._h, int(mesh), _p(o, C.c_double), _p(w, C.c_double)) != 0:
+            raise IndexError("no such mesh")
 
     # -- config
     def _push(self):

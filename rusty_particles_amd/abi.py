@@ -144,6 +144,7 @@ EXPORTS = (
     "rp_engine_upload_full",
     "rp_engine_add_mesh",
     "rp_engine_add_mesh_stl",
+    "rp_engine_set_mesh_motion",
     "rp_engine_step",
     "rp_engine_synchronize",
     "rp_engine_count",

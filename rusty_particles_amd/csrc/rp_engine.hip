@@ -170,6 +170,9 @@ struct BlockStats {
 };
 
 // Everything a kernel needs that is fixed between parameter changes.
+// Meshes whose surface moves (rp_engine_set_mesh_motion) at one time.
+constexpr int kMaxMoving = 4;
+
 struct KParams {
   double dt;
   double bmin[3], bmax[3], bsize[3];
@@ -208,6 +211,12 @@ struct KParams {
   // (below the cut-off -sqrt(3 pi^2 gamma^2 R* / E*^2) of any R* up to r_max / 2;
   // 0 when gamma <= 0, where only overlap > 0 has force)
   double sjkr_zero_pp;
+  // moving walls: triangles [mov_lo[k], mov_hi[k]) have the surface velocity
+  // of a rigid rotation mov_w[k] (rad/s) about the point mov_o[k]; the
+  // geometry itself stays where it is
+  int nmov;
+  unsigned int mov_lo[kMaxMoving], mov_hi[kMaxMoving];
+  double mov_o[kMaxMoving][3], mov_w[kMaxMoving][3];
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
   int pair_cap;      // recorded pair entries per particle
@@ -994,10 +1003,23 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
   if (RECORD) record_pairs(P, T, j, mask0, mask1, mask2, acc, ctl, prow);
 }
 
-// One particle-triangle contact (simulation.rs:465-523).
+// Surface velocity of triangle ti at the point c: omega x (c - origin) of
+// its mesh's rotation, 0 for a mesh at rest.
+__device__ inline dv3 wall_velocity(const KParams& P, unsigned int ti, dv3 c) {
+  for (int k = 0; k < P.nmov; ++k)
+    if (ti >= P.mov_lo[k] && ti < P.mov_hi[k])
+      return cross(mk(P.mov_w[k][0], P.mov_w[k][1], P.mov_w[k][2]),
+                   sub(c, mk(P.mov_o[k][0], P.mov_o[k][1], P.mov_o[k][2])));
+  return mk(0.0, 0.0, 0.0);
+}
+
+// One particle-triangle contact (simulation.rs:465-523).  A moving wall's
+// surface velocity at the contact point (the closest point) is subtracted
+// from the relative velocity; the reference's walls are at rest.
 template <int NORMAL, int JS>
 __device__ inline void contact_wall(const KParams& P, const MaterialConsts& mc, double far_pw, const Own& o,
-                                    const Tri& tr, Acc& acc, unsigned int& nwall, const SurrDev& sd) {
+                                    const Tri& tr, unsigned int ti, Acc& acc, unsigned int& nwall,
+                                    const SurrDev& sd) {
   const dv3 cp = closest_point(tr, o.x);
   const dv3 dv = sub(o.x, cp);
   const double dsq = dot(dv, dv);
@@ -1007,7 +1029,8 @@ __device__ inline void contact_wall(const KParams& P, const MaterialConsts& mc, 
   const dv3 nrm = dist > 1e-6 ? divs(dv, dist) : tr.n;
   if (pen > 0.2 * o.r) acc.overlap = true;
   const dv3 ra = mul(neg(nrm), o.r);
-  const dv3 vrel = add(o.v, cross(o.w, ra));
+  dv3 vrel = add(o.v, cross(o.w, ra));
+  if (P.nmov) vrel = sub(vrel, wall_velocity(P, ti, cp));
   dv3 fn;
   int jfail = 0;
   if (NORMAL == RP_NORMAL_HERTZ) {
@@ -1349,7 +1372,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
       for (uint32_t q = t0; q < t1; ++q)
-        contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], acc, nwall, sd);
+        contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], tri_idx[q], acc, nwall, sd);
 
 
 
@@ -1606,6 +1629,8 @@ __global__ void k_request_rebuild(Ctl* ctl, int par) {
 
 struct HostMesh {
   std::vector<float> v9, n3;
+  double origin[3] = {0.0, 0.0, 0.0};  // surface motion: rotation omega about origin
+  double omega[3] = {0.0, 0.0, 0.0};
 };
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
@@ -1907,8 +1932,34 @@ rp_status compute_geometry(rp_engine* e) {
   return RP_OK;
 }
 
+// The moving-wall table of the kernel parameters from the meshes' motions
+// (meshes are concatenated in the order added).
+rp_status fill_motion(rp_engine* e) {
+  KParams& K = e->kp;
+  int nmov = 0;
+  size_t first = 0;
+  for (const HostMesh& m : e->meshes) {
+    const size_t nt = m.n3.size() / 3;
+    if (m.omega[0] != 0.0 || m.omega[1] != 0.0 || m.omega[2] != 0.0) {
+      if (nmov == kMaxMoving) return fail(RP_ERR_INVALID, "more than 4 meshes move");
+      K.mov_lo[nmov] = (unsigned int)first;
+      K.mov_hi[nmov] = (unsigned int)(first + nt);
+      for (int k = 0; k < 3; ++k) {
+        K.mov_o[nmov][k] = m.origin[k];
+        K.mov_w[nmov][k] = m.omega[k];
+      }
+      ++nmov;
+    }
+    first += nt;
+  }
+  K.nmov = nmov;
+  e->gen++;  // captured step loops hold the kernel parameters
+  return RP_OK;
+}
+
 rp_status upload_meshes_binned(rp_engine* e) {
   e->gen++;
+  if (rp_status st = fill_motion(e)) return st;
   KParams& K = e->kp;
   std::vector<Tri> tris;
   for (const HostMesh& m : e->meshes) {
@@ -2697,6 +2748,27 @@ rp_status rp_engine_add_mesh(rp_engine* e, size_t ntri, const float* v9, const f
   m.n3.assign(n3, n3 + ntri * 3);
   e->meshes.push_back(std::move(m));
   e->geometry_dirty = true;
+  return RP_OK;
+}
+
+rp_status rp_engine_set_mesh_motion(rp_engine* e, int32_t mesh, const double* origin, const double* omega) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (!origin || !omega) return fail(RP_ERR_INVALID, "origin and omega are required");
+  if (mesh < 0 || (size_t)mesh >= e->meshes.size()) return fail(RP_ERR_INVALID, "no such mesh");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(origin[k]) || !std::isfinite(omega[k]))
+      return fail(RP_ERR_INVALID, "origin and omega must be finite");
+  HostMesh& m = e->meshes[(size_t)mesh];
+  const HostMesh old = m;
+  for (int k = 0; k < 3; ++k) {
+    m.origin[k] = origin[k];
+    m.omega[k] = omega[k];
+  }
+  if (rp_status st = fill_motion(e)) {
+    m = old;
+    (void)fill_motion(e);
+    return st;
+  }
   return RP_OK;
 }
 

@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
                                         _u8p]
     L.rp_engine_add_mesh.argtypes = [_eng, C.c_size_t, _fp, _fp]
     L.rp_engine_add_mesh_stl.argtypes = [_eng, C.c_char_p]
+    L.rp_engine_set_mesh_motion.argtypes = [_eng, C.c_int32, _dp, _dp]
     L.rp_engine_step.argtypes = [_eng, C.c_uint64]
     L.rp_engine_synchronize.argtypes = [_eng]
     L.rp_engine_count.argtypes = [_eng, C.POINTER(C.c_size_t)]
@@ -71,7 +72,7 @@ def lib() -> C.CDLL:
     L.rp_last_error.argtypes = []
     L.rp_last_error.restype = C.c_char_p
     for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
-                 "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_step", "rp_engine_synchronize",
+                 "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_set_mesh_motion", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
                  "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
@@ -200,6 +201,13 @@ class Engine:
         v9 = np.ascontiguousarray(np.asarray(v9, dtype=np.float32).reshape(-1, 9))
         n3 = np.ascontiguousarray(np.asarray(n3, dtype=np.float32).reshape(-1, 3))
         _check(self._L.rp_engine_add_mesh(self._h, v9.shape[0], _ptr(v9, C.c_float), _ptr(n3, C.c_float)))
+
+    def set_mesh_motion(self, mesh: int, origin, omega) -> None:
+        """Surface velocity of mesh `mesh` (0-based, in the order added): the
+        rigid rotation omega (rad/s) about origin; the triangles stay put."""
+        o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+        w = np.ascontiguousarray(omega, dtype=np.float64).reshape(3)
+        _check(self._L.rp_engine_set_mesh_motion(self._h, int(mesh), _ptr(o, C.c_double), _ptr(w, C.c_double)))
 
     def step(self, nsteps: int = 1) -> None:
         _check(self._L.rp_engine_step(self._h, int(nsteps)))

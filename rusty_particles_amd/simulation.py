@@ -61,6 +61,7 @@ class Simulation:
         self._engine_valid = False   # engine holds the latest state
         self._params_dirty = True
         self._meshes: list[tuple[np.ndarray, np.ndarray]] = []
+        self._motions: dict[int, tuple[tuple, tuple]] = {}  # mesh -> (origin, omega)
         self._engine: native.Engine | None = None
         self._record = False
         self._jkr = ("exact", None, False)  # solver, surrogate, check
@@ -108,6 +109,23 @@ class Simulation:
         self._meshes.append((v9, n3))
         if self._engine is not None:
             self._engine.add_mesh(v9, n3)
+
+    def set_mesh_motion(self, index, origin, angular_velocity):
+        """Extension (the reference's walls are at rest, src/simulation.rs:475-479):
+        mesh `index` (0-based, in the order added) moves as the rigid rotation
+        `angular_velocity` (rad/s) about the point `origin`.  Only the surface
+        velocity enters the wall contacts; the triangles stay where they are, which
+        is exact for surfaces of revolution about the axis (a rotating drum)."""
+        index = int(index)
+        if not 0 <= index < len(self._meshes):
+            raise IndexError("mesh index out of range")
+        o = tuple(float(c) for c in origin)
+        w = tuple(float(c) for c in angular_velocity)
+        if len(o) != 3 or len(w) != 3:
+            raise ValueError("origin and angular_velocity are 3-vectors")
+        if self._engine is not None:
+            self._engine.set_mesh_motion(index, o, w)
+        self._motions[index] = (o, w)
 
     # ---- stepping (src/lib.rs:94-96, 182-195) --------------------------------------
     def step(self):
@@ -306,6 +324,8 @@ class Simulation:
             self._engine_valid = False
             for v9, n3 in self._meshes:
                 eng.add_mesh(v9, n3)
+            for k, (o, w) in self._motions.items():
+                eng.set_mesh_motion(k, o, w)
             eng.set_record(self._record)
             if self._jkr[0] != "exact":
                 eng.set_jkr_solver(*self._jkr[:2], check=self._jkr[2])
@@ -344,6 +364,8 @@ class Ensemble:
             if len(s._meshes) != len(m0) or any(not (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]))
                                                 for a, b in zip(s._meshes, m0)):
                 raise ValueError("ensemble members must share their meshes")
+            if s._motions != self.sims[0]._motions:
+                raise ValueError("ensemble members must share their meshes' motions")
         self._engine: native.Engine | None = None
         self._dt32 = self.sims[0]._dt32
 
@@ -371,6 +393,8 @@ class Ensemble:
                              wall_materials=[s._params.wall for s in self.sims])
             for v9, n3 in self.sims[0]._meshes:
                 eng.add_mesh(v9, n3)
+            for k, (o, w) in self.sims[0]._motions.items():
+                eng.set_mesh_motion(k, o, w)
             if self.sims[0]._jkr[0] == "shortcut":
                 eng.set_jkr_solver("shortcut")
             self._engine = eng

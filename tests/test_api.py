@@ -113,3 +113,20 @@ def test_jkr_solver_names():
         s.set_jkr_solver("newton")
     with pytest.raises(ValueError):
         s.set_jkr_solver("surrogate")  # needs a trained network
+
+
+def test_set_mesh_motion_checks_its_arguments():
+    """Extension for rotating drums (the reference's walls are at rest)."""
+    from rusty_particles_amd import scenes
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    with pytest.raises(IndexError):
+        sim.set_mesh_motion(0, (0, 0, 0), (0, 0, 1))
+    sim.add_mesh_arrays(*scenes.box_mesh((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)))
+    sim.set_mesh_motion(0, (0, 0, 0), (0, 0, 1))
+    with pytest.raises(ValueError):
+        sim.set_mesh_motion(0, (0, 0), (0, 0, 1))
+    with pytest.raises(IndexError):
+        sim.set_mesh_motion(1, (0, 0, 0), (0, 0, 1))
+    lib = native.lib()
+    o = (native.C.c_double * 3)()
+    assert lib.rp_engine_set_mesh_motion(None, 0, o, o) == rp.abi.RP_ERR_INVALID
