@@ -35,8 +35,8 @@ one batched engine, once per JKR solver (exact, shortcut, surrogate); value
 = the surrogate+batched form.
 
 --workload drum: the capacity run, 16,000,000 spheres (r ~ U[0.4, 0.8] mm,
-rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m (walls at
-rest, as the reference's walls are), JKR (surface energy 0.01 J/m^2) +
+rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m turning at
+30 rpm (the surface velocity of the moving-wall extension), JKR (surface energy 0.01 J/m^2) +
 Coulomb (mu 0.5), E=5e7, nu=0.3, e=0.5, dt=2e-6 s, 1,000 steps per timed
 region; a jittered-lattice bed filling 30 % of the drum.  Forms: shortcut,
 then surrogate (the value).  CPU baseline sample: the bed's first 20,000
@@ -97,7 +97,7 @@ def build_workload(name: str, n: int | None):
         return p, x, r, m, v, [], "JKR+Coulomb compressed FCC crystal"
     if name == "drum":
         p, x, r, m, meshes = scenes.drum_workload(n or 16_000_000)
-        return p, x, r, m, None, meshes, "JKR+Coulomb drum (walls at rest)"
+        return p, x, r, m, None, meshes, "JKR+Coulomb drum turning at 30 rpm"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -304,7 +304,14 @@ def run_batch(args) -> int:
     return 0
 
 
-def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0):
+def mesh_motions(workload: str):
+    """(mesh, origin, omega) of the workload's moving meshes."""
+    from rusty_particles_amd import scenes
+    return [(0, scenes.DRUM_ORIGIN, scenes.DRUM_OMEGA)] if workload == "drum" else []
+
+
+def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0,
+                 motions=()):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
     largest contact cutoff (the cell-size heuristic of simulation.rs:64).
     Times at least `steps` steps, more when the warm-up step says that
@@ -320,6 +327,8 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, s
     o.set_state(x, r, m, v=v)
     for v9, n3 in meshes:
         o.add_mesh_arrays(v9, n3)
+    for k, org, om in motions:
+        o.set_mesh_motion(k, org, om)
     o.set_cell_size(cell)
     o.set_record(False)
     t0 = time.perf_counter()
@@ -417,6 +426,8 @@ def main() -> int:
         e.upload(x, r, m, v=v)
         for v9, n3 in meshes:
             e.add_mesh(v9, n3)
+        for k, org, om in mesh_motions(args.workload):
+            e.set_mesh_motion(k, org, om)
         if form != "plain":
             e.set_jkr_solver(form, net)
         e.step(args.warmup)
@@ -512,7 +523,8 @@ def main() -> int:
         what = f"the full {n}-particle workload" if ns == n else f"the first {ns} of the {n} particles"
         log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:g}s of {what}")
         cval, csec, csteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
-                                          args.cpu_steps, threads, cell, args.cpu_seconds)
+                                          args.cpu_steps, threads, cell, args.cpu_seconds,
+                                          mesh_motions(args.workload))
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{csteps} step(s) of {what} after 1 warm-up step, "
                          f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff), brute-force "

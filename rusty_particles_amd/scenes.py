@@ -318,11 +318,17 @@ def jkr_dense_workload(n: int = 2_000_000, seed: int = 0xD3E50007, spread: float
     return p, x, r, m, v
 
 
+# The capacity run's drum turns at 30 rpm about its axis (Z through the origin).
+DRUM_ORIGIN = (0.0, 0.0, 0.0)
+DRUM_OMEGA = (0.0, 0.0, math.pi)
+
+
 def drum_workload(n: int = 16_000_000, seed: int = 0xD3E50005, fill: float = 0.30, radius: float = 0.25):
     """Capacity run: n spheres (r ~ U[0.4, 0.8] mm, rho 2500) in the lower part
     of a closed drum (radius 0.25 m, axis along Z, 16,384-triangle STL), JKR
     normal (surface energy 0.01 J/m^2) + Coulomb (mu 0.5), E=5e7, nu=0.3,
-    e=0.5, dt=2e-6 s, gravity -Y, walls at rest.
+    e=0.5, dt=2e-6 s, gravity -Y; the drum's surface turns at 30 rpm
+    (DRUM_OMEGA about DRUM_ORIGIN, applied with set_mesh_motion on mesh 0).
 
     The bed starts as a jittered cubic lattice (spacing 2.02 r_max, jitter
     +-4 um, so no two spheres touch), filled layer by layer from the bottom

@@ -23,26 +23,30 @@ def _instances(k, seed):
     return inst
 
 
-def _single(p, sc, mat, meshes, steps, solver=None):
+def _single(p, sc, mat, meshes, steps, solver=None, motion=None):
     q = copy_params(p)
     q.particle = mat
     e = native.Engine(q)
     e.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
     for v9, n3 in meshes:
         e.add_mesh(v9, n3)
+    if motion:
+        e.set_mesh_motion(0, *motion)
     if solver:
         e.set_jkr_solver(solver)
     e.step(steps)
     return e.download()
 
 
-def _batched(p, inst, meshes, steps, solver=None, chunks=None):
+def _batched(p, inst, meshes, steps, solver=None, chunks=None, motion=None):
     e = native.Engine(copy_params(p))
     cat = lambda key: np.concatenate([sc[key] for sc, _ in inst])
     e.upload_batch([sc["x"].shape[0] for sc, _ in inst], [m for _, m in inst], cat("x"), cat("radius"),
                    cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"))
     for v9, n3 in meshes:
         e.add_mesh(v9, n3)
+    if motion:
+        e.set_mesh_motion(0, *motion)
     if solver:
         e.set_jkr_solver(solver)
     for c in (chunks or [steps]):
@@ -69,6 +73,27 @@ def test_instances_match_single_runs(normal, tangential, solver):
         n = ref["x"].shape[0]
         assert counts[q] == n
         for key in ("x", "v", "w", "a", "tau", "rx", "rv", "rw", "radius", "mass"):
+            assert np.array_equal(got[key][off:off + n], ref[key]), (q, key)
+        off += n
+    assert off == got["x"].shape[0]
+
+
+def test_instances_match_single_runs_with_a_moving_wall():
+    """The box's surface moves (the moving-wall extension); wall contacts
+    occur in several instances, and each instance still evolves as it does
+    alone, bit for bit."""
+    inst = _instances(7, seed=77)
+    p = params(1e-5, (-0.06, -0.06, -0.06), (0.06, 0.06, 0.06), normal=abi.RP_NORMAL_SJKR,
+               tangential=abi.RP_TANGENTIAL_COULOMB)
+    meshes = [scenes.box_mesh((-0.05, -0.05, -0.05), (0.05, 0.05, 0.05))]
+    motion = ((0.01, 0.0, -0.02), (0.3, 1.0, -0.5))
+    eng, got = _batched(p, inst, meshes, 60, chunks=[25, 35], motion=motion)
+    assert eng.stats()["wall_contacts"] > 0
+    off = 0
+    for q, (sc, mat) in enumerate(inst):
+        ref = _single(p, sc, mat, meshes, 60, motion=motion)
+        n = ref["x"].shape[0]
+        for key in ("x", "v", "w", "a", "tau", "rx", "rv", "rw"):
             assert np.array_equal(got[key][off:off + n], ref[key]), (q, key)
         off += n
     assert off == got["x"].shape[0]
