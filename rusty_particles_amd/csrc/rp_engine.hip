@@ -173,6 +173,14 @@ struct BlockStats {
 // Meshes whose surface moves (rp_engine_set_mesh_motion) at one time.
 constexpr int kMaxMoving = 4;
 
+// Surface motion of triangles [lo, hi): the rigid rotation w (rad/s) about
+// the point o.  A table of these lives in device memory, read only by wall
+// contacts.
+struct WallMotion {
+  unsigned int lo, hi;
+  double o[3], w[3];
+};
+
 struct KParams {
   double dt;
   double bmin[3], bmax[3], bsize[3];
@@ -211,12 +219,10 @@ struct KParams {
   // (below the cut-off -sqrt(3 pi^2 gamma^2 R* / E*^2) of any R* up to r_max / 2;
   // 0 when gamma <= 0, where only overlap > 0 has force)
   double sjkr_zero_pp;
-  // moving walls: triangles [mov_lo[k], mov_hi[k]) have the surface velocity
-  // of a rigid rotation mov_w[k] (rad/s) about the point mov_o[k]; the
-  // geometry itself stays where it is
+  // moving walls: nmov entries of mov (device memory); the geometry itself
+  // stays where it is
   int nmov;
-  unsigned int mov_lo[kMaxMoving], mov_hi[kMaxMoving];
-  double mov_o[kMaxMoving][3], mov_w[kMaxMoving][3];
+  const WallMotion* mov;
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
   int pair_cap;      // recorded pair entries per particle
@@ -1006,10 +1012,10 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
 // Surface velocity of triangle ti at the point c: omega x (c - origin) of
 // its mesh's rotation, 0 for a mesh at rest.
 __device__ inline dv3 wall_velocity(const KParams& P, unsigned int ti, dv3 c) {
-  for (int k = 0; k < P.nmov; ++k)
-    if (ti >= P.mov_lo[k] && ti < P.mov_hi[k])
-      return cross(mk(P.mov_w[k][0], P.mov_w[k][1], P.mov_w[k][2]),
-                   sub(c, mk(P.mov_o[k][0], P.mov_o[k][1], P.mov_o[k][2])));
+  for (int k = 0; k < P.nmov; ++k) {
+    const WallMotion& m = P.mov[k];
+    if (ti >= m.lo && ti < m.hi) return cross(mk(m.w[0], m.w[1], m.w[2]), sub(c, mk(m.o[0], m.o[1], m.o[2])));
+  }
   return mk(0.0, 0.0, 0.0);
 }
 
@@ -1712,6 +1718,7 @@ struct rp_engine {
   std::vector<rp_material> inst_particle, inst_wall;
   std::vector<size_t> inst_count;  // uploaded particles per instance
   InstConsts* d_itab = nullptr;
+  WallMotion* d_mov = nullptr;  // kMaxMoving entries, allocated by the first motion
   // JKR solver (RP_JKR_*) and the surrogate's network and buffers
   int jkr_solver = RP_JKR_EXACT;
   int surr_check = 0;
@@ -1936,23 +1943,30 @@ rp_status compute_geometry(rp_engine* e) {
 // (meshes are concatenated in the order added).
 rp_status fill_motion(rp_engine* e) {
   KParams& K = e->kp;
+  WallMotion tab[kMaxMoving] = {};
   int nmov = 0;
   size_t first = 0;
   for (const HostMesh& m : e->meshes) {
     const size_t nt = m.n3.size() / 3;
     if (m.omega[0] != 0.0 || m.omega[1] != 0.0 || m.omega[2] != 0.0) {
       if (nmov == kMaxMoving) return fail(RP_ERR_INVALID, "more than 4 meshes move");
-      K.mov_lo[nmov] = (unsigned int)first;
-      K.mov_hi[nmov] = (unsigned int)(first + nt);
+      tab[nmov].lo = (unsigned int)first;
+      tab[nmov].hi = (unsigned int)(first + nt);
       for (int k = 0; k < 3; ++k) {
-        K.mov_o[nmov][k] = m.origin[k];
-        K.mov_w[nmov][k] = m.omega[k];
+        tab[nmov].o[k] = m.origin[k];
+        tab[nmov].w[k] = m.omega[k];
       }
       ++nmov;
     }
     first += nt;
   }
+  if (nmov > 0) {
+    if (!e->d_mov) HIP_TRY(hipMalloc(&e->d_mov, sizeof(tab)));
+    HIP_TRY(hipStreamSynchronize(e->stream));  // queued steps may still read the old table
+    HIP_TRY(hipMemcpy(e->d_mov, tab, sizeof(tab), hipMemcpyHostToDevice));
+  }
   K.nmov = nmov;
+  K.mov = e->d_mov;
   e->gen++;  // captured step loops hold the kernel parameters
   return RP_OK;
 }
@@ -3373,6 +3387,7 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_tri_idx);
   free_ptr(e->d_ctl);
   free_ptr(e->d_itab);
+  free_ptr(e->d_mov);
   free_ptr(e->d_surr_inst);
   free_ptr(e->d_surr_counters);
   free_ptr(e->d_surr_x);
