@@ -81,3 +81,31 @@ def test_turning_wall_drags_and_zero_omega_is_static():
         assert np.array_equal(runs[None][k], runs[0.0][k])
     l0 = _lz(runs[None])
     assert _lz(runs[10.0]) > l0 and _lz(runs[-10.0]) < l0
+
+
+
+def test_simulation_api_applies_mesh_motion():
+    """Simulation.set_mesh_motion before the engine exists (it is created at
+    the first step) gives, bit for bit, the run of the engine-level call."""
+    import rusty_particles_amd as rp
+    p, sc, meshes = _setup(abi.RP_NORMAL_HERTZ, 0.0)
+    omega = (0.0, 0.0, 10.0)
+    sim = rp.Simulation(p.dt, tuple(p.bounds_min), tuple(p.bounds_max))
+    sim.set_contact_models("hertz", "coulomb")
+    m = p.particle
+    sim.set_particle_material(m.youngs_modulus, m.poissons_ratio, m.density, m.friction_coefficient,
+                              m.restitution_coefficient, m.surface_energy)
+    sim.add_particles(sc["x"], sc["radius"], sc["mass"])
+    for k in range(sc["x"].shape[0]):
+        sim.set_particle_velocity(k, *sc["v"][k])
+    sim.add_mesh_arrays(*meshes[0])
+    sim.set_mesh_motion(0, (0.0, 0.0, 0.0), omega)
+    for _ in range(200):
+        sim.step()
+    got = sim.state()
+    e = run_engine(sim._params, sc, meshes, steps=0)
+    e.set_mesh_motion(0, (0.0, 0.0, 0.0), omega)
+    e.step(200)
+    ref = e.download()
+    for k in ("x", "v", "w"):
+        assert np.array_equal(got[k], ref[k]), k
