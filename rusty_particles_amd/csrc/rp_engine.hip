@@ -1525,8 +1525,9 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
 // wave) and writes them.  base[i] = -1 when i has no rows or they would not
 // fit; rowmask[i] names the entries that have rows, in row order.  A served
 // contact without a row takes the exact solve in the force kernel.
-__global__ void __launch_bounds__(kBlock) k_jkr_collect(Ctl* ctl, unsigned long long off, KParams P, Dyn S, Stat T,
-                                                        const int* nbr_count, const int* nbr, SurrDev sd) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) k_jkr_collect(
+    Ctl* ctl, unsigned long long off, KParams P, Dyn S, Stat T,
+    const int* nbr_count, const int* nbr, SurrDev sd) {
   if (ctl->stall_off <= (long long)off || ctl->overflow) return;  // the force launch keeps the state: no rows
   const int n = ctl->n;
   const bool full = ctl->surr_mask_epoch != ctl->rebuilds + 1ull;
