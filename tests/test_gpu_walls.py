@@ -109,3 +109,24 @@ def test_simulation_api_applies_mesh_motion():
     ref = e.download()
     for k in ("x", "v", "w"):
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_mesh_motion_limits_and_errors():
+    """At most 4 meshes move at once; a fifth, an unknown mesh or a non-finite
+    value is refused with the previous motions kept, and stepping goes on."""
+    from rusty_particles_amd import native
+    p, sc, meshes = _setup(abi.RP_NORMAL_HERTZ, 0.0)
+    e = run_engine(p, sc, meshes * 5, steps=0)
+    for k in range(4):
+        e.set_mesh_motion(k, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0 + k))
+    with pytest.raises(native.EngineError):
+        e.set_mesh_motion(4, (0.0, 0.0, 0.0), (0.0, 0.0, 5.0))
+    with pytest.raises(native.EngineError):
+        e.set_mesh_motion(5, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    with pytest.raises(native.EngineError):
+        e.set_mesh_motion(0, (0.0, 0.0, 0.0), (0.0, float("nan"), 1.0))
+    e.set_mesh_motion(3, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))  # frees a slot
+    e.set_mesh_motion(4, (0.0, 0.0, 0.0), (0.0, 0.0, 5.0))
+    e.step(20)
+    st = e.download()
+    assert np.all(np.isfinite(st["x"]))
