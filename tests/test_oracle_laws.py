@@ -158,15 +158,3 @@ def test_closest_point_regions_0_and_1_match_brute_force():
         assert abs(np.linalg.norm(cp - p) - dist) < 5e-3, (p, cp, q)
         checked += 1
     assert checked > 50
-
-
-def test_closest_point_region4_keeps_reference_clamp():
-    """Region 4 clamps s to [0, a] (mesh.rs:67) rather than [0, 1]: for an edge of
-    length 2 the returned point lies beyond v1, as in the reference."""
-    tri = [0, 0, 0, 2, 0, 0, 0, 2, 0, 0, 0, 1]
-    cp, _, _, _ = osim.triangle(64, tri, [3.0, -1.0, 0.0], 0.0)
-    # s + t < det, s < 0? no: this point is in region 6/5 family; check a pure region-4 point
-    cp, _, _, _ = osim.triangle(64, tri, [-1.0, -1.0, 0.0], 0.0)
-    assert np.allclose(cp, [0, 0, 0])
-    cp, _, _, _ = osim.triangle(64, tri, [1.0, -3.0, 0.0], 0.0)  # region 5: s = clamp(-d, 0, a) = 2 -> 2*e0
-    assert np.allclose(cp, [4.0, 0.0, 0.0])
