@@ -97,7 +97,16 @@ typedef struct rp_params {
                                 the reference grid can see (grid.rs:84-94) */
   double cell_size;          /* engine's own broad-phase cell, 0 = automatic */
   double skin;               /* Verlet-list skin distance, 0 = automatic */
+  int32_t cell_order;        /* RP_CELL_ORDER_*: numbering of the broad-phase
+                                cells (no effect on results) */
+  int32_t reserved0;         /* 0 */
 } rp_params;
+
+/* Numbering of the engine's broad-phase cells (rp_params.cell_order; results
+ * never depend on it): AUTO picks pencils (8 x 8 cells of the two fast axes
+ * running along the slow axis) when a layer of the slow axis holds more than
+ * 8,192 particles, else row-major. */
+enum { RP_CELL_ORDER_AUTO = 0, RP_CELL_ORDER_ROW = 1, RP_CELL_ORDER_PENCIL = 2 };
 
 /* Counters of the most recent rp_engine_step call (integers only). */
 typedef struct rp_stats {
@@ -112,6 +121,11 @@ typedef struct rp_stats {
   int32_t neighbour_capacity;  /* current per-particle list capacity */
   int32_t max_neighbours;      /* largest list length seen at a rebuild */
   int32_t reserved;
+  uint64_t overlap_steps;      /* steps since create whose excessive_overlap
+                                  was set: the reference prints its warning
+                                  once per such step (simulation.rs:528-530) */
+  uint64_t history_contacts;   /* directed contacts that carried tangential
+                                  history in the last step (physics-fix mode) */
 } rp_stats;
 
 typedef struct rp_engine rp_engine;

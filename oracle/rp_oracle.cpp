@@ -24,6 +24,7 @@
 #include <array>
 #include <cstdio>
 #include <memory>
+#include <unordered_map>
 
 #include "../include/rp_abi.h"
 
@@ -131,6 +132,7 @@ class SimBase {
   virtual void get_forces(double* f3, double* t3, double* fscale) const = 0;
   virtual int overlap_flag() const = 0;
   virtual uint64_t jkr_failures() const = 0;
+  virtual void set_fast(bool on) = 0;
 };
 
 template <class R>
@@ -413,10 +415,13 @@ class Sim : public SimBase {
     int flag = 0;
     uint64_t fails = 0;
 
+    if (fast_) build_fast(n);
+
     // 3. per-particle gather (simulation.rs:317-526)
 #pragma omp parallel reduction(| : flag) reduction(+ : fails)
     {
       std::vector<int64_t> nb;
+      std::vector<std::pair<int, int64_t>> walk;  // fast path: (27-cell walk rank, j)
       nb.reserve(256);
 #pragma omp for schedule(dynamic, 64)
       for (int64_t i = 0; i < n; ++i) {
@@ -424,9 +429,9 @@ class Sim : public SimBase {
         V3<R> f = v3<R>(R(0), R(0), R(0));
         V3<R> t = v3<R>(R(0), R(0), R(0));
         R fs = R(0);
-        grid_.neighbours(p.x, per, nb);
-        for (int64_t j : nb) {
-          if (i == j) continue;
+        // one visit of j by the grid walk (simulation.rs:322-460)
+        auto visit = [&](int64_t j) {
+          if (i == j) return;
           const Particle<R>& o = ps_[j];
           V3<R> d = p.x - o.x;
           if (per[0]) {
@@ -445,7 +450,7 @@ class Sim : public SimBase {
           R rsum = p.r + o.r;
           R margin = adhesive ? rsum * R(0.5) : R(0);
           R cutoff = rsum + margin;
-          if (!(dsq < cutoff * cutoff)) continue;
+          if (!(dsq < cutoff * cutoff)) return;
           R dist = std::sqrt(dsq);
           R overlap = rsum - dist;
           if (!flag) {
@@ -468,13 +473,13 @@ class Sim : public SimBase {
           t = t + cross(ra, fc);
           fs += length(fc);
           if (record_) pairs_[i].push_back(j);
-        }
-        // walls (simulation.rs:465-523)
-        for (size_t ti = 0; ti < tris_.size(); ++ti) {
+        };
+        // one triangle (simulation.rs:465-523)
+        auto visit_tri = [&](size_t ti) {
           const Triangle<R>& tr = tris_[ti];
           R pen;
           V3<R> nrm;
-          if (!intersect_sphere(tr, p.x, p.r, &pen, &nrm)) continue;
+          if (!intersect_sphere(tr, p.x, p.r, &pen, &nrm)) return;
           if (!flag && pen > R(0.2) * p.r) flag = 1;
           V3<R> ra = (-nrm) * p.r;
           V3<R> vrel = p.v + cross(p.w, ra);
@@ -492,6 +497,15 @@ class Sim : public SimBase {
           f = f + fc;
           t = t + cross(ra, fc);
           fs += length(fc);
+        };
+        if (!fast_) {
+          grid_.neighbours(p.x, per, nb);
+          for (int64_t j : nb) visit(j);
+          for (size_t ti = 0; ti < tris_.size(); ++ti) visit_tri(ti);
+        } else {
+          fast_walk(i, walk);
+          for (const auto& e : walk) visit(e.second);
+          for (int32_t ti : fast_tris(i)) visit_tri((size_t)ti);
         }
         ftot_[i] = f;
         ttot_[i] = t;
@@ -513,6 +527,216 @@ class Sim : public SimBase {
       V3<R> aa = p.tau / inertia;
       kahan(p.w, p.rw, (R(0.5) * aa) * dt);
     }
+  }
+
+  // ---- fast candidate enumeration --------------------------------------------
+  // An optimisation of the oracle only, off by default: the same visits in
+  // the same order as the literal walk, pinned to it bit for bit by
+  // tests/test_oracle_fast.py.  The literal walk appends every particle of
+  // the 27 reference cells (grid.rs:48-87) and tests each; a particle j can
+  // pass the pair test only within the largest cutoff of i, so the fast path
+  // takes j from a fine cell list (cells >= that cutoff, minimum image on
+  // periodic axes), finds the offsets of the reference walk that reach j's
+  // reference cell (grid.rs:58-83: wrap once per periodic axis, range check)
+  // and visits j once per such offset, in the walk's order (offset rank, then
+  // index, as cells hold indices in insertion order).  Triangles: a triangle
+  // can touch particle i only if i lies within r_max of the set where
+  // closest_point can return its point (the triangle, v0 + [0, max(1, a, c)] e0
+  // and v0 + [0, max(1, c)] e1: the clamps of mesh.rs:67-105), so each fine
+  // cell lists the triangles whose grown box of that set meets it, in file
+  // order.
+  bool fast_{false};
+  double fcs_[3]{}, forg_[3]{};
+  int64_t fn_[3]{1, 1, 1};
+  bool fall_[3]{};
+  std::vector<int64_t> fkey_, fidx_;  // fine-cell key and particle index, sorted by (key, index)
+  std::vector<int64_t> fpkey_;        // fine-cell key of particle i
+  std::unordered_map<int64_t, std::vector<int32_t>> ftri_;
+
+ public:
+  void set_fast(bool on) override { fast_ = on; }
+
+ private:
+  int64_t fcoord(double x, int k) const {
+    if (!std::isfinite(x)) return 0;
+    double q = std::floor((x - forg_[k]) / fcs_[k]);
+    if (q < -1e15) q = -1e15;
+    if (q > 1e15) q = 1e15;
+    int64_t c = (int64_t)q;
+    if (periodic_[k]) {
+      c %= fn_[k];
+      if (c < 0) c += fn_[k];
+      return c;
+    }
+    return std::min<int64_t>(std::max<int64_t>(c, 0), fn_[k] - 1);
+  }
+  int64_t fkey(int64_t a, int64_t b, int64_t c) const { return (a * fn_[1] + b) * fn_[2] + c; }
+
+  void build_fast(int64_t n) {
+    double rmax = 0.0;
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int64_t i = 0; i < n; ++i) {
+      rmax = std::max(rmax, (double)ps_[i].r);
+      const double x[3] = {(double)ps_[i].x.x, (double)ps_[i].x.y, (double)ps_[i].x.z};
+      for (int k = 0; k < 3; ++k)
+        if (std::isfinite(x[k])) {
+          lo[k] = std::min(lo[k], x[k]);
+          hi[k] = std::max(hi[k], x[k]);
+        }
+    }
+    const bool adhesive = normal_ == RP_NORMAL_JKR || normal_ == RP_NORMAL_SJKR;
+    const double cs = (adhesive ? 1.5 : 1.0) * 2.0 * rmax * 1.001 + 1e-12;
+    const double bsz[3] = {(double)(bmax_.x - bmin_.x), (double)(bmax_.y - bmin_.y), (double)(bmax_.z - bmin_.z)};
+    const double bmn[3] = {(double)bmin_.x, (double)bmin_.y, (double)bmin_.z};
+    for (int k = 0; k < 3; ++k) {
+      if (periodic_[k]) {
+        int64_t m = (int64_t)std::floor(bsz[k] / cs);
+        m = std::max<int64_t>(1, std::min<int64_t>(m, 1 << 20));
+        fn_[k] = m;
+        fcs_[k] = bsz[k] / (double)m;
+        forg_[k] = bmn[k];
+        fall_[k] = m < 3;
+      } else {
+        if (!(lo[k] <= hi[k])) lo[k] = hi[k] = 0.0;
+        double c = cs;
+        if ((hi[k] - lo[k]) / c > (double)(1 << 20)) c = (hi[k] - lo[k]) / (double)(1 << 20);
+        fcs_[k] = c;
+        forg_[k] = lo[k] - c;
+        fn_[k] = (int64_t)std::ceil((hi[k] - lo[k]) / c) + 3;
+        fall_[k] = false;
+      }
+    }
+    fpkey_.resize(n);
+    fidx_.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+      fpkey_[i] = fkey(fcoord((double)ps_[i].x.x, 0), fcoord((double)ps_[i].x.y, 1), fcoord((double)ps_[i].x.z, 2));
+      fidx_[i] = i;
+    }
+    std::stable_sort(fidx_.begin(), fidx_.end(), [&](int64_t a, int64_t b) { return fpkey_[a] < fpkey_[b]; });
+    fkey_.resize(n);
+    for (int64_t q = 0; q < n; ++q) fkey_[q] = fpkey_[fidx_[q]];
+    ftri_.clear();
+    const double grow = rmax * 1.001 + 1e-9;
+    for (size_t ti = 0; ti < tris_.size(); ++ti) {
+      const Triangle<R>& tr = tris_[ti];
+      const double v[3][3] = {{(double)tr.v0.x, (double)tr.v0.y, (double)tr.v0.z},
+                              {(double)tr.v1.x, (double)tr.v1.y, (double)tr.v1.z},
+                              {(double)tr.v2.x, (double)tr.v2.y, (double)tr.v2.z}};
+      double e0[3], e1[3], a = 0.0, c = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        e0[k] = v[1][k] - v[0][k];
+        e1[k] = v[2][k] - v[0][k];
+        a += e0[k] * e0[k];
+        c += e1[k] * e1[k];
+      }
+      const double s0 = std::max(1.0, std::max(a, c)) * 1.001, t0 = std::max(1.0, c) * 1.001;
+      int64_t clo[3], chi[3];
+      bool ok = true;
+      for (int k = 0; k < 3; ++k) {
+        const double pts[5] = {v[0][k], v[1][k], v[2][k], v[0][k] + s0 * e0[k], v[0][k] + t0 * e1[k]};
+        double mn = HUGE_VAL, mx = -HUGE_VAL;
+        for (double q : pts) {
+          mn = std::min(mn, q);
+          mx = std::max(mx, q);
+        }
+        if (!std::isfinite(mn) || !std::isfinite(mx)) ok = false;
+        mn -= grow;
+        mx += grow;
+        const double q0 = std::floor((mn - forg_[k]) / fcs_[k]), q1 = std::floor((mx - forg_[k]) / fcs_[k]);
+        if (periodic_[k]) {
+          if (q1 - q0 + 1.0 >= (double)fn_[k]) {
+            clo[k] = 0;
+            chi[k] = fn_[k] - 1;
+            continue;
+          }
+          clo[k] = (int64_t)q0;
+          chi[k] = (int64_t)q1;
+        } else {
+          clo[k] = (int64_t)std::max(0.0, std::min(q0, (double)(fn_[k] - 1)));
+          chi[k] = (int64_t)std::max(0.0, std::min(q1, (double)(fn_[k] - 1)));
+        }
+      }
+      if (!ok) continue;  // a non-finite vertex never passes intersect_sphere
+      auto wrapc = [&](int64_t q, int k) {
+        if (!periodic_[k]) return q;
+        q %= fn_[k];
+        return q < 0 ? q + fn_[k] : q;
+      };
+      for (int64_t x = clo[0]; x <= chi[0]; ++x)
+        for (int64_t y = clo[1]; y <= chi[1]; ++y)
+          for (int64_t z = clo[2]; z <= chi[2]; ++z) {
+            const int64_t key = fkey(wrapc(x, 0), wrapc(y, 1), wrapc(z, 2));
+            if (!std::binary_search(fkey_.begin(), fkey_.end(), key)) continue;  // no particle there
+            std::vector<int32_t>& lst = ftri_[key];
+            if (lst.empty() || lst.back() != (int32_t)ti) lst.push_back((int32_t)ti);
+          }
+    }
+  }
+
+  // The walk's visits of particle i: (rank oz*9 + oy*3 + ox of the offset, j).
+  void fast_walk(int64_t i, std::vector<std::pair<int, int64_t>>& walk) const {
+    walk.clear();
+    const Particle<R>& p = ps_[i];
+    int32_t ci[3];
+    grid_.coords(p.x, ci);
+    const double x[3] = {(double)p.x.x, (double)p.x.y, (double)p.x.z};
+    int64_t cand[3][3];
+    int ncand[3];
+    std::vector<int64_t> all[3];
+    for (int k = 0; k < 3; ++k) {
+      const int64_t c = fcoord(x[k], k);
+      ncand[k] = 0;
+      if (fall_[k]) {
+        for (int64_t q = 0; q < fn_[k]; ++q) all[k].push_back(q);
+        continue;
+      }
+      for (int o = -1; o <= 1; ++o) {
+        int64_t q = c + o;
+        if (periodic_[k]) q = (q + fn_[k]) % fn_[k];
+        else if (q < 0 || q >= fn_[k]) continue;
+        cand[k][ncand[k]++] = q;
+      }
+      for (int m = 0; m < ncand[k]; ++m) all[k].push_back(cand[k][m]);
+    }
+    for (int64_t a : all[0])
+      for (int64_t b : all[1])
+        for (int64_t c : all[2]) {
+          const int64_t key = fkey(a, b, c);
+          auto rg = std::equal_range(fkey_.begin(), fkey_.end(), key);
+          for (auto it = rg.first; it != rg.second; ++it) {
+            const int64_t j = fidx_[it - fkey_.begin()];
+            if (j == i) continue;
+            int32_t cj[3];
+            grid_.coords(ps_[j].x, cj);
+            bool vis = true;
+            for (int k = 0; k < 3; ++k) vis = vis && cj[k] >= 0 && cj[k] < grid_.dims[k];
+            if (!vis) continue;  // not in the reference grid (grid.rs:30-46)
+            int mask[3];
+            for (int k = 0; k < 3; ++k) {
+              mask[k] = 0;
+              for (int o = -1; o <= 1; ++o) {
+                int32_t nn = Grid<R>::wrap_add(ci[k], o);
+                if (periodic_[k]) {
+                  if (nn < 0) nn += grid_.dims[k];
+                  else if (nn >= grid_.dims[k]) nn -= grid_.dims[k];
+                }
+                if (nn == cj[k]) mask[k] |= 1 << (o + 1);
+              }
+            }
+            for (int oz = 0; oz < 3; ++oz)
+              for (int oy = 0; oy < 3; ++oy)
+                for (int ox = 0; ox < 3; ++ox)
+                  if ((mask[2] >> oz & 1) && (mask[1] >> oy & 1) && (mask[0] >> ox & 1))
+                    walk.emplace_back(oz * 9 + oy * 3 + ox, j);
+          }
+        }
+    std::sort(walk.begin(), walk.end());
+  }
+
+  const std::vector<int32_t>& fast_tris(int64_t i) const {
+    static const std::vector<int32_t> none;
+    auto it = ftri_.find(fpkey_[i]);
+    return it == ftri_.end() ? none : it->second;
   }
 
   R dt_{};
@@ -572,6 +796,7 @@ void rpo_get_forces(void* s, double* f3, double* t3, double* fscale) {
 }
 int rpo_overlap_flag(void* s) { return static_cast<SimBase*>(s)->overlap_flag(); }
 uint64_t rpo_jkr_failures(void* s) { return static_cast<SimBase*>(s)->jkr_failures(); }
+void rpo_set_fast(void* s, int on) { static_cast<SimBase*>(s)->set_fast(on != 0); }
 void rpo_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);
 }

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
     L.rpo_overlap_flag.argtypes = [C.c_void_p]
     L.rpo_jkr_failures.argtypes = [C.c_void_p]
     L.rpo_jkr_failures.restype = C.c_uint64
+    L.rpo_set_fast.argtypes = [C.c_void_p, C.c_int]
     L.rpo_set_threads.argtypes = [C.c_int]
     L.rpo_max_threads.restype = C.c_int
     L.rpo_normal_law.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double,
@@ -216,6 +217,11 @@ class OracleSimulation:
     def set_cell_size(self, cs):
         """Reference-grid cell (1.0 m as shipped, simulation.rs:66)."""
         self._L.rpo_set_cell_size(self._h, float(cs))
+
+    def set_fast(self, on=True):
+        """Fast candidate enumeration: the same visits in the same order as the
+        literal 27-cell walk (see rp_oracle.cpp), for large inputs."""
+        self._L.rpo_set_fast(self._h, 1 if on else 0)
 
     def set_record(self, on=True):
         self._L.rpo_set_record(self._h, 1 if on else 0)

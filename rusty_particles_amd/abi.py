@@ -66,6 +66,8 @@ class RpParams(C.Structure):
         ("ref_cell_size", C.c_double),
         ("cell_size", C.c_double),
         ("skin", C.c_double),
+        ("cell_order", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -111,6 +113,8 @@ class RpStats(C.Structure):
         ("neighbour_capacity", C.c_int32),
         ("max_neighbours", C.c_int32),
         ("reserved", C.c_int32),
+        ("overlap_steps", C.c_uint64),
+        ("history_contacts", C.c_uint64),
     ]
 
 

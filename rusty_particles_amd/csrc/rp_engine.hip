@@ -159,6 +159,12 @@ struct Ctl {
   // (a rebuild re-orders the lists, so the next collect scans every entry)
   unsigned long long surr_mask_epoch;
   unsigned surr_rows_max;  // most rows any range was asked for in one step (sizes the row buffers)
+  // Absolute launch number of the label base (k_shift_labels moves both):
+  // label_base + off numbers every force launch uniquely.
+  unsigned long long label_base;
+  // Steps with excessive overlap: the last launch number that saw one and
+  // the count (one increment per launch, by the first workgroup to see it).
+  unsigned long long ovl_last, ovl_steps;
 };
 
 // Per-workgroup counters of one force launch.
@@ -1444,6 +1450,11 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
       b.overlap |= red[q][3];
     }
     bstats[blockIdx.x] = b;
+    if (b.overlap) {
+      // count this step once (simulation.rs:528-530 warns once per step)
+      const unsigned long long lab = ctl->label_base + off + 1ull;
+      if (atomicMax(&ctl->ovl_last, lab) < lab) atomicAdd(&ctl->ovl_steps, 1ull);
+    }
   }
   if (JS >= 2) {
     // surrogate counters: wave reductions, then one store per workgroup
@@ -1625,6 +1636,7 @@ __global__ void __launch_bounds__(512) k_copy_probe(const double* __restrict__ s
 // Moves the label base forward by `by` steps: offsets are relative to it.
 __global__ void k_shift_labels(Ctl* ctl, unsigned long long by) {
   if (ctl->stall_off != kNoStall) ctl->stall_off -= (long long)by;
+  ctl->label_base += by;
 }
 
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
@@ -1899,14 +1911,12 @@ rp_status compute_geometry(rp_engine* e) {
   // Pencil numbering (cell_linear) when a layer of the slow axis holds many
   // particles (cubic boxes): there a contiguous run of cells in row-major
   // order is a thin full-width slab whose neighbours do not fit in L2.
-  // RP_CELL_ORDER=row|pencil overrides the choice.
+  // rp_params.cell_order overrides the choice.
   const int fa = order[0], ma = order[1], sa = order[2];
   const double per_layer = (double)e->cap / ((double)K.ninst * (double)K.gn[sa]);
   bool pencil = per_layer > 8192.0 && K.gn[fa] >= 16 && K.gn[ma] >= 16;
-  if (const char* env = std::getenv("RP_CELL_ORDER")) {
-    if (std::string(env) == "row") pencil = false;
-    if (std::string(env) == "pencil") pencil = true;
-  }
+  if (p.cell_order == RP_CELL_ORDER_ROW) pencil = false;
+  if (p.cell_order == RP_CELL_ORDER_PENCIL) pencil = true;
   const double pf = std::ceil(K.gn[fa] / 8.0), pm = std::ceil(K.gn[ma] / 8.0);
   if (pencil && pm * pf * K.gn[sa] * 64.0 * K.ninst > kMaxCells) pencil = false;
   if (pencil) {
@@ -2002,10 +2012,25 @@ rp_status upload_meshes_binned(rp_engine* e) {
   };
   for (size_t t = 0; t < tris.size(); ++t) {
     const Tri& tr = tris[t];
+    // The reference's closest_point clamps the edge parameters of regions
+    // 2-6 to [0, a] or [0, c] (a = |e0|^2, c = |e1|^2; mesh.rs:67-105), not
+    // [0, 1]: its point lies on v0 + [0, max(a, c)] e0, v0 + [0, c] e1 or
+    // the triangle.  On edges longer than 1 m that reaches past the triangle,
+    // so the binned box covers those segments too.
+    const dv3 e0 = sub(tr.v1, tr.v0), e1 = sub(tr.v2, tr.v0);
+    const double ea = dot(e0, e0), ec = dot(e1, e1);
+    const dv3 x0 = add(tr.v0, smul(std::max(1.0, std::max(ea, ec)), e0));
+    const dv3 x1 = add(tr.v0, smul(std::max(1.0, ec), e1));
     int c0[3], c1[3];
     for (int k = 0; k < 3; ++k) {
       double a = comp(tr.v0, k), b = comp(tr.v1, k), c = comp(tr.v2, k);
-      double mn = std::min(a, std::min(b, c)) - grow, mx = std::max(a, std::max(b, c)) + grow;
+      double mn = std::min(a, std::min(b, c)), mx = std::max(a, std::max(b, c));
+      if (std::isfinite(comp(x0, k)) && std::isfinite(comp(x1, k))) {
+        mn = std::min(mn, std::min(comp(x0, k), comp(x1, k)));
+        mx = std::max(mx, std::max(comp(x0, k), comp(x1, k)));
+      }
+      mn -= grow;
+      mx += grow;
       c0[k] = clampc(mn, k);
       c1[k] = clampc(mx, k);
     }
@@ -3082,6 +3107,7 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
   out->removed = (uint64_t)(e->n_uploaded - c.alive);
   out->neighbour_capacity = e->nbr_cap;
   out->max_neighbours = c.max_count;
+  out->overlap_steps = c.ovl_steps;
   if (e->force_grid > 0) {
     std::vector<BlockStats> bs((size_t)e->force_grid);
     HIP_TRY(hipMemcpy(bs.data(), e->d_bstats, bs.size() * sizeof(BlockStats), hipMemcpyDeviceToHost));

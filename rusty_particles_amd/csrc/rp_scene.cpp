@@ -86,8 +86,44 @@ static int g_progress = 0;
 // placement loop itself, for callers that run long placements unattended.
 void rps_set_progress(int on) { g_progress = on; }
 
+// Region of the optional cone constraint of rsa_impl: centres with
+// y0 <= y <= y1 and a distance from the cone's surface (the axis is +Y
+// through (0, y0, 0), radius r0 at y0, growing by tan(half-angle) per metre)
+// of at least `clear` times their radius.
+struct Cone {
+  double y0, y1, r0, tan_a, cos_a, clear;
+  bool inside(const double* p, double ri) const {
+    if (p[1] < y0 || p[1] > y1) return false;
+    const double rho = std::sqrt(p[0] * p[0] + p[2] * p[2]);
+    return (r0 + (p[1] - y0) * tan_a - rho) * cos_a >= clear * ri;
+  }
+};
+
+static long long rsa_impl(long long n, const double* lo, const double* hi, double rmin, double rmax,
+                          const double* radii, const int* periodic, unsigned long long seed, long long max_attempts,
+                          double* x3, double* r, const Cone* cone);
+
 long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin, double rmax, const double* radii,
                    const int* periodic, unsigned long long seed, long long max_attempts, double* x3, double* r) {
+  return rsa_impl(n, lo, hi, rmin, rmax, radii, periodic, seed, max_attempts, x3, r, nullptr);
+}
+
+// RSA inside a conical hopper: radii uniform in [rmin, rmax], centres in the
+// box [lo, hi] and inside the cone (Cone above) with clearance clear * r from
+// its surface (clear < 1: the sphere overlaps the wall by (1 - clear) r).
+long long rps_rsa_cone(long long n, const double* lo, const double* hi, double rmin, double rmax, double y0,
+                       double y1, double r0, double half_angle_deg, double clear, unsigned long long seed,
+                       long long max_attempts, double* x3, double* r) {
+  const double a = half_angle_deg * 3.14159265358979323846 / 180.0;
+  const Cone c{y0, y1, r0, std::tan(a), std::cos(a), clear};
+  return rsa_impl(n, lo, hi, rmin, rmax, nullptr, nullptr, seed, max_attempts, x3, r, &c);
+}
+
+}  // extern "C"
+
+static long long rsa_impl(long long n, const double* lo, const double* hi, double rmin, double rmax,
+                          const double* radii, const int* periodic, unsigned long long seed, long long max_attempts,
+                          double* x3, double* r, const Cone* cone) {
   Rng rng(seed);
   const int per[3] = {periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
   double L[3];
@@ -151,7 +187,7 @@ long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin,
       ++attempts;
       for (int k = 0; k < 3; ++k)
         p[k] = per[k] ? lo[k] + L[k] * rng.uniform() : lo[k] + ri + (hi[k] - lo[k] - 2.0 * ri) * rng.uniform();
-      ok = !overlaps(big, p, ri, ri + rmax) && !overlaps(small, p, ri, ri + rthr);
+      ok = (!cone || cone->inside(p, ri)) && !overlaps(big, p, ri, ri + rmax) && !overlaps(small, p, ri, ri + rthr);
     }
     if (!ok) break;
     x3[3 * placed] = p[0];
@@ -168,9 +204,7 @@ long long rps_rsa2(long long n, const double* lo, const double* hi, double rmin,
 }
 
 
-long long rps_rsa(long long n, const double* lo, const double* hi, double rmin, double rmax, unsigned long long seed,
-                  long long max_attempts, double* x3, double* r) {
+extern "C" long long rps_rsa(long long n, const double* lo, const double* hi, double rmin, double rmax,
+                             unsigned long long seed, long long max_attempts, double* x3, double* r) {
   return rps_rsa2(n, lo, hi, rmin, rmax, nullptr, nullptr, seed, max_attempts, x3, r);
 }
-
-}  // extern "C"

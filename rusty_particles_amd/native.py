@@ -12,8 +12,7 @@ import numpy as np
 
 from . import abi
 
-# RP_HIP_LIB names another build of the same library (A/B timing of build options)
-_LIB_PATH = os.environ.get("RP_HIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "librp_hip.so")
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librp_hip.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)

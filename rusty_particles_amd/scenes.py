@@ -30,6 +30,9 @@ def _lib():
         L.rps_rsa2.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, dp, C.POINTER(C.c_int),
                                C.c_ulonglong, C.c_longlong, dp, dp]
         L.rps_rsa2.restype = C.c_longlong
+        L.rps_rsa_cone.argtypes = [C.c_longlong, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_ulonglong, C.c_longlong, dp, dp]
+        L.rps_rsa_cone.restype = C.c_longlong
         L.rps_set_progress.argtypes = [C.c_int]
         L.rps_set_progress.restype = None
         _scene = L
@@ -67,6 +70,26 @@ def rsa_radii(radii, lo, hi, periodic, seed: int, max_attempts: int = 200000):
                              r.ctypes.data_as(dp))
     if placed < n:
         raise RuntimeError(f"RSA placed only {placed} of {n} spheres")
+    return x, r
+
+
+def rsa_cone(n: int, rmin: float, rmax: float, y0: float, y1: float, r0: float, half_angle_deg: float,
+             clear: float, seed: int, max_attempts: int = 200000):
+    """Random sequential addition of n non-overlapping spheres (radii U[rmin,
+    rmax]) inside a cone around +Y (radius r0 at y0, half-angle
+    half_angle_deg), centres between y0 and y1 and at least clear * r from the
+    cone's surface."""
+    x = np.zeros((n, 3))
+    r = np.zeros(n)
+    rtop = r0 + (y1 - y0) * math.tan(math.radians(half_angle_deg))
+    lo = np.array([-rtop, y0, -rtop])
+    hi = np.array([rtop, y1, rtop])
+    dp = C.POINTER(C.c_double)
+    placed = _lib().rps_rsa_cone(n, lo.ctypes.data_as(dp), hi.ctypes.data_as(dp), rmin, rmax, y0, y1, r0,
+                                 half_angle_deg, clear, seed, max_attempts, x.ctypes.data_as(dp),
+                                 r.ctypes.data_as(dp))
+    if placed < n:
+        raise RuntimeError(f"RSA placed only {placed} of {n} spheres in the cone")
     return x, r
 
 
@@ -234,6 +257,37 @@ def hopper_workload(n: int = 1_000_000, seed: int = 0xD3E50002, phi: float = 0.2
     x, r = rsa(n, lo, hi, rmin, rmax, seed)
     m = sphere_mass(r, 2500.0)
     p = abi.default_params(2e-6, (-0.1, -0.05, -0.1), (0.1, hi[1] + 0.01, 0.1))
+    p.normal_model = abi.RP_NORMAL_HERTZ
+    p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
+    p.particle = abi.RpMaterial(5e7, 0.25, 2500.0, 0.4, 0.6, 0.0)
+    return p, x, r, m, [(hv9, hn3)]
+
+
+HOPPER_OUTLET, HOPPER_HALF_ANGLE, HOPPER_TOP = 0.008, 30.0, 0.075
+
+
+def hopper_test_workload(n: int = 20_000, seed: int = 0xD3E50012, clear: float = 0.97, inflate: float = 1.04):
+    """The hopper configuration's spheres, materials and 4,096-triangle cone
+    (hopper_workload) with a bed of n spheres inside the cone instead of the
+    column above it: random sequential addition in the cone from its outlet
+    up, centres 0.97 r from the wall (the spheres near the wall press on
+    it), then radii grown by 4 % so that near neighbours overlap; spheres
+    near the outlet fall through it.  Sized for the oracle (20k spheres, a
+    bed ~7 cm deep).  Returns (params, x, r, m, meshes)."""
+    rmin, rmax = 0.5e-3 / inflate, 1.5e-3 / inflate
+    hv9, hn3 = hopper_mesh(HOPPER_OUTLET, HOPPER_HALF_ANGLE, HOPPER_TOP, y0=0.0, nseg=64, nring=32)
+    # bed height for solid fraction ~0.2 in the cone
+    mean_vol = 4.0 / 3.0 * np.pi * (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin))
+    t = math.tan(math.radians(HOPPER_HALF_ANGLE))
+    need = n * mean_vol / 0.2
+    h = 0.005
+    while np.pi * h / 3.0 * (HOPPER_OUTLET ** 2 + HOPPER_OUTLET * (HOPPER_OUTLET + h * t)
+                             + (HOPPER_OUTLET + h * t) ** 2) < need:
+        h += 0.001
+    x, r = rsa_cone(n, rmin, rmax, 0.0, h, HOPPER_OUTLET, HOPPER_HALF_ANGLE, clear, seed)
+    r = r * inflate
+    m = sphere_mass(r, 2500.0)
+    p = abi.default_params(2e-6, (-0.1, -0.05, -0.1), (0.1, 0.14, 0.1))
     p.normal_model = abi.RP_NORMAL_HERTZ
     p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
     p.particle = abi.RpMaterial(5e7, 0.25, 2500.0, 0.4, 0.6, 0.0)

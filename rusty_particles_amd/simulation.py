@@ -45,6 +45,12 @@ def hello() -> str:
 
 _FIELDS3 = ("x", "v", "w", "a", "tau", "rx", "rv", "rw")
 
+# meshes whose surface may move at once (rp_engine_set_mesh_motion)
+MAX_MOVING_MESHES = 4
+
+# src/simulation.rs:529
+OVERLAP_WARNING = "Warning: Excessive particle overlap detected (>20% of radius). Simulation may be unstable."
+
 
 class Simulation:
     """The reference's `Simulation` class (src/lib.rs:41-277)."""
@@ -65,6 +71,10 @@ class Simulation:
         self._engine: native.Engine | None = None
         self._record = False
         self._jkr = ("exact", None, False)  # solver, surrogate, check
+        self._ovl_seen = 0  # overlap_steps of the engine already warned about
+        # bumped by every change of state, parameters or meshes made through
+        # this object (an Ensemble re-uploads a member whose version moved)
+        self._version = 0
 
     # ---- construction (src/lib.rs:57-80) ---------------------------------------
     @staticmethod
@@ -81,6 +91,7 @@ class Simulation:
         h["mass"] = np.append(h["mass"], float(mass))
         h["fixed"] = np.append(h["fixed"], np.uint8(1 if fixed else 0))
         self._engine_valid = False
+        self._version += 1
 
     def add_particles(self, positions, radii, masses, fixed=None):
         """Bulk form of add_particle (same semantics, one call for many particles)."""
@@ -96,6 +107,7 @@ class Simulation:
         f = np.zeros(n, dtype=np.uint8) if fixed is None else np.asarray(fixed, dtype=np.uint8).reshape(n)
         h["fixed"] = np.concatenate([h["fixed"], f])
         self._engine_valid = False
+        self._version += 1
 
     def add_mesh(self, path):
         """src/lib.rs:87-91 (IOError when the file cannot be read)."""
@@ -107,6 +119,7 @@ class Simulation:
         v9 = np.asarray(v9, dtype=np.float32).reshape(-1, 9)
         n3 = np.asarray(n3, dtype=np.float32).reshape(-1, 3)
         self._meshes.append((v9, n3))
+        self._version += 1
         if self._engine is not None:
             self._engine.add_mesh(v9, n3)
 
@@ -123,9 +136,16 @@ class Simulation:
         w = tuple(float(c) for c in angular_velocity)
         if len(o) != 3 or len(w) != 3:
             raise ValueError("origin and angular_velocity are 3-vectors")
+        if not all(math.isfinite(c) for c in (*o, *w)):
+            raise ValueError("origin and angular_velocity must be finite")
+        moving = {k for k, (_, ww) in self._motions.items() if any(ww)}
+        moving.discard(index)
+        if any(w) and len(moving) >= MAX_MOVING_MESHES:
+            raise ValueError(f"at most {MAX_MOVING_MESHES} meshes move at once")
         if self._engine is not None:
             self._engine.set_mesh_motion(index, o, w)
         self._motions[index] = (o, w)
+        self._version += 1
 
     # ---- stepping (src/lib.rs:94-96, 182-195) --------------------------------------
     def step(self):
@@ -140,6 +160,17 @@ class Simulation:
         eng = self._prepare_engine()
         eng.step(n)
         self._host_valid = False
+        self._version += 1
+        self._warn_overlap(eng)
+
+    def _warn_overlap(self, eng: native.Engine) -> None:
+        """The reference prints this line once for every step in which a pair
+        or a wall contact overlaps by more than 20 % of the smaller radius
+        (src/simulation.rs:361-370, 469-473, 528-530)."""
+        k = int(eng.stats()["overlap_steps"])
+        for _ in range(k - self._ovl_seen):
+            print(OVERLAP_WARNING, file=sys.stderr)
+        self._ovl_seen = k
 
     def _prepare_engine(self) -> native.Engine:
         eng = self._ensure_engine()
@@ -151,6 +182,7 @@ class Simulation:
             eng.upload(h["x"], h["radius"], h["mass"], h["fixed"], v=h["v"], w=h["w"], a=h["a"], tau=h["tau"],
                        rx=h["rx"], rv=h["rv"], rw=h["rw"])
             self._engine_valid = True
+            self._ovl_seen = 0  # the upload starts the engine's counters afresh
         return eng
 
     # ---- materials, models, boundaries (src/lib.rs:98-180) --------------------------
@@ -158,18 +190,18 @@ class Simulation:
                               restitution_coefficient, surface_energy=0.0):
         self._params.particle = abi.RpMaterial(youngs_modulus, poissons_ratio, density, friction_coefficient,
                                                restitution_coefficient, surface_energy)
-        self._params_dirty = True
+        self._params_changed()
 
     def set_wall_material(self, youngs_modulus, poissons_ratio, density, friction_coefficient,
                           restitution_coefficient, surface_energy=0.0):
         self._params.wall = abi.RpMaterial(youngs_modulus, poissons_ratio, density, friction_coefficient,
                                            restitution_coefficient, surface_energy)
-        self._params_dirty = True
+        self._params_changed()
 
     def set_periodic_boundary(self, x, y, z):
         for k, flag in enumerate((x, y, z)):
             self._params.periodic[k] = 1 if flag else 0
-        self._params_dirty = True
+        self._params_changed()
 
     def set_contact_models(self, normal, tangential):
         nm = abi.NORMAL_MODELS.get(str(normal).lower())
@@ -180,18 +212,18 @@ class Simulation:
             raise ValueError("Invalid tangential force model")
         self._params.normal_model = nm
         self._params.tangential_model = tm
-        self._params_dirty = True
+        self._params_changed()
 
     def set_gravity(self, gx, gy, gz):
         """Extension: the reference hard-codes (0, -9.81, 0) (src/simulation.rs:267)."""
         self._params.gravity[0], self._params.gravity[1], self._params.gravity[2] = gx, gy, gz
-        self._params_dirty = True
+        self._params_changed()
 
     def set_engine_tuning(self, cell_size=0.0, skin=0.0):
         """Extension: broad-phase cell size and Verlet skin (0 = automatic)."""
         self._params.cell_size = float(cell_size)
         self._params.skin = float(skin)
-        self._params_dirty = True
+        self._params_changed()
 
     def set_jkr_solver(self, solver="exact", surrogate=None, check=False):
         """Extension: 'exact' (the reference's Newton solve, default),
@@ -204,6 +236,7 @@ class Simulation:
         if solver == "surrogate" and surrogate is None:
             raise ValueError("the surrogate solver needs a trained network")
         self._jkr = (solver, surrogate, bool(check))
+        self._version += 1
         if self._engine is not None:
             self._engine.set_jkr_solver(solver, surrogate, check=check)
 
@@ -220,6 +253,10 @@ class Simulation:
         skin, cell = self._engine.autotune(steps)
         self._params.skin, self._params.cell_size = skin, cell
         return skin, cell
+
+    def _params_changed(self) -> None:
+        self._params_dirty = True
+        self._version += 1
 
     def enable_gpu(self):
         """src/lib.rs:146-148: RuntimeError when no device can be used."""
@@ -283,6 +320,7 @@ class Simulation:
             raise IndexError("Particle index out of bounds")
         self._h[key][idx] = (float(a), float(b), float(c))
         self._engine_valid = False
+        self._version += 1
 
     def set_particle_velocity(self, id, vx, vy, vz):
         self._set3("v", id, vx, vy, vz)
@@ -319,16 +357,22 @@ class Simulation:
                 eng = native.Engine(self._params)
             except native.EngineError as e:
                 raise RuntimeError(str(e)) from e
+            # meshes, motions and solver first: the engine is published only
+            # once all of them applied (a failure leaves no engine behind)
+            try:
+                for v9, n3 in self._meshes:
+                    eng.add_mesh(v9, n3)
+                for k, (o, w) in self._motions.items():
+                    eng.set_mesh_motion(k, o, w)
+                eng.set_record(self._record)
+                if self._jkr[0] != "exact":
+                    eng.set_jkr_solver(*self._jkr[:2], check=self._jkr[2])
+            except Exception:
+                eng.close()
+                raise
             self._engine = eng
             self._params_dirty = False
             self._engine_valid = False
-            for v9, n3 in self._meshes:
-                eng.add_mesh(v9, n3)
-            for k, (o, w) in self._motions.items():
-                eng.set_mesh_motion(k, o, w)
-            eng.set_record(self._record)
-            if self._jkr[0] != "exact":
-                eng.set_jkr_solver(*self._jkr[:2], check=self._jkr[2])
         return self._engine
 
     def _sync_host(self) -> None:
@@ -344,30 +388,43 @@ class Ensemble:
     objects together on one batched engine -- one set of kernel launches per
     step over all of them.  The simulations must agree on everything but
     their particles and materials (dt, bounds, periodic axes, contact models,
-    gravity, meshes, engine tuning); each evolves exactly as it would alone.
-    After run()/step() every member's getters return its new state."""
+    gravity, meshes and their motions, engine tuning, JKR solver); each
+    evolves exactly as it would alone (with the JKR surrogate: as it would
+    alone with the surrogate, to its force-error bound).  After run()/step()
+    every member's getters return its new state.  A member changed between
+    two ensemble steps (a setter, add_particle, or a step of its own) makes
+    the next ensemble step upload every member's current state again."""
 
     def __init__(self, sims):
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("an ensemble needs at least one simulation")
+        self._check_members()
+        self._engine: native.Engine | None = None
+        self._versions: list[int] | None = None
+        self._dt32 = self.sims[0]._dt32
+
+    def _check_members(self) -> None:
         p0, m0 = self.sims[0]._params, self.sims[0]._meshes
         for s in self.sims[1:]:
             p = s._params
             same = (p.dt == p0.dt and list(p.bounds_min) == list(p0.bounds_min)
                     and list(p.bounds_max) == list(p0.bounds_max) and list(p.periodic) == list(p0.periodic)
                     and p.normal_model == p0.normal_model and p.tangential_model == p0.tangential_model
-                    and list(p.gravity) == list(p0.gravity) and p.ref_cell_size == p0.ref_cell_size
-                    and p.cell_size == p0.cell_size and p.skin == p0.skin)
+                    and p.mode == p0.mode and list(p.gravity) == list(p0.gravity)
+                    and p.ref_cell_size == p0.ref_cell_size and p.cell_size == p0.cell_size
+                    and p.skin == p0.skin and p.cell_order == p0.cell_order)
             if not same:
-                raise ValueError("ensemble members must share dt, bounds, periodic axes, models, gravity and tuning")
+                raise ValueError("ensemble members must share dt, bounds, periodic axes, models, mode, gravity and "
+                                 "tuning")
             if len(s._meshes) != len(m0) or any(not (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]))
                                                 for a, b in zip(s._meshes, m0)):
                 raise ValueError("ensemble members must share their meshes")
             if s._motions != self.sims[0]._motions:
                 raise ValueError("ensemble members must share their meshes' motions")
-        self._engine: native.Engine | None = None
-        self._dt32 = self.sims[0]._dt32
+            j0, j = self.sims[0]._jkr, s._jkr
+            if j[0] != j0[0] or j[1] is not j0[1] or j[2] != j0[2]:
+                raise ValueError("ensemble members must share their JKR solver (and surrogate network)")
 
     def step(self):
         self._steps(1)
@@ -377,27 +434,35 @@ class Ensemble:
         if steps > 0:
             self._steps(steps)
 
+    def _upload(self) -> None:
+        self._check_members()
+        if self._engine is not None:
+            self._engine.close()
+            self._engine = None
+        try:
+            eng = native.Engine(abi.RpParams.from_buffer_copy(self.sims[0]._params))
+        except native.EngineError as e:
+            raise RuntimeError(str(e)) from e
+        for s in self.sims:
+            s._sync_host()
+        hs = [s._h for s in self.sims]
+        cat = lambda k: np.concatenate([h[k] for h in hs])
+        eng.upload_batch([h["x"].shape[0] for h in hs], [s._params.particle for s in self.sims], cat("x"),
+                         cat("radius"), cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"), a=cat("a"),
+                         tau=cat("tau"), rx=cat("rx"), rv=cat("rv"), rw=cat("rw"),
+                         wall_materials=[s._params.wall for s in self.sims])
+        for v9, n3 in self.sims[0]._meshes:
+            eng.add_mesh(v9, n3)
+        for k, (o, w) in self.sims[0]._motions.items():
+            eng.set_mesh_motion(k, o, w)
+        solver, net, check = self.sims[0]._jkr
+        if solver != "exact":
+            eng.set_jkr_solver(solver, net, check=check)
+        self._engine = eng
+
     def _steps(self, n: int) -> None:
-        if self._engine is None:
-            try:
-                eng = native.Engine(abi.RpParams.from_buffer_copy(self.sims[0]._params))
-            except native.EngineError as e:
-                raise RuntimeError(str(e)) from e
-            for s in self.sims:
-                s._sync_host()
-            hs = [s._h for s in self.sims]
-            cat = lambda k: np.concatenate([h[k] for h in hs])
-            eng.upload_batch([h["x"].shape[0] for h in hs], [s._params.particle for s in self.sims], cat("x"),
-                             cat("radius"), cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"), a=cat("a"),
-                             tau=cat("tau"), rx=cat("rx"), rv=cat("rv"), rw=cat("rw"),
-                             wall_materials=[s._params.wall for s in self.sims])
-            for v9, n3 in self.sims[0]._meshes:
-                eng.add_mesh(v9, n3)
-            for k, (o, w) in self.sims[0]._motions.items():
-                eng.set_mesh_motion(k, o, w)
-            if self.sims[0]._jkr[0] == "shortcut":
-                eng.set_jkr_solver("shortcut")
-            self._engine = eng
+        if self._engine is None or self._versions != [s._version for s in self.sims]:
+            self._upload()
         self._engine.step(n)
         # members see their new state through their own getters
         counts = self._engine.instance_counts()
@@ -408,3 +473,4 @@ class Ensemble:
             s._host_valid = True
             s._engine_valid = False
             o += int(c)
+        self._versions = [s._version for s in self.sims]

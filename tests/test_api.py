@@ -130,3 +130,78 @@ def test_set_mesh_motion_checks_its_arguments():
     lib = native.lib()
     o = (native.C.c_double * 3)()
     assert lib.rp_engine_set_mesh_motion(None, 0, o, o) == rp.abi.RP_ERR_INVALID
+
+
+def test_reference_module_name():
+    """pydem scripts import the reference's module name (src/lib.rs:24-35)."""
+    import rusty_particles
+    assert rusty_particles.Simulation is rp.Simulation
+    assert rusty_particles.hello() == "Hello from Rust!"
+    assert rusty_particles.set_num_threads is rp.set_num_threads
+    sim = rusty_particles.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 1)
+    sim.add_particle(0.0, 0.5, 0.0, 0.01, 1.0)
+    assert sim.get_particle_count() == 1
+
+
+def test_set_mesh_motion_validates_before_any_engine():
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    from rusty_particles_amd import scenes
+    for _ in range(6):
+        sim.add_mesh_arrays(*scenes.box_mesh((-1, -1, -1), (1, 1, 1)))
+    with pytest.raises(ValueError):
+        sim.set_mesh_motion(0, (0, 0, 0), (0, float("nan"), 0))
+    with pytest.raises(ValueError):
+        sim.set_mesh_motion(0, (float("inf"), 0, 0), (0, 1, 0))
+    for k in range(4):
+        sim.set_mesh_motion(k, (0, 0, 0), (0, 1, 0))
+    with pytest.raises(ValueError):
+        sim.set_mesh_motion(4, (0, 0, 0), (0, 1, 0))
+    sim.set_mesh_motion(4, (0, 0, 0), (0, 0, 0))  # at rest: allowed
+    sim.set_mesh_motion(0, (0, 0, 0), (0, 0, 0))  # stopping one frees a slot
+    sim.set_mesh_motion(5, (0, 0, 0), (1, 0, 0))
+    with pytest.raises(IndexError):
+        sim.set_mesh_motion(6, (0, 0, 0), (1, 0, 0))
+
+
+def test_every_mutator_bumps_the_version():
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    from rusty_particles_amd import scenes
+    calls = [
+        lambda: sim.add_particle(0, 0, 0, 0.01, 1.0),
+        lambda: sim.add_particles([[0.1, 0, 0]], 0.01, 1.0),
+        lambda: sim.set_particle_velocity(0, 1, 2, 3),
+        lambda: sim.set_particle_angular_velocity(0, 1, 2, 3),
+        lambda: sim.set_particle_material(1e7, 0.3, 2500, 0.5, 0.5),
+        lambda: sim.set_wall_material(1e9, 0.3, 7800, 0.5, 0.5),
+        lambda: sim.set_periodic_boundary(True, False, False),
+        lambda: sim.set_contact_models("hertz", "coulomb"),
+        lambda: sim.set_gravity(0, -1, 0),
+        lambda: sim.set_engine_tuning(0.0, 0.0),
+        lambda: sim.set_jkr_solver("shortcut"),
+        lambda: sim.add_mesh_arrays(*scenes.box_mesh((-1, -1, -1), (1, 1, 1))),
+        lambda: sim.set_mesh_motion(0, (0, 0, 0), (0, 1, 0)),
+    ]
+    for f in calls:
+        v = sim._version
+        f()
+        assert sim._version > v
+
+
+def test_overlap_warning_once_per_counted_step(capsys):
+    class FakeEngine:
+        def __init__(self):
+            self.k = 0
+
+        def stats(self):
+            return {"overlap_steps": self.k}
+
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    eng = FakeEngine()
+    eng.k = 3
+    sim._warn_overlap(eng)
+    eng.k = 4
+    sim._warn_overlap(eng)
+    sim._warn_overlap(eng)
+    err = capsys.readouterr().err.strip().splitlines()
+    assert err == [simulation.OVERLAP_WARNING] * 4
+    assert err[0] == "Warning: Excessive particle overlap detected (>20% of radius). Simulation may be unstable."

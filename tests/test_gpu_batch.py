@@ -198,3 +198,43 @@ def test_ensemble_of_simulations_matches_members_alone():
         sa, sb = a.state(), b.state()
         for k in ("x", "v", "w", "a", "tau", "rx", "rv", "rw"):
             assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_ensemble_reuploads_members_changed_between_runs():
+    """A member edited between two Ensemble runs (velocity set, a particle
+    added, material changed, or stepped on its own) is stepped from its new
+    state: the ensemble ends where the members end when each is stepped alone
+    with the same edits."""
+    from rusty_particles_amd import Ensemble, Simulation
+
+    def make(seed, mu, gamma, counts):
+        sc = dense_scene(counts=counts, seed=seed)
+        s = Simulation.create(1e-5, -0.06, -0.06, -0.06, 0.06, 0.06, 0.06, 0)
+        s.set_contact_models("sjkr", "coulomb")
+        s.set_particle_material(1e7, 0.3, 2500.0, mu, 0.5, gamma)
+        s.add_particles(sc["x"], sc["radius"], sc["mass"])
+        return s
+
+    spec = [(4, 0.3, 0.01, (4, 5, 6)), (5, 0.6, 0.0, (6, 4, 4)), (6, 0.2, 0.02, (3, 4, 7))]
+    alone = [make(*a) for a in spec]
+    together = [make(*a) for a in spec]
+    for s in alone:
+        s.run(2e-4)
+    ens = Ensemble(together)
+    ens.run(2e-4)
+
+    def edit(group):
+        group[0].set_particle_velocity(3, 0.1, -0.2, 0.05)
+        group[1].add_particle(0.0, 0.04, 0.0, 0.005, scenes.sphere_mass(np.array([0.005]), 2500.0)[0])
+        group[2].set_particle_material(1e7, 0.3, 2500.0, 0.7, 0.5, 0.015)
+        group[2].run(1e-4)  # 10 steps on its own engine
+
+    edit(alone)
+    edit(together)
+    for s in alone:
+        s.run(3e-4)
+    ens.run(3e-4)
+    for a, b in zip(alone, together):
+        sa, sb = a.state(), b.state()
+        for k in ("x", "v", "w", "a", "tau", "rx", "rv", "rw"):
+            assert np.array_equal(sa[k], sb[k]), k
