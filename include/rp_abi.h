@@ -267,6 +267,25 @@ rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
 rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
+/* Cells of the broad phase, for checking them (no counterpart in the
+ * reference, whose Grid keeps them private, src/grid.rs:30-46).  With
+ * rebuild != 0 the neighbour lists are first rebuilt from the current
+ * positions (results never depend on when that happens).  Per particle in
+ * API order: ref_cell = the reference grid's cell index of its position
+ * (grid.rs:30-40: coords = floor((x - bounds_min) / ref_cell_size), range
+ * check against ceil(size / ref_cell_size) + 1 cells per axis, index
+ * cx + cy dx + cz dx dy; -1 outside the grid), fine_key = the engine's own
+ * sort key of the current position (instance * cells + cell number).
+ * slot_api lists, in device slot order, the API index of each particle: the
+ * permutation of the last rebuild, which sorts the particles by the key of
+ * their position at that rebuild, ties by API index.  grid (22 doubles):
+ * origin[3], 1/cell[3], cells[3], and per axis the cell-number terms
+ * shift[3], hi[3], lo[3] (cell number of fine coordinate c on axis k =
+ * (c >> shift) * hi + (c & (2^shift - 1)) * lo, summed over the axes),
+ * periodic[3], cells per instance.  Any pointer may be NULL. */
+rp_status rp_engine_cells(rp_engine* e, int32_t rebuild, int64_t* ref_cell, uint32_t* fine_key,
+                          int64_t* slot_api, double* grid);
+
 /* Autotuner (no counterpart in the reference): from the current state,
  * times each candidate Verlet skin (the current one, then 0.2 .. 0.005 of
  * the largest radius, never below the automatic skin's motion bound nor the

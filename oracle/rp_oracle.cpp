@@ -133,6 +133,7 @@ class SimBase {
   virtual int overlap_flag() const = 0;
   virtual uint64_t jkr_failures() const = 0;
   virtual void set_fast(bool on) = 0;
+  virtual void cell_ids(int64_t* out) const = 0;
 };
 
 template <class R>
@@ -552,9 +553,18 @@ class Sim : public SimBase {
   std::vector<int64_t> fkey_, fidx_;  // fine-cell key and particle index, sorted by (key, index)
   std::vector<int64_t> fpkey_;        // fine-cell key of particle i
   std::unordered_map<int64_t, std::vector<int32_t>> ftri_;
+  double bsz_[3]{};  // box size (minimum image of the pre-test)
+  bool adh_{false};   // adhesive cutoff margin (JKR, sJKR)
 
  public:
   void set_fast(bool on) override { fast_ = on; }
+  // grid.rs:30-40 for every particle of the current state (-1: outside the grid)
+  void cell_ids(int64_t* out) const override {
+    for (size_t i = 0; i < ps_.size(); ++i) {
+      size_t idx;
+      out[i] = grid_.cell_index(ps_[i].x, &idx) ? (int64_t)idx : -1;
+    }
+  }
 
  private:
   int64_t fcoord(double x, int k) const {
@@ -587,6 +597,8 @@ class Sim : public SimBase {
     const bool adhesive = normal_ == RP_NORMAL_JKR || normal_ == RP_NORMAL_SJKR;
     const double cs = (adhesive ? 1.5 : 1.0) * 2.0 * rmax * 1.001 + 1e-12;
     const double bsz[3] = {(double)(bmax_.x - bmin_.x), (double)(bmax_.y - bmin_.y), (double)(bmax_.z - bmin_.z)};
+    for (int k = 0; k < 3; ++k) bsz_[k] = bsz[k];
+    adh_ = adhesive;
     const double bmn[3] = {(double)bmin_.x, (double)bmin_.y, (double)bmin_.z};
     for (int k = 0; k < 3; ++k) {
       if (periodic_[k]) {
@@ -706,6 +718,24 @@ class Sim : public SimBase {
           for (auto it = rg.first; it != rg.second; ++it) {
             const int64_t j = fidx_[it - fkey_.begin()];
             if (j == i) continue;
+            // conservative pre-test: minimum-image distance against the pair's
+            // cutoff widened by 1e-6 (a superset of the pair test in either
+            // precision); the visit itself evaluates the exact predicate
+            {
+              const double xj[3] = {(double)ps_[j].x.x, (double)ps_[j].x.y, (double)ps_[j].x.z};
+              double d2 = 0.0;
+              for (int k = 0; k < 3; ++k) {
+                double d = x[k] - xj[k];
+                if (periodic_[k]) {
+                  const double L = bsz_[k];
+                  if (d > 0.5 * L) d -= L;
+                  else if (d < -0.5 * L) d += L;
+                }
+                d2 += d * d;
+              }
+              const double cut = (adh_ ? 1.5 : 1.0) * ((double)p.r + (double)ps_[j].r) * (1.0 + 1e-6) + 1e-300;
+              if (!(d2 < cut * cut) && std::isfinite(d2)) continue;
+            }
             int32_t cj[3];
             grid_.coords(ps_[j].x, cj);
             bool vis = true;
@@ -796,6 +826,7 @@ void rpo_get_forces(void* s, double* f3, double* t3, double* fscale) {
 }
 int rpo_overlap_flag(void* s) { return static_cast<SimBase*>(s)->overlap_flag(); }
 uint64_t rpo_jkr_failures(void* s) { return static_cast<SimBase*>(s)->jkr_failures(); }
+void rpo_cell_ids(void* s, int64_t* out) { static_cast<SimBase*>(s)->cell_ids(out); }
 void rpo_set_fast(void* s, int on) { static_cast<SimBase*>(s)->set_fast(on != 0); }
 void rpo_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);

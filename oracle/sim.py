@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
     L.rpo_jkr_failures.argtypes = [C.c_void_p]
     L.rpo_jkr_failures.restype = C.c_uint64
     L.rpo_set_fast.argtypes = [C.c_void_p, C.c_int]
+    L.rpo_cell_ids.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     L.rpo_set_threads.argtypes = [C.c_int]
     L.rpo_max_threads.restype = C.c_int
     L.rpo_normal_law.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double,
@@ -217,6 +218,12 @@ class OracleSimulation:
     def set_cell_size(self, cs):
         """Reference-grid cell (1.0 m as shipped, simulation.rs:66)."""
         self._L.rpo_set_cell_size(self._h, float(cs))
+
+    def cell_ids(self) -> np.ndarray:
+        """The reference grid's cell index of every particle (grid.rs:30-40), -1 outside."""
+        out = np.zeros(int(self._L.rpo_count(self._h)), dtype=np.int64)
+        self._L.rpo_cell_ids(self._h, _p(out, C.c_int64))
+        return out
 
     def set_fast(self, on=True):
         """Fast candidate enumeration: the same visits in the same order as the

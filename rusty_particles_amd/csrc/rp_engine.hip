@@ -1639,6 +1639,29 @@ __global__ void k_shift_labels(Ctl* ctl, unsigned long long by) {
   ctl->label_base += by;
 }
 
+// Cells of every slot (rp_engine_cells): the reference-grid cell index of
+// grid.rs:30-40 (coords = floor((x - origin) / cs) as i32, range check,
+// cx + cy dx + cz dx dy; -1 outside the grid) and the engine's fine-cell key
+// as k_keys computes it.  Removed slots: -2 and UINT32_MAX.
+__global__ void __launch_bounds__(kBlock) k_cells(const Ctl* ctl, KParams P, Dyn S, Stat T, long long* ref_id,
+                                                  uint32_t* key) {
+  const int n = ctl->n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (T.flags[i] & FLAG_DEAD) {
+    ref_id[i] = -2;
+    key[i] = 0xffffffffu;
+    return;
+  }
+  const dv3 x = ldx(S.xr, i);
+  const int c0 = ref_coord<0>(x.x, P), c1 = ref_coord<1>(x.y, P), c2 = ref_coord<2>(x.z, P);
+  const bool in = c0 >= 0 && c1 >= 0 && c2 >= 0 && c0 < P.ref_dims[0] && c1 < P.ref_dims[1] && c2 < P.ref_dims[2];
+  ref_id[i] = in ? (long long)c0 + (long long)c1 * P.ref_dims[0] + (long long)c2 * P.ref_dims[0] * P.ref_dims[1]
+                 : -1ll;
+  key[i] = (uint32_t)T.inst[i] * P.ncells +
+           cell_linear(P, fine_coord(x.x, 0, P, false), fine_coord(x.y, 1, P, false), fine_coord(x.z, 2, P, false));
+}
+
 __global__ void k_request_rebuild(Ctl* ctl, int par) {
   ctl->rebuild[par] = 1;
   ctl->rebuild[par ^ 1] = 0;
@@ -3046,6 +3069,69 @@ rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3, double*
   if (fixed && ns > 0) {
     HIP_TRY(hipMemcpy(fl.data(), cst.flags, (size_t)ns, hipMemcpyDeviceToHost));
     for (size_t q = 0; q < order.size(); ++q) fixed[q] = (fl[order[q]] & FLAG_FIXED) ? 1 : 0;
+  }
+  return RP_OK;
+}
+
+rp_status rp_engine_cells(rp_engine* e, int32_t rebuild, int64_t* ref_cell, uint32_t* fine_key, int64_t* slot_api,
+                          double* grid) {
+  if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
+  if (!e->have_particles) return fail(RP_ERR_STATE, "no particles uploaded");
+  rp_status st = prepare(e);
+  if (st) return st;
+  if (rebuild) {
+    // a rebuild never changes a result: the lists only have to be supersets
+    hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(1), 0, e->stream, e->d_ctl, (int)(e->chain_no & 1));
+    if ((st = launch_rebuild(e))) return st;
+    int clear[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&e->d_ctl->rebuild[0], clear, sizeof(clear), hipMemcpyHostToDevice, e->stream));
+  }
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  if ((st = slot_order(e, order, cur, cst, ns))) return st;
+  long long* d_ref = nullptr;
+  uint32_t* d_key = nullptr;
+  HIP_TRY(hipMalloc(&d_ref, (size_t)std::max(ns, 1) * sizeof(long long)));
+  HIP_TRY(hipMalloc(&d_key, (size_t)std::max(ns, 1) * sizeof(uint32_t)));
+  std::vector<long long> ref((size_t)std::max(ns, 1));
+  std::vector<uint32_t> key((size_t)std::max(ns, 1));
+  std::vector<uint32_t> sid((size_t)std::max(ns, 1));
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_cells, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, e->stream, e->d_ctl, e->kp, cur,
+                       cst, d_ref, d_key);
+    HIP_TRY(hipMemcpyAsync(ref.data(), d_ref, (size_t)ns * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(key.data(), d_key, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(sid.data(), cst.sid, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  (void)hipFree(d_ref);
+  (void)hipFree(d_key);
+  // API index of each slot (order lists the alive slots by stable id)
+  std::vector<int64_t> api((size_t)std::max(ns, 1), -1);
+  for (size_t q = 0; q < order.size(); ++q) api[(size_t)order[q]] = (int64_t)q;
+  for (size_t q = 0; q < order.size(); ++q) {
+    if (ref_cell) ref_cell[q] = ref[(size_t)order[q]];
+    if (fine_key) fine_key[q] = key[(size_t)order[q]];
+  }
+  if (slot_api) {
+    size_t o = 0;
+    for (int i = 0; i < ns; ++i)
+      if (api[(size_t)i] >= 0) slot_api[o++] = api[(size_t)i];
+  }
+  if (grid) {
+    const KParams& K = e->kp;
+    for (int k = 0; k < 3; ++k) {
+      grid[k] = K.go[k];
+      grid[3 + k] = K.ginv[k];
+      grid[6 + k] = (double)K.gn[k];
+      grid[9 + k] = (double)K.gsh[k];
+      grid[12 + k] = (double)K.ghi[k];
+      grid[15 + k] = (double)K.glo[k];
+      grid[18 + k] = (double)K.periodic[k];
+    }
+    grid[21] = (double)K.ncells;
   }
   return RP_OK;
 }

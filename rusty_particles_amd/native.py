@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
     L.rp_engine_set_record.argtypes = [_eng, C.c_int]
     L.rp_stl_read.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), _fp, _fp]
     L.rp_engine_pairs.argtypes = [_eng, _i64p, _i64p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rp_engine_cells.argtypes = [_eng, C.c_int32, _i64p, C.POINTER(C.c_uint32), _i64p, _dp]
     L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
     L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
     L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
@@ -73,7 +74,7 @@ def lib() -> C.CDLL:
     for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_set_mesh_motion", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
-                 "rp_stl_read", "rp_engine_pairs", "rp_engine_stats", "rp_engine_set_timing",
+                 "rp_stl_read", "rp_engine_pairs", "rp_engine_cells", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
                  "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph",
                  "rp_engine_autotune", "rp_engine_set_tiling", "rp_engine_get_tiling"):
@@ -244,6 +245,23 @@ class Engine:
         _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
                                        C.byref(tot)))
         return offs, js[: tot.value]
+
+    def cells(self, rebuild: bool = True) -> dict:
+        """Broad-phase cells (rp_engine_cells): per particle in API order the
+        reference-grid cell index (-1 outside) and the engine's fine-cell key;
+        the slot -> API permutation of the last rebuild (forced first when
+        rebuild); the fine grid's geometry."""
+        n = self.count()
+        ref = np.zeros(n, dtype=np.int64)
+        key = np.zeros(n, dtype=np.uint32)
+        perm = np.zeros(n, dtype=np.int64)
+        grid = np.zeros(22)
+        _check(self._L.rp_engine_cells(self._h, 1 if rebuild else 0, _ptr(ref, C.c_int64), _ptr(key, C.c_uint32),
+                                       _ptr(perm, C.c_int64), _ptr(grid)))
+        g = {"origin": grid[0:3], "inv_cell": grid[3:6], "cells": grid[6:9].astype(int),
+             "shift": grid[9:12].astype(int), "hi": grid[12:15].astype(int), "lo": grid[15:18].astype(int),
+             "periodic": grid[18:21].astype(int), "ncells": int(grid[21])}
+        return {"ref_cell": ref, "fine_key": key, "slot_api": perm, "grid": g}
 
     def autotune(self, steps: int = 20) -> tuple[float, float]:
         """Pick the fastest Verlet skin and cell size for the current state,

@@ -424,7 +424,8 @@ def drum_workload(n: int = 16_000_000, seed: int = 0xD3E50005, fill: float = 0.3
     return p, x, r, m, [(v9, n3)]
 
 
-def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int = 2000, nmax: int = 50000):
+def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int = 2000, nmax: int = 50000,
+                            spacing: float = 0.0065):
     """Ensemble of `ninst` independent simulations for a material sweep:
     N_k ~ U{nmin..nmax} spheres, r ~ U[1, 3] mm, rho 2500, per-instance
     friction mu ~ U[0.1, 0.8], restitution e ~ U[0.2, 0.9], surface energy
@@ -437,7 +438,7 @@ def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int 
     r, m, meshes)."""
     rng = np.random.default_rng(seed)
     counts = rng.integers(nmin, nmax + 1, ninst)
-    spacing, side = 0.0065, 0.2
+    side = 0.2
     per_axis = int(side / spacing) - 1            # 31 sites across, 0.2 m
     layers = int(np.ceil(nmax / per_axis ** 2))   # tallest instance
     origin = np.array([0.5 * (side - (per_axis - 1) * spacing), 0.005, 0.5 * (side - (per_axis - 1) * spacing)])
@@ -457,7 +458,7 @@ def batch_ensemble_workload(ninst: int = 256, seed: int = 0xD3E50004, nmin: int 
     x = np.concatenate(xs)
     r = np.concatenate(rs)
     m = sphere_mass(r, 2500.0)
-    top = 0.45
+    top = max(0.45, float(x[:, 1].max()) + 0.01)
     p = abi.default_params(5e-6, (-0.01, -0.01, -0.01), (side + 0.01, top + 0.01, side + 0.01))
     p.normal_model = abi.RP_NORMAL_SJKR
     p.tangential_model = abi.RP_TANGENTIAL_COULOMB
@@ -499,3 +500,40 @@ def jkr_ensemble_workload(ninst: int = 64, seed: int = 0xD3E50006, nmin: int = 5
     p.tangential_model = abi.RP_TANGENTIAL_COULOMB
     p.particle = abi.RpMaterial(1e8, 0.3, 1500.0, 0.3, 0.5, 0.05)
     return p, [int(c) for c in counts], mats, x, r, m, v
+
+
+def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, spacing: float = 1.15e-3,
+                       clear: float = 0.95):
+    """The capacity run's spheres, materials and drum (drum_workload: r ~
+    U[0.4, 0.8] mm, JKR gamma 0.01 + Coulomb, E=5e7, 16,384-triangle drum
+    turning at 30 rpm) at a size the oracle follows: a 6 cm drum whose
+    lower part holds a jittered cubic lattice of n spheres, spacing 1.15 mm
+    (below most radius sums: compressed adhesive contacts, many with a JKR
+    root), the outer spheres pressing on the drum's side (centres 0.95 r
+    from it).  Returns (params, x, r, m, meshes); mesh 0 turns with
+    DRUM_OMEGA about DRUM_ORIGIN."""
+    rng = np.random.default_rng(seed)
+    rmin, rmax = 0.4e-3, 0.8e-3
+    length = 0.02
+    g = np.arange(-radius, radius + 1e-12, spacing)
+    zs = np.arange(-0.5 * length + rmax, 0.5 * length - rmax + 1e-12, spacing)
+    X, Y, Z = np.meshgrid(g, g, zs, indexing="ij")
+    sites = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    sites += rng.uniform(-4e-6, 4e-6, sites.shape)
+    rad = rng.uniform(rmin, rmax, sites.shape[0])
+    rho = np.sqrt(sites[:, 0] ** 2 + sites[:, 1] ** 2)
+    ok = rho <= radius - clear * rad
+    sites, rad = sites[ok], rad[ok]
+    order = np.lexsort((sites[:, 2], sites[:, 0], sites[:, 1]))[:n]  # bottom layers first
+    if order.size < n:
+        raise RuntimeError(f"the test drum holds only {order.size} of {n} sites")
+    x, r = sites[order], rad[order]
+    m = sphere_mass(r, 2500.0)
+    v9, n3 = drum_mesh(radius, length)
+    pad = 0.002
+    p = abi.default_params(2e-6, (-radius - pad, -radius - pad, -0.5 * length - pad),
+                           (radius + pad, radius + pad, 0.5 * length + pad))
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(5e7, 0.3, 2500.0, 0.5, 0.5, 0.01)
+    return p, x, r, m, [(v9, n3)]

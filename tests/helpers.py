@@ -148,3 +148,36 @@ def jkr_dimers(d_values, r0=50e-6, E=1e8, nu=0.3, gamma=0.05, rho=1500.0, seed=5
     v = rng.normal(0.0, vel, (2 * npair, 3))
     return dict(x=x, radius=rad, mass=m, v=v, w=np.zeros((2 * npair, 3)), fixed=np.zeros(2 * npair, np.uint8),
                 rstar=rstar, t=t, s=s, d=d_values)
+
+
+def fine_keys(x, grid):
+    """The engine's fine-cell key of positions x from its grid geometry
+    (rp_engine_cells): floor((x - origin) * (1 / cell)) per axis, wrapped on
+    periodic axes, clamped to the grid on the others, then the cell number
+    sum_k ((c >> shift_k) * hi_k + (c & (2^shift_k - 1)) * lo_k)."""
+    key = np.zeros(x.shape[0], dtype=np.int64)
+    for k in range(3):
+        q = np.floor((x[:, k] - grid["origin"][k]) * grid["inv_cell"][k])
+        q = np.where(np.isnan(q), 0.0, np.clip(q, -1e9, 1e9)).astype(np.int64)
+        n = int(grid["cells"][k])
+        c = np.mod(q, n) if grid["periodic"][k] else np.clip(q, 0, n - 1)
+        sh = int(grid["shift"][k])
+        key += (c >> sh) * int(grid["hi"][k]) + (c & ((1 << sh) - 1)) * int(grid["lo"][k])
+    return key
+
+
+def assert_cells_equal(e, o):
+    """Cell ids and the sort permutation (rp_engine_cells after a forced
+    rebuild) against the oracle and the grid formulas, bit for bit:
+    reference-grid cell index of every particle = the oracle's
+    (grid.rs:30-40); fine keys = the host formula on the engine's own
+    positions; slot order = a stable sort of the API indices by key."""
+    c = e.cells(rebuild=True)
+    ref = o.cell_ids()
+    assert np.array_equal(c["ref_cell"], ref), "reference-grid cell ids differ"
+    x = e.download()["x"]
+    key = fine_keys(x, c["grid"])
+    assert np.array_equal(c["fine_key"].astype(np.int64), key), "fine-cell keys differ"
+    perm = np.lexsort((np.arange(key.size), key))
+    assert np.array_equal(c["slot_api"], perm), "sort permutation differs"
+    return c
