@@ -66,8 +66,26 @@ enum {
 };
 
 /* Semantics mode.  PARITY reproduces the reference's step_cpu exactly,
- * quirks included (shear modulus placeholder 0, material.rs:49).  */
-enum { RP_MODE_PARITY = 0 };
+ * quirks included (shear modulus placeholder 0, material.rs:49).
+ * PHYSICS_FIX (no counterpart in the reference) corrects what the reference
+ * leaves as placeholders or gets wrong, keeping everything else:
+ *   - shear modulus G = E / (2 (1 + nu)) (material.rs:52-54) and the Mindlin
+ *     effective 1/G* = (2 - nu1)/G1 + (2 - nu2)/G2 (material.rs:39-50 returns 0);
+ *   - Mindlin kt = 8 G* sqrt(R* delta) only for delta > 0 (physics.rs:452
+ *     takes the root of a negative overlap: NaN with JKR/sJKR margins);
+ *   - tangential history per directed contact, kept while the spring is
+ *     loaded and dropped when the contact opens (the reference keys one entry
+ *     per unordered pair that both particles add to, never removed,
+ *     simulation.rs:397-399), projected onto the current tangent plane each
+ *     step; slip direction for any non-zero trial force (physics.rs:425's
+ *     1e-6 N threshold);
+ *   - the closest point of a triangle with Eberly's normalised edge
+ *     parameters (mesh.rs:63-113 clamps unnormalised ones to [0, a] / [0, c]);
+ *   - every pair inside the cutoff acts once: no reference-grid visibility
+ *     or double visits (grid.rs:48-87 depends on a 1 m cell).
+ * Walls keep Coulomb friction (simulation.rs:512-514).  Single engines only
+ * (not rp_engine_upload_batch). */
+enum { RP_MODE_PARITY = 0, RP_MODE_PHYSICS_FIX = 1 };
 
 /* Material, field order as Material::new (src/material.rs:12) and the
  * Python setter (src/lib.rs:109). */

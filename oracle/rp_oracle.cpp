@@ -30,11 +30,21 @@
 
 namespace rpo {
 
+// One directed contact's tangential history in the physics-fix mode: the
+// partner's stable id (its insertion index) and the stored displacement.
+template <class R>
+struct HistEntry {
+  uint32_t key;
+  V3<R> xi;
+};
+
 template <class R>
 struct Particle {
   V3<R> x, v, a, rx, rv, w, tau, rw;
   R r, m;
   uint8_t fixed;
+  uint32_t sid;                     // stable id = insertion index
+  std::vector<HistEntry<R>> hist;   // physics-fix mode: sorted by key
 };
 
 // grid.rs:4-87
@@ -134,6 +144,8 @@ class SimBase {
   virtual uint64_t jkr_failures() const = 0;
   virtual void set_fast(bool on) = 0;
   virtual void cell_ids(int64_t* out) const = 0;
+  virtual size_t history_total() const = 0;
+  virtual void history(int64_t* offsets, int64_t* partner, double* xi3) const = 0;
 };
 
 template <class R>
@@ -153,6 +165,8 @@ class Sim : public SimBase {
     tangential_ = p.tangential_model;
     pm_ = mat(p.particle);
     wm_ = mat(p.wall);
+    fix_ = p.mode == RP_MODE_PHYSICS_FIX;
+    gstar_pp_ = effective_shear_fix(pm_.E, pm_.nu, pm_.E, pm_.nu);
     g_ = v3<R>(R(p.gravity[0]), R(p.gravity[1]), R(p.gravity[2]));
   }
 
@@ -177,6 +191,8 @@ class Sim : public SimBase {
       q.r = R(r[i]);
       q.m = R(m[i]);
       q.fixed = fixed ? (fixed[i] ? 1 : 0) : 0;
+      q.sid = (uint32_t)i;
+      q.hist.clear();
     }
   }
 
@@ -416,7 +432,7 @@ class Sim : public SimBase {
     int flag = 0;
     uint64_t fails = 0;
 
-    if (fast_) build_fast(n);
+    if (fast_ || fix_) build_fast(n);
 
     // 3. per-particle gather (simulation.rs:317-526)
 #pragma omp parallel reduction(| : flag) reduction(+ : fails)
@@ -430,6 +446,7 @@ class Sim : public SimBase {
         V3<R> f = v3<R>(R(0), R(0), R(0));
         V3<R> t = v3<R>(R(0), R(0), R(0));
         R fs = R(0);
+        std::vector<HistEntry<R>> nhist;  // physics-fix history written this step
         // one visit of j by the grid walk (simulation.rs:322-460)
         auto visit = [&](int64_t j) {
           if (i == j) return;
@@ -468,7 +485,20 @@ class Sim : public SimBase {
           R mstar = effective_mass(p.m, o.m);
           V3<R> fn = normal_force(overlap, nrm, vrel, estar_pp, rstar, mstar, gamma_pp, false, &fails);
           R fnmag = length(fn);
-          V3<R> ft = tangential(fnmag, overlap, vrel, nrm, rstar);
+          V3<R> ft;
+          if (fix_ && tangential_ != RP_TANGENTIAL_COULOMB) {
+            // physics-fix history of the directed contact (i, j), keyed by j's stable id
+            V3<R> xi = v3<R>(R(0), R(0), R(0));
+            auto it = std::lower_bound(p.hist.begin(), p.hist.end(), o.sid,
+                                       [](const HistEntry<R>& h, uint32_t k) { return h.key < k; });
+            if (it != p.hist.end() && it->key == o.sid) xi = it->xi;
+            const R kt = tangential_ == RP_TANGENTIAL_MINDLIN ? mindlin_kt_fix(gstar_pp_, rstar, overlap)
+                                                              : gstar_pp_ * rstar;
+            ft = spring_coulomb_fix(fnmag, vrel, nrm, xi, kt, pm.mu, dt);
+            if (kt > R(0) && (xi.x != R(0) || xi.y != R(0) || xi.z != R(0))) nhist.push_back({o.sid, xi});
+          } else {
+            ft = tangential(fnmag, overlap, vrel, nrm, rstar);
+          }
           V3<R> fc = fn + ft;
           f = f + fc;
           t = t + cross(ra, fc);
@@ -480,14 +510,15 @@ class Sim : public SimBase {
           const Triangle<R>& tr = tris_[ti];
           R pen;
           V3<R> nrm;
-          if (!intersect_sphere(tr, p.x, p.r, &pen, &nrm)) return;
+          if (!(fix_ ? intersect_sphere_fix(tr, p.x, p.r, &pen, &nrm) : intersect_sphere(tr, p.x, p.r, &pen, &nrm)))
+            return;
           if (!flag && pen > R(0.2) * p.r) flag = 1;
           V3<R> ra = (-nrm) * p.r;
           V3<R> vrel = p.v + cross(p.w, ra);
           const std::array<double, 6>& mo = motion_[tri_mesh_[ti]];
           if (mo[3] != 0.0 || mo[4] != 0.0 || mo[5] != 0.0) {
             // a moving wall's surface velocity at the contact (closest) point
-            const V3<R> cp = closest_point(tr, p.x);
+            const V3<R> cp = fix_ ? closest_point_fix(tr, p.x) : closest_point(tr, p.x);
             const V3<R> org = v3<R>(R(mo[0]), R(mo[1]), R(mo[2]));
             vrel = vrel - cross(v3<R>(R(mo[3]), R(mo[4]), R(mo[5])), cp - org);
           }
@@ -499,7 +530,13 @@ class Sim : public SimBase {
           t = t + cross(ra, fc);
           fs += length(fc);
         };
-        if (!fast_) {
+        if (fix_) {
+          // physics-fix mode: every pair inside the cutoff once, ascending index
+          fix_walk(i, nb);
+          for (int64_t j : nb) visit(j);
+          for (int32_t ti : fast_tris(i)) visit_tri((size_t)ti);
+          ps_[i].hist.swap(nhist);
+        } else if (!fast_) {
           grid_.neighbours(p.x, per, nb);
           for (int64_t j : nb) visit(j);
           for (size_t ti = 0; ti < tris_.size(); ++ti) visit_tri(ti);
@@ -547,6 +584,8 @@ class Sim : public SimBase {
   // cell lists the triangles whose grown box of that set meets it, in file
   // order.
   bool fast_{false};
+  bool fix_{false};   // RP_MODE_PHYSICS_FIX
+  R gstar_pp_{};      // physics-fix effective shear modulus, particle-particle
   double fcs_[3]{}, forg_[3]{};
   int64_t fn_[3]{1, 1, 1};
   bool fall_[3]{};
@@ -558,6 +597,29 @@ class Sim : public SimBase {
 
  public:
   void set_fast(bool on) override { fast_ = on; }
+  size_t history_total() const override {
+    size_t t = 0;
+    for (const auto& q : ps_) t += q.hist.size();
+    return t;
+  }
+  // Physics-fix history in API order: per particle its entries (partner's
+  // API index, displacement), by ascending partner.
+  void history(int64_t* offsets, int64_t* partner, double* xi3) const override {
+    std::vector<std::pair<uint32_t, int64_t>> idx(ps_.size());
+    for (size_t i = 0; i < ps_.size(); ++i) idx[i] = {ps_[i].sid, (int64_t)i};
+    std::sort(idx.begin(), idx.end());
+    int64_t o = 0;
+    for (size_t i = 0; i < ps_.size(); ++i) {
+      offsets[i] = o;
+      for (const auto& h : ps_[i].hist) {
+        auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(h.key, (int64_t)-1));
+        partner[o] = (it != idx.end() && it->first == h.key) ? it->second : -1;
+        st(xi3, (size_t)o, h.xi);
+        ++o;
+      }
+    }
+    offsets[ps_.size()] = o;
+  }
   // grid.rs:30-40 for every particle of the current state (-1: outside the grid)
   void cell_ids(int64_t* out) const override {
     for (size_t i = 0; i < ps_.size(); ++i) {
@@ -763,6 +825,50 @@ class Sim : public SimBase {
     std::sort(walk.begin(), walk.end());
   }
 
+  // Physics-fix mode: the indices j != i whose minimum-image distance passes
+  // the pre-test, ascending (the pair test itself runs in the visit).
+  void fix_walk(int64_t i, std::vector<int64_t>& out) const {
+    out.clear();
+    const Particle<R>& p = ps_[i];
+    const double x[3] = {(double)p.x.x, (double)p.x.y, (double)p.x.z};
+    std::vector<int64_t> all[3];
+    for (int k = 0; k < 3; ++k) {
+      const int64_t c = fcoord(x[k], k);
+      if (fall_[k]) {
+        for (int64_t q = 0; q < fn_[k]; ++q) all[k].push_back(q);
+        continue;
+      }
+      for (int o = -1; o <= 1; ++o) {
+        int64_t q = c + o;
+        if (periodic_[k]) q = (q + fn_[k]) % fn_[k];
+        else if (q < 0 || q >= fn_[k]) continue;
+        all[k].push_back(q);
+      }
+    }
+    for (int64_t a : all[0])
+      for (int64_t b : all[1])
+        for (int64_t c : all[2]) {
+          auto rg = std::equal_range(fkey_.begin(), fkey_.end(), fkey(a, b, c));
+          for (auto it = rg.first; it != rg.second; ++it) {
+            const int64_t j = fidx_[it - fkey_.begin()];
+            if (j == i) continue;
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+              double d = x[k] - (double)(k == 0 ? ps_[j].x.x : (k == 1 ? ps_[j].x.y : ps_[j].x.z));
+              if (periodic_[k]) {
+                if (d > 0.5 * bsz_[k]) d -= bsz_[k];
+                else if (d < -0.5 * bsz_[k]) d += bsz_[k];
+              }
+              d2 += d * d;
+            }
+            const double cut = (adh_ ? 1.5 : 1.0) * ((double)p.r + (double)ps_[j].r) * (1.0 + 1e-6) + 1e-300;
+            if (!(d2 < cut * cut) && std::isfinite(d2)) continue;
+            out.push_back(j);
+          }
+        }
+    std::sort(out.begin(), out.end());
+  }
+
   const std::vector<int32_t>& fast_tris(int64_t i) const {
     static const std::vector<int32_t> none;
     auto it = ftri_.find(fpkey_[i]);
@@ -827,6 +933,10 @@ void rpo_get_forces(void* s, double* f3, double* t3, double* fscale) {
 int rpo_overlap_flag(void* s) { return static_cast<SimBase*>(s)->overlap_flag(); }
 uint64_t rpo_jkr_failures(void* s) { return static_cast<SimBase*>(s)->jkr_failures(); }
 void rpo_cell_ids(void* s, int64_t* out) { static_cast<SimBase*>(s)->cell_ids(out); }
+size_t rpo_history_total(void* s) { return static_cast<SimBase*>(s)->history_total(); }
+void rpo_history(void* s, int64_t* offsets, int64_t* partner, double* xi3) {
+  static_cast<SimBase*>(s)->history(offsets, partner, xi3);
+}
 void rpo_set_fast(void* s, int on) { static_cast<SimBase*>(s)->set_fast(on != 0); }
 void rpo_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);
@@ -951,3 +1061,40 @@ extern "C" int rpo_triangle(int bits, const double* tri12, const double* p, doub
   return tri_law<double>(tri12, p, radius, cp, pen, nrm);
 }
 
+
+// ---- unit-level access to the physics-fix laws ------------------------------
+template <class R>
+static void spring_fix_law(double fn, const double* vrel, const double* n, double* xi, double kt, double mu,
+                           double dt, double* out) {
+  rpo::V3<R> x = rpo::v3<R>(R(xi[0]), R(xi[1]), R(xi[2]));
+  rpo::V3<R> f = rpo::spring_coulomb_fix<R>(R(fn), rpo::v3<R>(R(vrel[0]), R(vrel[1]), R(vrel[2])),
+                                            rpo::v3<R>(R(n[0]), R(n[1]), R(n[2])), x, R(kt), R(mu), R(dt));
+  xi[0] = (double)x.x;
+  xi[1] = (double)x.y;
+  xi[2] = (double)x.z;
+  out[0] = (double)f.x;
+  out[1] = (double)f.y;
+  out[2] = (double)f.z;
+}
+extern "C" void rpo_spring_fix(int bits, double fn, const double* vrel, const double* n, double* xi, double kt,
+                               double mu, double dt, double* out) {
+  if (bits == 32) spring_fix_law<float>(fn, vrel, n, xi, kt, mu, dt, out);
+  else spring_fix_law<double>(fn, vrel, n, xi, kt, mu, dt, out);
+}
+extern "C" double rpo_gstar_fix(double e1, double nu1, double e2, double nu2) {
+  return rpo::effective_shear_fix<double>(e1, nu1, e2, nu2);
+}
+extern "C" double rpo_mindlin_kt_fix(double gstar, double rstar, double overlap) {
+  return rpo::mindlin_kt_fix<double>(gstar, rstar, overlap);
+}
+extern "C" void rpo_closest_point_fix(const double* tri12, const double* p, double* cp) {
+  rpo::Triangle<double> tr;
+  tr.v0 = rpo::v3<double>(tri12[0], tri12[1], tri12[2]);
+  tr.v1 = rpo::v3<double>(tri12[3], tri12[4], tri12[5]);
+  tr.v2 = rpo::v3<double>(tri12[6], tri12[7], tri12[8]);
+  tr.n = rpo::v3<double>(tri12[9], tri12[10], tri12[11]);
+  rpo::V3<double> c = rpo::closest_point_fix(tr, rpo::v3<double>(p[0], p[1], p[2]));
+  cp[0] = c.x;
+  cp[1] = c.y;
+  cp[2] = c.z;
+}

@@ -317,6 +317,123 @@ inline bool intersect_sphere(const Triangle<R>& tr, V3<R> center, R radius, R* p
   return false;
 }
 
+// ---- physics-fix mode (no reference counterpart: corrections of its
+// placeholders, see rp_abi.h RP_MODE_PHYSICS_FIX) -------------------------
+
+// Shear modulus G = E / (2 (1 + nu)) (material.rs:52-54) and the Mindlin
+// effective shear modulus 1/G* = (2 - nu1)/G1 + (2 - nu2)/G2, in place of
+// the placeholder 0 of material.rs:39-50.
+template <class R> inline R shear_modulus(R e, R nu) { return e / (R(2) * (R(1) + nu)); }
+template <class R> inline R effective_shear_fix(R e1, R nu1, R e2, R nu2) {
+  R inv = (R(2) - nu1) / shear_modulus(e1, nu1) + (R(2) - nu2) / shear_modulus(e2, nu2);
+  return R(1) / inv;
+}
+
+// Tangential spring with Coulomb cap and history (linear_spring_coulomb,
+// physics.rs:400-434, with three corrections): the stored displacement is
+// first projected onto the current tangent plane (the contact normal turns
+// with the particles); kt <= 0 (Mindlin at zero or negative overlap, where
+// physics.rs:452 takes the square root of a negative number) carries no
+// force and no history; the slip direction uses any non-zero trial force
+// (the reference's 1e-6 N threshold zeroes the slip force of small grains).
+template <class R>
+inline V3<R> spring_coulomb_fix(R fn, V3<R> vrel, V3<R> n, V3<R>& xi, R kt, R mu, R dt) {
+  if (!(kt > R(0))) {
+    xi = V3<R>{R(0), R(0), R(0)};
+    return xi;
+  }
+  xi = xi - dot(xi, n) * n;
+  V3<R> vt = vrel - dot(vrel, n) * n;
+  xi = xi + vt * dt;
+  V3<R> trial = (-kt) * xi;
+  R tm = length(trial);
+  R fc = mu * fn;
+  if (tm > fc) {
+    V3<R> dir = tm > R(0) ? trial / tm : V3<R>{R(0), R(0), R(0)};
+    V3<R> ft = fc * dir;
+    xi = (-ft) / kt;
+    return ft;
+  }
+  return trial;
+}
+
+// Mindlin stiffness kt = 8 G* sqrt(R* delta) for delta > 0 and 0 otherwise
+// (physics.rs:443-459 without the NaN); the spring law's kt = G* R*
+// (simulation.rs:450).
+template <class R> inline R mindlin_kt_fix(R gstar, R rstar, R overlap) {
+  return overlap > R(0) ? R(8) * gstar * std::sqrt(rstar * overlap) : R(0);
+}
+
+// Closest point of a triangle (Eberly, "Distance Between Point and Triangle
+// in 3D", Geometric Tools): the same regions as mesh.rs:63-113 with each
+// edge parameter divided by its edge's squared length before the clamp to
+// [0, 1], and region 2/6's fallback onto the correct edge.
+template <class R>
+inline V3<R> closest_point_fix(const Triangle<R>& tr, V3<R> p) {
+  V3<R> e0 = tr.v1 - tr.v0;
+  V3<R> e1 = tr.v2 - tr.v0;
+  V3<R> v0 = tr.v0 - p;
+  R a = dot(e0, e0), b = dot(e0, e1), c = dot(e1, e1), d = dot(e0, v0), e = dot(e1, v0);
+  R det = a * c - b * b;
+  R s = b * e - c * d;
+  R t = b * d - a * e;
+  auto edge = [](R num, R den) { return num <= R(0) ? R(0) : (num >= den ? R(1) : num / den); };
+  if (s + t <= det) {
+    if (s < R(0)) {
+      if (t < R(0) && d < R(0)) {  // region 4, towards edge v0-v1
+        s = edge(-d, a);
+        t = R(0);
+      } else {  // regions 3 and 4: edge v0-v2
+        s = R(0);
+        t = edge(-e, c);
+      }
+    } else if (t < R(0)) {  // region 5: edge v0-v1
+      s = edge(-d, a);
+      t = R(0);
+    } else {  // region 0: inside
+      s = s / det;
+      t = t / det;
+    }
+  } else if (s < R(0)) {  // region 2
+    R tmp0 = b + d, tmp1 = c + e;
+    if (tmp1 > tmp0) {  // edge v1-v2
+      s = edge(tmp1 - tmp0, a - R(2) * b + c);
+      t = R(1) - s;
+    } else {  // edge v0-v2
+      s = R(0);
+      t = edge(-e, c);
+    }
+  } else if (t < R(0)) {  // region 6
+    R tmp0 = b + e, tmp1 = a + d;
+    if (tmp1 > tmp0) {  // edge v1-v2
+      t = edge(tmp1 - tmp0, a - R(2) * b + c);
+      s = R(1) - t;
+    } else {  // edge v0-v1
+      t = R(0);
+      s = edge(-d, a);
+    }
+  } else {  // region 1: edge v1-v2
+    s = edge((c + e) - (b + d), a - R(2) * b + c);
+    t = R(1) - s;
+  }
+  return tr.v0 + s * e0 + t * e1;
+}
+
+// intersect_sphere (mesh.rs:20-43) on closest_point_fix
+template <class R>
+inline bool intersect_sphere_fix(const Triangle<R>& tr, V3<R> center, R radius, R* pen, V3<R>* normal) {
+  V3<R> cp = closest_point_fix(tr, center);
+  V3<R> dv = center - cp;
+  R dsq = length_sq(dv);
+  if (dsq < radius * radius) {
+    R dist = std::sqrt(dsq);
+    *pen = radius - dist;
+    *normal = dist > R(1e-6) ? dv / dist : tr.n;
+    return true;
+  }
+  return false;
+}
+
 // Kahan update, simulation.rs:238-242: y = d - res; t = val + y; res = (t - val) - y; val = t
 template <class R>
 inline void kahan(V3<R>& val, V3<R>& res, V3<R> d) {

@@ -57,6 +57,15 @@ def lib() -> C.CDLL:
     L.rpo_jkr_failures.restype = C.c_uint64
     L.rpo_set_fast.argtypes = [C.c_void_p, C.c_int]
     L.rpo_cell_ids.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.rpo_history_total.argtypes = [C.c_void_p]
+    L.rpo_history_total.restype = C.c_size_t
+    L.rpo_history.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp]
+    L.rpo_spring_fix.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
+    L.rpo_gstar_fix.argtypes = [C.c_double] * 4
+    L.rpo_gstar_fix.restype = C.c_double
+    L.rpo_mindlin_kt_fix.argtypes = [C.c_double] * 3
+    L.rpo_mindlin_kt_fix.restype = C.c_double
+    L.rpo_closest_point_fix.argtypes = [_dp, _dp, _dp]
     L.rpo_set_threads.argtypes = [C.c_int]
     L.rpo_max_threads.restype = C.c_int
     L.rpo_normal_law.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double,
@@ -225,6 +234,17 @@ class OracleSimulation:
         self._L.rpo_cell_ids(self._h, _p(out, C.c_int64))
         return out
 
+    def history(self):
+        """Physics-fix tangential history: (offsets[n+1], partner API index,
+        displacement[total, 3]), per particle by ascending partner."""
+        tot = int(self._L.rpo_history_total(self._h))
+        n = int(self._L.rpo_count(self._h))
+        offs = np.zeros(n + 1, dtype=np.int64)
+        part = np.zeros(max(tot, 1), dtype=np.int64)
+        xi = np.zeros((max(tot, 1), 3))
+        self._L.rpo_history(self._h, _p(offs, C.c_int64), _p(part, C.c_int64), _p(xi))
+        return offs, part[:tot], xi[:tot]
+
     def set_fast(self, on=True):
         """Fast candidate enumeration: the same visits in the same order as the
         literal 27-cell walk (see rp_oracle.cpp), for large inputs."""
@@ -352,3 +372,28 @@ def triangle(bits, tri12, p, radius):
     hit = L.rpo_triangle(bits, _p(np.ascontiguousarray(tri12, dtype=np.float64)),
                          _p(np.ascontiguousarray(p, dtype=np.float64)), radius, _p(cp), C.byref(pen), _p(nrm))
     return cp, bool(hit), pen.value, nrm
+
+
+def spring_fix(bits, fn, vrel, n, xi, kt, mu, dt):
+    """Physics-fix tangential law: (force[3], new history[3])."""
+    L = lib()
+    x = np.ascontiguousarray(xi, dtype=np.float64).copy()
+    out = np.zeros(3)
+    L.rpo_spring_fix(bits, fn, _p(np.ascontiguousarray(vrel, dtype=np.float64)),
+                     _p(np.ascontiguousarray(n, dtype=np.float64)), _p(x), kt, mu, dt, _p(out))
+    return out, x
+
+
+def gstar_fix(e1, nu1, e2, nu2):
+    return float(lib().rpo_gstar_fix(e1, nu1, e2, nu2))
+
+
+def mindlin_kt_fix(gstar, rstar, overlap):
+    return float(lib().rpo_mindlin_kt_fix(gstar, rstar, overlap))
+
+
+def closest_point_fix(tri12, p):
+    cp = np.zeros(3)
+    lib().rpo_closest_point_fix(_p(np.ascontiguousarray(tri12, dtype=np.float64)),
+                                _p(np.ascontiguousarray(p, dtype=np.float64)), _p(cp))
+    return cp

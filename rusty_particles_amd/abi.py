@@ -20,6 +20,8 @@ RP_TANGENTIAL_COULOMB = 0
 RP_TANGENTIAL_LSC = 1
 RP_TANGENTIAL_MINDLIN = 2
 RP_MODE_PARITY = 0
+RP_MODE_PHYSICS_FIX = 1
+RP_CELL_ORDER_AUTO, RP_CELL_ORDER_ROW, RP_CELL_ORDER_PENCIL = 0, 1, 2
 
 # Model names accepted by set_contact_models (src/lib.rs:160-180), lower-cased.
 NORMAL_MODELS = {

@@ -502,32 +502,49 @@ def jkr_ensemble_workload(ninst: int = 64, seed: int = 0xD3E50006, nmin: int = 5
     return p, [int(c) for c in counts], mats, x, r, m, v
 
 
-def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, spacing: float = 1.15e-3,
-                       clear: float = 0.95):
+def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, length: float = 0.03):
     """The capacity run's spheres, materials and drum (drum_workload: r ~
     U[0.4, 0.8] mm, JKR gamma 0.01 + Coulomb, E=5e7, 16,384-triangle drum
-    turning at 30 rpm) at a size the oracle follows: a 6 cm drum whose
-    lower part holds a jittered cubic lattice of n spheres, spacing 1.15 mm
-    (below most radius sums: compressed adhesive contacts, many with a JKR
-    root), the outer spheres pressing on the drum's side (centres 0.95 r
-    from it).  Returns (params, x, r, m, meshes); mesh 0 turns with
+    turning at 30 rpm) at a size the oracle follows: a 6 cm drum whose lower
+    part holds n spheres in chains along the drum's axis.  The spheres of one
+    chain share a radius (drawn per chain) and touch their neighbours with
+    overlaps U[0.05, 3] um -- JKR contacts with a root, the range a settled
+    bed has and the surrogate serves; chains are 2.2 r_max apart (inside each
+    other's JKR cut-off, without a root); the lowest chains press 1 um into
+    the drum's side.  Returns (params, x, r, m, meshes); mesh 0 turns with
     DRUM_OMEGA about DRUM_ORIGIN."""
     rng = np.random.default_rng(seed)
     rmin, rmax = 0.4e-3, 0.8e-3
-    length = 0.02
-    g = np.arange(-radius, radius + 1e-12, spacing)
-    zs = np.arange(-0.5 * length + rmax, 0.5 * length - rmax + 1e-12, spacing)
-    X, Y, Z = np.meshgrid(g, g, zs, indexing="ij")
-    sites = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
-    sites += rng.uniform(-4e-6, 4e-6, sites.shape)
-    rad = rng.uniform(rmin, rmax, sites.shape[0])
-    rho = np.sqrt(sites[:, 0] ** 2 + sites[:, 1] ** 2)
-    ok = rho <= radius - clear * rad
-    sites, rad = sites[ok], rad[ok]
-    order = np.lexsort((sites[:, 2], sites[:, 0], sites[:, 1]))[:n]  # bottom layers first
-    if order.size < n:
-        raise RuntimeError(f"the test drum holds only {order.size} of {n} sites")
-    x, r = sites[order], rad[order]
+    pitch = 2.2 * rmax
+    g = np.arange(-radius, radius + 1e-12, pitch)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    cx, cy = X.ravel(), Y.ravel()
+    order = np.lexsort((cx, cy))  # lowest chains first
+    xs, rs = [], []
+    count = 0
+    for q in order:
+        rc = rng.uniform(rmin, rmax)
+        rho = math.hypot(cx[q], cy[q])
+        if rho > radius - rc + 1e-6:
+            continue
+        if rho > 1e-9 and rho > radius - rc - 0.5 * pitch:
+            # press the chain onto the side: 1 um of overlap with the wall
+            s_ = (radius - rc + 1e-6) / rho
+            px, py = cx[q] * s_, cy[q] * s_
+        else:
+            px, py = cx[q], cy[q]
+        z = -0.5 * length + rc + 1e-4
+        while z + rc < 0.5 * length - 1e-4 and count < n:
+            xs.append((px, py, z))
+            rs.append(rc)
+            count += 1
+            z += 2.0 * rc - rng.uniform(0.05e-6, 3e-6)
+        if count >= n:
+            break
+    if count < n:
+        raise RuntimeError(f"the test drum holds only {count} of {n} spheres")
+    x = np.array(xs)
+    r = np.array(rs)
     m = sphere_mass(r, 2500.0)
     v9, n3 = drum_mesh(radius, length)
     pad = 0.002
