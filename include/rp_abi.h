@@ -285,6 +285,16 @@ rp_status rp_stl_read(const char* path, size_t* ntri, float* v9, float* n3);
 rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
                           size_t cap, size_t* total);
 
+/* Tangential history of the physics-fix mode (no counterpart in the
+ * reference, whose Contact map is internal, src/contact.rs:3-18): per
+ * particle in API order, its loaded directed contacts -- partner's API index
+ * (ascending) and stored displacement xi (3 doubles) -- as the next step
+ * will read them.  offsets has n+1 entries; partner and xi3 receive up to cap
+ * entries (xi3: 3 per entry); *total receives the count.  Empty when no
+ * history is kept (parity mode or Coulomb friction). */
+rp_status rp_engine_history(rp_engine* e, int64_t* offsets, int64_t* partner, double* xi3, size_t cap,
+                            size_t* total);
+
 /* Cells of the broad phase, for checking them (no counterpart in the
  * reference, whose Grid keeps them private, src/grid.rs:30-46).  With
  * rebuild != 0 the neighbour lists are first rebuilt from the current

@@ -173,6 +173,20 @@ struct BlockStats {
   unsigned int wall_contacts;  // particle-triangle contacts
   unsigned int jkr_failures;   // JKR residual rejections
   unsigned int overlap;        // excessive-overlap flag
+  unsigned int history;        // physics-fix: directed contacts stored with a history
+};
+
+// Tangential history of the physics-fix mode, one table per slot: the
+// loaded directed contacts of the slot's particle, keyed by the partner's
+// stable id and sorted by it, with the stored displacement.  Entry k of
+// slot i: key[k*cap + i], xi[(c*hcap + k)*cap + i] (c = component), count in
+// cnt[i].  The key is a stable id, so a table stays valid when a rebuild
+// moves its owner to another slot: it moves with the owner.  The force
+// kernel reads one copy and writes the other (like Dyn).
+struct Hist {
+  uint32_t* key;
+  double* xi;
+  int* cnt;
 };
 
 // Everything a kernel needs that is fixed between parameter changes.
@@ -232,6 +246,8 @@ struct KParams {
   int cap;           // slot capacity (plane stride)
   int nbr_cap;       // neighbour list capacity per particle
   int pair_cap;      // recorded pair entries per particle
+  int fix;           // RP_MODE_PHYSICS_FIX
+  int hcap;          // history entries per slot (= nbr_cap when a history is kept)
 };
 
 // ---- small device helpers ---------------------------------------------------
@@ -386,9 +402,25 @@ __device__ inline int xcd_block(int b, int nb) {
 // such a contact adds nothing and is only counted, in a pass of its own.
 // Mindlin propagates its NaN for negative overlaps, and recording needs every
 // pair in order.
-constexpr bool split_pass(int normal, int tang, bool record, int js) {
-  return !record && tang != RP_TANGENTIAL_MINDLIN &&
+// In the physics-fix mode (hist) Mindlin has no NaN: a zero normal force at
+// negative overlap gives zero stiffness, no force and no history.
+constexpr bool split_pass(int normal, int tang, bool record, int js, bool hist = false) {
+  return !record && (tang != RP_TANGENTIAL_MINDLIN || hist) &&
          ((normal == RP_NORMAL_JKR && js >= 1) || normal == RP_NORMAL_SJKR);
+}
+
+// Copies slot `from`'s history table of A to slot `to` of B.
+__device__ inline void copy_hist(const Hist& A, const Hist& B, int from, int to, const KParams& P) {
+  const int c = A.cnt[from];
+  B.cnt[to] = c;
+  const size_t cap = (size_t)P.cap, st = (size_t)P.hcap * cap;
+  for (int k = 0; k < c; ++k) {
+    const size_t a = (size_t)k * cap + (size_t)from, b = (size_t)k * cap + (size_t)to;
+    B.key[b] = A.key[a];
+    B.xi[b] = A.xi[a];
+    B.xi[b + st] = A.xi[a + st];
+    B.xi[b + 2 * st] = A.xi[a + 2 * st];
+  }
 }
 
 // Orders the LDS accesses of one wavefront's lanes around this point (each
@@ -535,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const Ctl* ctl, int par, con
 // particles (sentinel cell) are dropped.
 __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KParams P, Dyn S, Stat T, Dyn D,
                                                          Stat U, const uint32_t* key, const uint32_t* cell_start,
-                                                         const uint32_t* tmp) {
+                                                         const uint32_t* tmp, Hist HS, Hist HD) {
   if (!ctl->rebuild[par]) return;
   const int n = ctl->n;
   const int cap = P.cap;
@@ -563,6 +595,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
     U.flags[d] = T.flags[i];
     U.sid[d] = my;
     U.inst[d] = T.inst[i];
+    if (HS.cnt) copy_hist(HS, HD, i, d, P);  // the history table moves with its owner
   }
 }
 
@@ -571,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
 // i back into the working buffers (W, V) and records the new slot count.
 __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, Dyn D, Stat U, Dyn W, Stat V,
                                                   const uint32_t* cell_start, int* nbr_count, int* nbr,
-                                                  float* xb) {
+                                                  float* xb, Hist HD, Hist HW) {
   if (!ctl->rebuild[par]) return;
   const int n = (int)cell_start[P.ncells_all];  // alive particles after compaction
   const int cap = P.cap;
@@ -670,6 +703,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
     V.flags[i] = U.flags[i];
     V.sid[i] = U.sid[i];
     V.inst[i] = U.inst[i];
+    if (HW.cnt) copy_hist(HD, HW, i, i, P);
   }
   const int wmax = wave_max_i32(local_max);
   if ((threadIdx.x & 63) == 0 && wmax > 0) {
@@ -853,6 +887,8 @@ struct Acc {
   unsigned int sserved, sfall;
   float emax, rmax;
   double esum, rsum;
+  // physics-fix history: cursor and size of the old table, entries written
+  int hp, hold, hn;
 };
 
 // JKR normal force with the exact no-root shortcut (JS >= 1) and the network
@@ -938,10 +974,12 @@ __device__ inline void record_pairs(const KParams& P, const Stat& T, int j, int 
 
 // One particle-particle contact of the gather (simulation.rs:357-460), added
 // `mult` times when the reference's grid walk reaches j more than once.
-template <int NORMAL, int TANG, bool RECORD, int JS>
+// HIST: physics-fix tangential history (tables HR -> HW of slot i).
+template <int NORMAL, int TANG, bool RECORD, int JS, bool HIST>
 __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, double far_pp, const Dyn& S,
                                   const Stat& T, const Own& o, int j, int k, Acc& acc, Ctl* ctl,
-                                  unsigned long long* prow, const SurrDev& sd) {
+                                  unsigned long long* prow, const SurrDev& sd, const Hist& HR, const Hist& HW,
+                                  int i) {
   dv3 xj;
   double rj;
   ldxr(S.xr, j, xj, rj);
@@ -949,17 +987,22 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
   const double dsq = dot(d, d);
   const double rsum = o.r + rj;
   // visible to the reference grid, and reached how often by its 27-cell walk
-  const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
-  if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
-    return;
-  const int mask0 = ref_mask<0>(o.rc0, cj0, P), mask1 = ref_mask<1>(o.rc1, cj1, P),
-            mask2 = ref_mask<2>(o.rc2, cj2, P);
+  // (physics-fix mode: every pair once, as the walk's own cell)
+  int mask0 = 2, mask1 = 2, mask2 = 2;
+  if (!P.fix) {
+    const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+    if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
+      return;
+    mask0 = ref_mask<0>(o.rc0, cj0, P);
+    mask1 = ref_mask<1>(o.rc1, cj1, P);
+    mask2 = ref_mask<2>(o.rc2, cj2, P);
+  }
   const int mult = __popc(mask0) * __popc(mask1) * __popc(mask2);
   if (mult == 0) return;
   const double dist = sqrt(dsq);
   const double overlap = rsum - dist;
   if (overlap > 0.2 * fmin(o.r, rj)) acc.overlap = true;
-  if (NORMAL == RP_NORMAL_JKR && JS >= 1 && TANG != RP_TANGENTIAL_MINDLIN && mc.pp.gamma > 0.0 &&
+  if (NORMAL == RP_NORMAL_JKR && JS >= 1 && (TANG != RP_TANGENTIAL_MINDLIN || HIST) && mc.pp.gamma > 0.0 &&
       overlap < far_pp - 1e-8 * fabs(overlap)) {
     // no root: zero normal force and a rejected root; Coulomb and the spring
     // law with G* = 0 then give a zero tangential force, so the contact adds
@@ -1000,6 +1043,29 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
   dv3 ft;
   if (TANG == RP_TANGENTIAL_COULOMB) {
     ft = coulomb(fnmag, vrel, nrm, mc.mu);
+  } else if constexpr (HIST) {
+    // the stored displacement of (i, j): the old table is sorted by partner
+    // id and the contacts come in ascending id, so a cursor finds it
+    const uint32_t sj = T.sid[j];
+    const size_t cap = (size_t)P.cap, st = (size_t)P.hcap * cap;
+    dv3 xi = zero3();
+    while (acc.hp < acc.hold && HR.key[(size_t)acc.hp * cap + i] < sj) ++acc.hp;
+    if (acc.hp < acc.hold && HR.key[(size_t)acc.hp * cap + i] == sj) {
+      const size_t b = (size_t)acc.hp * cap + i;
+      xi = mk(HR.xi[b], HR.xi[b + st], HR.xi[b + 2 * st]);
+      ++acc.hp;
+    }
+    const double kt = TANG == RP_TANGENTIAL_MINDLIN ? mindlin_kt_fix(mc.gstar_pp, rstar, overlap)
+                                                    : mc.gstar_pp * rstar;
+    ft = spring_coulomb_fix(fnmag, vrel, nrm, xi, kt, mc.mu, P.dt);
+    if (kt > 0.0 && (xi.x != 0.0 || xi.y != 0.0 || xi.z != 0.0)) {
+      const size_t b = (size_t)acc.hn * cap + i;
+      HW.key[b] = sj;
+      HW.xi[b] = xi.x;
+      HW.xi[b + st] = xi.y;
+      HW.xi[b + 2 * st] = xi.z;
+      ++acc.hn;
+    }
   } else {
     // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
     const double kt = TANG == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
@@ -1032,7 +1098,7 @@ template <int NORMAL, int JS>
 __device__ inline void contact_wall(const KParams& P, const MaterialConsts& mc, double far_pw, const Own& o,
                                     const Tri& tr, unsigned int ti, Acc& acc, unsigned int& nwall,
                                     const SurrDev& sd) {
-  const dv3 cp = closest_point(tr, o.x);
+  const dv3 cp = P.fix ? closest_point_fix(tr, o.x) : closest_point(tr, o.x);
   const dv3 dv = sub(o.x, cp);
   const double dsq = dot(dv, dv);
   if (!(dsq < o.r * o.r)) return;
@@ -1097,16 +1163,17 @@ __device__ inline void copy_slot(const Dyn& S, const Dyn& D, int i, int cap) {
 // RECORD: writes the reference-ordered pair entries for rp_engine_pairs.
 // JS: JKR solver (0 exact as the reference, 1 with the no-root shortcut,
 // 2 with the network's radii from k_jkr_collect + the MLP kernel).
-template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH, bool DEAL>
+// HIST: physics-fix mode with a tangential history (reads table HR, writes HW).
+template <int NORMAL, int TANG, bool ADVANCE, bool RECORD, int JS, bool BATCH, bool DEAL, bool HIST>
 __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, int flag_slot, unsigned long long off, KParams P,
                                                   Dyn S, Dyn D, Stat T, const int* nbr_count, const int* nbr,
                                                   const Tri* tris, const uint32_t* tri_start,
                                                   const uint32_t* tri_idx, const float* xb, BlockStats* bstats,
                                                   unsigned long long* pairbuf, int* paircount, SurrDev sd,
-                                                  SurrStats* sstats) {
+                                                  SurrStats* sstats, Hist HR, Hist HW) {
   const int n = ctl->n;
   // zero-force pass (split_pass), dealt over the wave (DEAL) or per lane
-  constexpr bool kSplit = split_pass(NORMAL, TANG, RECORD, JS);
+  constexpr bool kSplit = split_pass(NORMAL, TANG, RECORD, JS, HIST);
   // the rows of this step have been read by the MLP kernel; the next step's
   // collect kernel starts from zero
   if (JS >= 2 && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1123,7 +1190,8 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
     // the host run this step again after a rebuild.
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) copy_slot(S, D, i, P.cap);
-    if (threadIdx.x == 0) bstats[blockIdx.x] = BlockStats{0u, 0u, 0u, 0u};
+    if (HIST && i < n) copy_hist(HR, HW, i, i, P);
+    if (threadIdx.x == 0) bstats[blockIdx.x] = BlockStats{0u, 0u, 0u, 0u, 0u};
     if (JS >= 2 && threadIdx.x == 0) sstats[blockIdx.x] = SurrStats{0u, 0u, 0.f, 0.f, 0.0, 0.0};
     return;
   }
@@ -1147,6 +1215,9 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
   acc.rmax = 0.f;
   acc.esum = 0.0;
   acc.rsum = 0.0;
+  acc.hp = 0;
+  acc.hold = 0;
+  acc.hn = 0;
   bool far = false;
   uint8_t flags = 0;
   Own o{};
@@ -1257,14 +1328,17 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
           atomicOr(&s_def[wv][p], 1ull << k);
           continue;
         }
-        const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
-                  cj2 = ref_coord_fast<2>(xq[q].z, P);
-        if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
-            cj2 >= P.ref_dims[2])
-          continue;
-        const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(s_rc[wv][0][p], cj0, P)) *
-                                                 __popc(ref_mask_fast<1>(s_rc[wv][1][p], cj1, P)) *
-                                                 __popc(ref_mask_fast<2>(s_rc[wv][2][p], cj2, P)));
+        unsigned int mult = 1u;  // physics-fix mode: every pair once
+        if (!P.fix) {
+          const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
+                    cj2 = ref_coord_fast<2>(xq[q].z, P);
+          if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
+              cj2 >= P.ref_dims[2])
+            continue;
+          mult = (unsigned int)(__popc(ref_mask_fast<0>(s_rc[wv][0][p], cj0, P)) *
+                                __popc(ref_mask_fast<1>(s_rc[wv][1][p], cj1, P)) *
+                                __popc(ref_mask_fast<2>(s_rc[wv][2][p], cj2, P)));
+        }
         if (mult) {
           atomicAdd(&s_cnt[wv][0][p], mult);
           if (NORMAL == RP_NORMAL_JKR) atomicAdd(&s_cnt[wv][1][p], mult);
@@ -1280,7 +1354,9 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
     if (flags & FLAG_DEAD) {
       const double qnan = __builtin_nan("");
       stxr(D.xr, i, mk(qnan, qnan, qnan), qnan);
+      if (HIST) HW.cnt[i] = 0;
     } else {
+      if (HIST) acc.hold = HR.cnt[i];
       const int cx = fine_coord(o.x.x, 0, P, true), cy = fine_coord(o.x.y, 1, P, true),
                 cz = fine_coord(o.x.z, 2, P, true);
       const uint32_t cell = cell_linear(P, cx, cy, cz);
@@ -1326,14 +1402,17 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
                 defer |= 1ull << (base + q);
                 continue;
               }
-              const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
-                        cj2 = ref_coord_fast<2>(xq[q].z, P);
-              if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
-                  cj2 >= P.ref_dims[2])
-                continue;
-              const unsigned int mult = (unsigned int)(__popc(ref_mask_fast<0>(o.rc0, cj0, P)) *
-                                                       __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
-                                                       __popc(ref_mask_fast<2>(o.rc2, cj2, P)));
+              unsigned int mult = 1u;  // physics-fix mode: every pair once
+              if (!P.fix) {
+                const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
+                          cj2 = ref_coord_fast<2>(xq[q].z, P);
+                if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 ||
+                    cj2 >= P.ref_dims[2])
+                  continue;
+                mult = (unsigned int)(__popc(ref_mask_fast<0>(o.rc0, cj0, P)) *
+                                      __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
+                                      __popc(ref_mask_fast<2>(o.rc2, cj2, P)));
+              }
               acc.ncont += mult;
               if (NORMAL == RP_NORMAL_JKR) acc.njkr += mult;
             }
@@ -1344,7 +1423,8 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
         while (defer) {
           const int k = __ffsll((long long)defer) - 1;
           defer &= defer - 1ull;
-          contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, nbr[k * cap + i], k, acc, ctl, prow, sd);
+          contact_pp<NORMAL, TANG, RECORD, JS, HIST>(P, mc, far_pp, S, T, o, nbr[k * cap + i], k, acc, ctl, prow, sd,
+                                                     HR, HW, i);
         }
         for (int k = 64; k < cnt; ++k) {
           const int j = nbr[k * cap + i];
@@ -1353,7 +1433,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
           ldxr(S.xr, j, xj, rj);
           const dv3 d = pair_delta(P, o.x, xj);
           if (in_contact(P, dot(d, d), o.r + rj))
-            contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, k, acc, ctl, prow, sd);
+            contact_pp<NORMAL, TANG, RECORD, JS, HIST>(P, mc, far_pp, S, T, o, j, k, acc, ctl, prow, sd, HR, HW, i);
         }
       } else {
         for (int base = 0; base < cnt; base += kNbrBatch) {
@@ -1378,11 +1458,13 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
             hit &= hit - 1;
             const int j = kNbrBatch == 2 ? (q == 0 ? jj[0] : jj[kNbrBatch - 1])
                                          : sel4(q, jj[0], jj[1 % kNbrBatch], jj[2 % kNbrBatch], jj[3 % kNbrBatch]);
-            contact_pp<NORMAL, TANG, RECORD, JS>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd);
+            contact_pp<NORMAL, TANG, RECORD, JS, HIST>(P, mc, far_pp, S, T, o, j, base + q, acc, ctl, prow, sd, HR,
+                                                       HW, i);
           }
         }
       }
       if (RECORD) paircount[i] = min(acc.npair, P.pair_cap);
+      if (HIST) HW.cnt[i] = acc.hn;
       for (uint32_t q = t0; q < t1; ++q)
         contact_wall<NORMAL, JS>(P, mc, far_pw, o, tris[tri_idx[q]], tri_idx[q], acc, nwall, sd);
 
@@ -1430,24 +1512,27 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
   // counters: per-workgroup plain stores; the rebuild request is the only
   // atomic, issued by a wave only when one of its particles needs it
   const unsigned int c1 = wave_sum_u32(acc.ncont), c2 = wave_sum_u32(nwall), c3 = wave_sum_u32(acc.njkr);
+  const unsigned int c4 = HIST ? wave_sum_u32((unsigned int)acc.hn) : 0u;
   const bool anyo = __any(acc.overlap), anyf = __any(far);
-  __shared__ unsigned int red[kForceBlock / 64][4];
+  __shared__ unsigned int red[kForceBlock / 64][5];
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[wv][0] = c1;
     red[wv][1] = c2;
     red[wv][2] = c3;
     red[wv][3] = anyo ? 1u : 0u;
+    red[wv][4] = c4;
     if (ADVANCE && anyf) atomicOr(&ctl->rebuild[flag_slot], 1);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    BlockStats b{0u, 0u, 0u, 0u};
+    BlockStats b{0u, 0u, 0u, 0u, 0u};
     for (int q = 0; q < kForceBlock / 64; ++q) {
       b.contacts += red[q][0];
       b.wall_contacts += red[q][1];
       b.jkr_failures += red[q][2];
       b.overlap |= red[q][3];
+      b.history += red[q][4];
     }
     bstats[blockIdx.x] = b;
     if (b.overlap) {
@@ -1504,12 +1589,14 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
   const double dsq = dot(dd, dd);
   const double rsum = o.r + rj;
   if (!in_contact(P, dsq, rsum)) return false;
-  const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
-  if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
-    return false;
-  const int mult = __popc(ref_mask<0>(o.rc0, cj0, P)) * __popc(ref_mask<1>(o.rc1, cj1, P)) *
-                   __popc(ref_mask<2>(o.rc2, cj2, P));
-  if (mult == 0) return false;
+  if (!P.fix) {  // the physics-fix mode has no reference-grid visibility
+    const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+    if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
+      return false;
+    const int mult = __popc(ref_mask<0>(o.rc0, cj0, P)) * __popc(ref_mask<1>(o.rc1, cj1, P)) *
+                     __popc(ref_mask<2>(o.rc2, cj2, P));
+    if (mult == 0) return false;
+  }
   const double overlap = rsum - sqrt(dsq);
   const double rstar = (o.r * rj) / (o.r + rj);
   if (!(c.gamma > 0.0)) return false;
@@ -1771,12 +1858,36 @@ struct rp_engine {
   unsigned* d_surr_counters = nullptr;  // [kSurrRanges * kSurrCounterStride] row claims
   bool surr_inst_valid = false;
   bool sstats_valid = false;     // the last force launch wrote d_sstats
+  // physics-fix tangential history: hist[cur] holds the tables, the force
+  // kernel writes hist[cur ^ 1]; capacity hcap = nbr_cap entries per slot
+  bool hist_on = false;
+  void* d_hist_mem[2] = {nullptr, nullptr};
+  Hist hist[2]{};
+  int hcap = 0;
 };
 
 namespace {
 
 void free_ptr(void* p) {
   if (p) (void)hipFree(p);
+}
+
+// History buffer: keys (u32) [hcap][cap], xi (3 doubles) [hcap][cap], counts (int) [cap].
+size_t hist_bytes(int cap, int hcap) {
+  const size_t c = (size_t)cap, h = (size_t)hcap;
+  return round_up(h * c * sizeof(uint32_t), 256) + round_up(3 * h * c * sizeof(double), 256) +
+         round_up(c * sizeof(int), 256);
+}
+Hist carve_hist(void* mem, int cap, int hcap) {
+  Hist H{};
+  char* p = (char*)mem;
+  const size_t c = (size_t)cap, h = (size_t)hcap;
+  H.key = (uint32_t*)p;
+  p += round_up(h * c * sizeof(uint32_t), 256);
+  H.xi = (double*)p;
+  p += round_up(3 * h * c * sizeof(double), 256);
+  H.cnt = (int*)p;
+  return H;
 }
 
 MaterialConsts material_consts(const rp_material& pm, const rp_material& wm) {
@@ -1790,6 +1901,11 @@ MaterialConsts material_consts(const rp_material& pm, const rp_material& wm) {
   m.pw = pair(pm.youngs_modulus, pm.poissons_ratio, wm.youngs_modulus, wm.poissons_ratio, pm.surface_energy,
               wm.surface_energy);
   m.mu = pm.friction_coefficient;
+  // physics-fix shear modulus (the reference's effective_shear_modulus is a
+  // placeholder 0, material.rs:39-50): G = E / (2 (1 + nu)), 1/G* = 2 (2 - nu) / G
+  const double g = pm.youngs_modulus / (2.0 * (1.0 + pm.poissons_ratio));
+  const double ginv = (2.0 - pm.poissons_ratio) / g + (2.0 - pm.poissons_ratio) / g;
+  m.gstar_pp = 1.0 / ginv;
   double lne = std::log(pm.restitution_coefficient);
   double beta = lne / std::sqrt(lne * lne + kPi * kPi);
   m.beta_abs = std::fabs(beta);
@@ -1824,7 +1940,9 @@ rp_status validate(const rp_params* p) {
   }
   if (p->normal_model < 0 || p->normal_model > 4) return fail(RP_ERR_INVALID, "invalid normal model");
   if (p->tangential_model < 0 || p->tangential_model > 2) return fail(RP_ERR_INVALID, "invalid tangential model");
-  if (p->mode != RP_MODE_PARITY) return fail(RP_ERR_INVALID, "unsupported mode");
+  if (p->mode != RP_MODE_PARITY && p->mode != RP_MODE_PHYSICS_FIX) return fail(RP_ERR_INVALID, "unsupported mode");
+  if (p->cell_order < RP_CELL_ORDER_AUTO || p->cell_order > RP_CELL_ORDER_PENCIL)
+    return fail(RP_ERR_INVALID, "invalid cell_order");
   if (!(p->ref_cell_size > 0.0)) return fail(RP_ERR_INVALID, "ref_cell_size must be positive");
   if (p->cell_size < 0.0 || p->skin < 0.0) return fail(RP_ERR_INVALID, "cell_size and skin must be >= 0");
   return RP_OK;
@@ -1848,6 +1966,7 @@ rp_status compute_geometry(rp_engine* e) {
   }
   K.normal = p.normal_model;
   K.tangential = p.tangential_model;
+  K.fix = p.mode == RP_MODE_PHYSICS_FIX ? 1 : 0;
   K.mc = material_consts(p.particle, p.wall);
   K.margin_factor = (p.normal_model == RP_NORMAL_JKR || p.normal_model == RP_NORMAL_SJKR) ? 1.5 : 1.0;
   K.ref_cs = p.ref_cell_size;
@@ -1970,6 +2089,7 @@ rp_status compute_geometry(rp_engine* e) {
   K.cap = e->cap;
   K.nbr_cap = e->nbr_cap;
   K.pair_cap = e->pair_cap;
+  K.hcap = e->hcap;
   return RP_OK;
 }
 
@@ -2155,6 +2275,13 @@ void free_particles(rp_engine* e) {
   free_ptr(e->d_paircount);
   free_ptr(e->d_bstats);
   e->d_bstats = nullptr;
+  for (int b = 0; b < 2; ++b) {
+    free_ptr(e->d_hist_mem[b]);
+    e->d_hist_mem[b] = nullptr;
+    e->hist[b] = Hist{};
+  }
+  e->hcap = 0;
+  e->kp.hcap = 0;
   free_ptr(e->d_surr_base);
   free_ptr(e->d_surr_rowmask);
   free_ptr(e->d_surr_nmask);
@@ -2188,6 +2315,51 @@ rp_status alloc_cells(rp_engine* e) {
   return RP_OK;
 }
 
+// Moves the current history tables (hist[cur], old capacity) into a buffer of
+// the new capacity: a list can hold at most nbr_cap contacts, so a table
+// never needs more entries than the lists have.
+__global__ void __launch_bounds__(kBlock) k_hist_regrow(Hist A, int hcap_a, Hist B, int hcap_b, int cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  const int c = A.cnt[i];
+  B.cnt[i] = c;
+  const size_t sa = (size_t)hcap_a * cap, sb = (size_t)hcap_b * cap;
+  for (int k = 0; k < c; ++k) {
+    const size_t o = (size_t)k * cap + i;
+    B.key[o] = A.key[o];
+    B.xi[o] = A.xi[o];
+    B.xi[o + sb] = A.xi[o + sa];
+    B.xi[o + 2 * sb] = A.xi[o + 2 * sa];
+  }
+}
+
+// (Re)allocates the history buffers with capacity hcap per slot, keeping the
+// current tables (fresh buffers start empty).
+rp_status alloc_hist(rp_engine* e, int hcap) {
+  e->gen++;
+  void* mem[2] = {nullptr, nullptr};
+  const size_t bytes = hist_bytes(e->cap, hcap);
+  for (int b = 0; b < 2; ++b) HIP_TRY(hipMalloc(&mem[b], bytes));
+  Hist nh[2] = {carve_hist(mem[0], e->cap, hcap), carve_hist(mem[1], e->cap, hcap)};
+  HIP_TRY(hipMemsetAsync(nh[0].cnt, 0, (size_t)e->cap * sizeof(int), e->stream));
+  HIP_TRY(hipMemsetAsync(nh[1].cnt, 0, (size_t)e->cap * sizeof(int), e->stream));
+  if (e->d_hist_mem[e->cur]) {
+    const int c = e->cur;
+    hipLaunchKernelGGL(k_hist_regrow, dim3((e->cap + kBlock - 1) / kBlock), dim3(kBlock), 0, e->stream, e->hist[c],
+                       e->hcap, nh[c], hcap, e->cap);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int b = 0; b < 2; ++b) {
+    free_ptr(e->d_hist_mem[b]);
+    e->d_hist_mem[b] = mem[b];
+    e->hist[b] = nh[b];
+  }
+  e->hcap = hcap;
+  e->kp.hcap = hcap;
+  return RP_OK;
+}
+
 rp_status alloc_nbr(rp_engine* e, int nbr_cap) {
   // the kernels address list entry k of slot i as k * cap + i in 32 bits
   if ((size_t)nbr_cap * (size_t)e->cap >= ((size_t)1 << 31))
@@ -2199,6 +2371,7 @@ rp_status alloc_nbr(rp_engine* e, int nbr_cap) {
   HIP_TRY(hipMalloc(&e->d_nbr, (size_t)nbr_cap * (size_t)e->cap * sizeof(int)));
   e->nbr_cap = nbr_cap;
   e->kp.nbr_cap = nbr_cap;
+  if (e->hist_on) return alloc_hist(e, nbr_cap);
   return RP_OK;
 }
 
@@ -2231,10 +2404,11 @@ rp_status launch_rebuild(rp_engine* e) {
                      e->d_block_sums, e->d_cell_start);
   hipLaunchKernelGGL(k_scatter, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, e->d_key, e->d_cell_start,
                      e->d_cell_count, e->d_tmp);
+  const Hist HS = e->hist_on ? e->hist[e->cur] : Hist{}, HD = e->hist_on ? e->hist[e->cur ^ 1] : Hist{};
   hipLaunchKernelGGL(k_rank_permute, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, W, V, D, U, e->d_key,
-                     e->d_cell_start, e->d_tmp);
+                     e->d_cell_start, e->d_tmp, HS, HD);
   hipLaunchKernelGGL(k_build, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D, U, W, V, e->d_cell_start,
-                     e->d_nbr_count, e->d_nbr, e->d_xb);
+                     e->d_nbr_count, e->d_nbr, e->d_xb, HD, HS);
   HIP_TRY(hipGetLastError());
   return RP_OK;
 }
@@ -2250,6 +2424,24 @@ rp_status read_ctl(rp_engine* e) {
 rp_status prepare(rp_engine* e) {
   if (!e->have_particles) return fail(RP_ERR_STATE, "no particles uploaded");
   if (!e->geometry_dirty) return RP_OK;
+  if (e->prm.mode == RP_MODE_PHYSICS_FIX && e->ninst > 1)
+    return fail(RP_ERR_INVALID, "the physics-fix mode runs single engines, not batched ensembles");
+  const bool hist = e->prm.mode == RP_MODE_PHYSICS_FIX && e->prm.tangential_model != RP_TANGENTIAL_COULOMB;
+  if (hist != e->hist_on) {
+    e->hist_on = hist;
+    if (hist) {
+      if (rp_status st = alloc_hist(e, e->nbr_cap)) return st;
+    } else {
+      for (int b = 0; b < 2; ++b) {
+        free_ptr(e->d_hist_mem[b]);
+        e->d_hist_mem[b] = nullptr;
+        e->hist[b] = Hist{};
+      }
+      e->hcap = 0;
+    }
+  } else if (hist && !e->d_hist_mem[0]) {
+    if (rp_status st = alloc_hist(e, e->nbr_cap)) return st;
+  }
   rp_status st = compute_geometry(e);
   if (st) return st;
   if ((st = alloc_cells(e))) return st;
@@ -2405,24 +2597,25 @@ void launch_surrogate(rp_engine* e) {
   mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
 }
 
-template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH, bool DEAL>
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH, bool DEAL, bool HIST>
 void launch_force_d(rp_engine* e, int flag_slot) {
   int g = (int)((e->cap + kForceBlock - 1) / kForceBlock);
+  const Hist hr = HIST ? e->hist[e->cur] : Hist{}, hw = HIST ? e->hist[e->cur ^ 1] : Hist{};
   if (e->ext_start) {
     // timing mode: the dispatch records the kernel's start and end
-    hipExtLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL>), dim3(g), dim3(kForceBlock), 0, e->stream,
-                          e->ext_start, e->ext_stop, 0u, e->d_ctl, flag_slot, label_off(e), e->kp,
+    hipExtLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL, HIST>), dim3(g), dim3(kForceBlock), 0,
+                          e->stream, e->ext_start, e->ext_stop, 0u, e->d_ctl, flag_slot, label_off(e), e->kp,
                           e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], (const int*)e->d_nbr_count,
                           (const int*)e->d_nbr, (const Tri*)e->d_tris, (const uint32_t*)e->d_tri_start,
                           (const uint32_t*)e->d_tri_idx, (const float*)e->d_xb, e->d_bstats, e->d_pairbuf,
-                          e->d_paircount, JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+                          e->d_paircount, JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats, hr, hw);
     e->ext_start = e->ext_stop = nullptr;
   } else {
-    hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL>), dim3(g), dim3(kForceBlock), 0, e->stream, e->d_ctl, flag_slot,
-                       label_off(e), e->kp,
-                       e->dyn[e->cur], e->dyn[e->cur ^ 1], e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris,
-                       e->d_tri_start, e->d_tri_idx, e->d_xb, e->d_bstats, e->d_pairbuf, e->d_paircount,
-                       JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats);
+    hipLaunchKernelGGL((k_force<NORMAL, TANG, ADV, REC, JS, BATCH, DEAL, HIST>), dim3(g), dim3(kForceBlock), 0,
+                       e->stream, e->d_ctl, flag_slot, label_off(e), e->kp, e->dyn[e->cur], e->dyn[e->cur ^ 1],
+                       e->stat[0], e->d_nbr_count, e->d_nbr, e->d_tris, e->d_tri_start, e->d_tri_idx, e->d_xb,
+                       e->d_bstats, e->d_pairbuf, e->d_paircount, JS >= 2 ? surr_dev(e) : SurrDev{}, e->d_sstats,
+                       hr, hw);
   }
   e->force_grid = g;
   e->sstats_valid = JS >= 2;
@@ -2431,52 +2624,63 @@ void launch_force_d(rp_engine* e, int flag_slot) {
 }
 
 // Kernels with a zero-force pass come in both tilings of it.
-template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH>
+template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH, bool HIST>
 void launch_force_t(rp_engine* e, int flag_slot) {
-  if constexpr (split_pass(NORMAL, TANG, REC, JS)) {
-    if (e->tiling == RP_TILING_WAVE) launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, true>(e, flag_slot);
-    else launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false>(e, flag_slot);
+  if constexpr (split_pass(NORMAL, TANG, REC, JS, HIST)) {
+    if (e->tiling == RP_TILING_WAVE) launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, true, HIST>(e, flag_slot);
+    else launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false, HIST>(e, flag_slot);
   } else {
-    launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false>(e, flag_slot);
+    launch_force_d<NORMAL, TANG, ADV, REC, JS, BATCH, false, HIST>(e, flag_slot);
   }
 }
 
-// JKR solver variants are compiled for the JKR model only; a recording
-// launch uses the exact solve with the shortcut (no surrogate rows).
+// Single engines: the JKR solver variants (compiled for the JKR model only; a
+// recording launch uses the exact solve with the shortcut, no surrogate rows).
+template <int NORMAL, int TANG, bool ADV, bool REC, bool HIST>
+void launch_force_s(rp_engine* e, int flag_slot) {
+  if constexpr (NORMAL != RP_NORMAL_JKR) {
+    launch_force_t<NORMAL, TANG, ADV, REC, 0, false, HIST>(e, flag_slot);
+  } else {
+    constexpr int kSurr = REC ? 1 : 2, kSurrCheck = REC ? 1 : 3;
+    switch (e->jkr_solver) {
+      case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1, false, HIST>(e, flag_slot); break;
+      case RP_JKR_SURROGATE:
+        if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, REC, kSurrCheck, false, HIST>(e, flag_slot);
+        else launch_force_t<NORMAL, TANG, ADV, REC, kSurr, false, HIST>(e, flag_slot);
+        break;
+      default: launch_force_t<NORMAL, TANG, ADV, REC, 0, false, HIST>(e, flag_slot); break;
+    }
+  }
+}
+
 // Batched engines (several instances) use the per-instance material table;
-// they have no pair recording.
+// they have no pair recording and no physics-fix history.
 template <int NORMAL, int TANG, bool ADV, bool REC>
 void launch_force_j(rp_engine* e, int flag_slot) {
   if constexpr (!REC) {
     if (e->ninst > 1) {
       if constexpr (NORMAL == RP_NORMAL_JKR) {
         if (e->jkr_solver == RP_JKR_SURROGATE) {
-          if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, false, 3, true>(e, flag_slot);
-          else launch_force_t<NORMAL, TANG, ADV, false, 2, true>(e, flag_slot);
+          if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, false, 3, true, false>(e, flag_slot);
+          else launch_force_t<NORMAL, TANG, ADV, false, 2, true, false>(e, flag_slot);
           return;
         }
         if (e->jkr_solver == RP_JKR_SHORTCUT) {
-          launch_force_t<NORMAL, TANG, ADV, false, 1, true>(e, flag_slot);
+          launch_force_t<NORMAL, TANG, ADV, false, 1, true, false>(e, flag_slot);
           return;
         }
       }
-      launch_force_t<NORMAL, TANG, ADV, false, 0, true>(e, flag_slot);
+      launch_force_t<NORMAL, TANG, ADV, false, 0, true, false>(e, flag_slot);
       return;
     }
   }
-  if constexpr (NORMAL != RP_NORMAL_JKR) {
-    launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot);
-  } else {
-    constexpr int kSurr = REC ? 1 : 2, kSurrCheck = REC ? 1 : 3;
-    switch (e->jkr_solver) {
-      case RP_JKR_SHORTCUT: launch_force_t<NORMAL, TANG, ADV, REC, 1, false>(e, flag_slot); break;
-      case RP_JKR_SURROGATE:
-        if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, REC, kSurrCheck, false>(e, flag_slot);
-        else launch_force_t<NORMAL, TANG, ADV, REC, kSurr, false>(e, flag_slot);
-        break;
-      default: launch_force_t<NORMAL, TANG, ADV, REC, 0, false>(e, flag_slot); break;
+  if constexpr (TANG != RP_TANGENTIAL_COULOMB) {
+    if (e->hist_on) {
+      launch_force_s<NORMAL, TANG, ADV, REC, true>(e, flag_slot);
+      return;
     }
   }
+  launch_force_s<NORMAL, TANG, ADV, REC, false>(e, flag_slot);
 }
 
 // Tangential kinds compiled: Coulomb, and the spring law (LSC / Mindlin) with
@@ -2766,6 +2970,8 @@ rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* coun
   if (ninst < 1 || ninst > 65535) return fail(RP_ERR_INVALID, "ninst must lie in [1, 65535]");
   if (!counts || !particle) return fail(RP_ERR_INVALID, "counts and particle materials are required");
   if (e->record && ninst > 1) return fail(RP_ERR_STATE, "pair recording is not available for batched engines");
+  if (e->prm.mode == RP_MODE_PHYSICS_FIX && ninst > 1)
+    return fail(RP_ERR_INVALID, "the physics-fix mode runs single engines, not batched ensembles");
   size_t n = 0;
   for (int q = 0; q < ninst; ++q) n += counts[q];
   std::vector<uint16_t> inst(n);
@@ -3073,6 +3279,63 @@ rp_status rp_engine_download_aux(rp_engine* e, double* rx3, double* rv3, double*
   return RP_OK;
 }
 
+rp_status rp_engine_history(rp_engine* e, int64_t* offsets, int64_t* partner, double* xi3, size_t cap,
+                            size_t* total) {
+  if (!e || !total) return fail(RP_ERR_INVALID, "NULL argument");
+  *total = 0;
+  if (!e->have_particles || !e->hist_on || !e->d_hist_mem[0]) {
+    if (offsets) {
+      size_t n = 0;
+      if (e->have_particles) {
+        rp_status st = rp_engine_count(e, &n);
+        if (st) return st;
+      }
+      for (size_t q = 0; q <= n; ++q) offsets[q] = 0;
+    }
+    return RP_OK;
+  }
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  rp_status st = slot_order(e, order, cur, cst, ns);
+  if (st) return st;
+  const Hist& H = e->hist[e->cur];
+  const size_t c = (size_t)e->cap, h = (size_t)e->hcap;
+  std::vector<int> cnt((size_t)std::max(ns, 1));
+  std::vector<uint32_t> sid((size_t)std::max(ns, 1)), key(h * c);
+  std::vector<double> xi(3 * h * c);
+  if (ns > 0) {
+    HIP_TRY(hipMemcpy(cnt.data(), H.cnt, (size_t)ns * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sid.data(), cst.sid, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(key.data(), H.key, key.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(xi.data(), H.xi, xi.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  std::vector<std::pair<uint32_t, int64_t>> api;
+  api.reserve(order.size());
+  for (size_t q = 0; q < order.size(); ++q) api.emplace_back(sid[order[q]], (int64_t)q);
+  size_t tot = 0;
+  for (size_t q = 0; q < order.size(); ++q) tot += (size_t)cnt[order[q]];
+  *total = tot;
+  if (!offsets || !partner || !xi3) return RP_OK;
+  if (cap < tot) return fail(RP_ERR_INVALID, "history buffer too small");
+  size_t o = 0;
+  for (size_t q = 0; q < order.size(); ++q) {
+    const size_t i = (size_t)order[q];
+    offsets[q] = (int64_t)o;
+    for (int k = 0; k < cnt[i]; ++k, ++o) {
+      const size_t b = (size_t)k * c + i;
+      auto it = std::lower_bound(api.begin(), api.end(), std::make_pair(key[b], (int64_t)-1));
+      partner[o] = (it != api.end() && it->first == key[b]) ? it->second : -1;
+      xi3[3 * o] = xi[b];
+      xi3[3 * o + 1] = xi[b + h * c];
+      xi3[3 * o + 2] = xi[b + 2 * h * c];
+    }
+  }
+  offsets[order.size()] = (int64_t)o;
+  return RP_OK;
+}
+
 rp_status rp_engine_cells(rp_engine* e, int32_t rebuild, int64_t* ref_cell, uint32_t* fine_key, int64_t* slot_api,
                           double* grid) {
   if (!e) return fail(RP_ERR_INVALID, "engine is NULL");
@@ -3202,6 +3465,7 @@ rp_status rp_engine_stats(rp_engine* e, rp_stats* out) {
       out->wall_contacts += b.wall_contacts;
       out->jkr_failures += b.jkr_failures;
       out->excessive_overlap |= (int32_t)b.overlap;
+      out->history_contacts += b.history;
     }
   }
   return RP_OK;
@@ -3269,6 +3533,8 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   if (steps == 0) return fail(RP_ERR_INVALID, "steps must be positive");
   rp_status st = prepare(e);
   if (st) return st;
+  if (e->hist_on)  // the candidates would advance the history tables the snapshot does not hold
+    return fail(RP_ERR_STATE, "the autotuner does not run with a tangential history (physics-fix mode)");
   e->skin_tuned = false;  // no guard while the candidates run
   HIP_TRY(hipStreamSynchronize(e->stream));
   // snapshot of everything a step reads: state buffers, control block, host

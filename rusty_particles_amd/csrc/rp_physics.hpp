@@ -100,6 +100,7 @@ struct MaterialConsts {
   PairConsts pp, pw;
   double mu;         // particle friction (walls use it too, simulation.rs:514)
   double beta_abs;   // |ln e / sqrt(ln^2 e + pi^2)|, particle restitution (physics.rs:111-112)
+  double gstar_pp;   // physics-fix mode: 1 / ((2 - nu)/G + (2 - nu)/G), G = E / (2 (1 + nu))
 };
 
 // ---- normal laws -----------------------------------------------------------
@@ -234,6 +235,36 @@ __device__ inline dv3 spring_coulomb_g0(double fn, double kt, double mu) {
   return trial;
 }
 
+// ---- physics-fix mode: tangential spring with history -------------------------
+// linear_spring_coulomb (physics.rs:400-434) with the stored displacement xi
+// projected onto the current tangent plane first, no force and no history
+// for kt <= 0 (Mindlin at delta <= 0, where physics.rs:452 is NaN), and the
+// slip direction for any non-zero trial force.
+__device__ inline dv3 spring_coulomb_fix(double fn, dv3 vrel, dv3 n, dv3& xi, double kt, double mu, double dt) {
+  if (!(kt > 0.0)) {
+    xi = zero3();
+    return zero3();
+  }
+  xi = sub(xi, smul(dot(xi, n), n));
+  const dv3 vt = sub(vrel, smul(dot(vrel, n), n));
+  xi = add(xi, mul(vt, dt));
+  const dv3 trial = smul(-kt, xi);
+  const double tm = len(trial);
+  const double fc = mu * fn;
+  if (tm > fc) {
+    const dv3 dir = tm > 0.0 ? divs(trial, tm) : zero3();
+    const dv3 ft = smul(fc, dir);
+    xi = divs(neg(ft), kt);
+    return ft;
+  }
+  return trial;
+}
+
+// Mindlin stiffness 8 G* sqrt(R* delta) for delta > 0, else 0 (physics.rs:443-459)
+__device__ inline double mindlin_kt_fix(double gstar, double rstar, double ov) {
+  return ov > 0.0 ? 8.0 * gstar * sqrt(rstar * ov) : 0.0;
+}
+
 // ---- triangle ---------------------------------------------------------------
 
 struct Tri {
@@ -303,6 +334,62 @@ __device__ inline dv3 closest_point(const Tri& tr, dv3 p) {
     }
   }
   return add(add(tr.v0, smul(sc, e0)), smul(tc, e1));
+}
+
+// Closest point of a triangle, physics-fix mode (Eberly, "Distance Between
+// Point and Triangle in 3D", Geometric Tools): the regions of mesh.rs:63-113
+// with each edge parameter normalised by its edge's squared length before
+// the clamp to [0, 1].
+__device__ inline double edge_param(double num, double den) {
+  return num <= 0.0 ? 0.0 : (num >= den ? 1.0 : num / den);
+}
+__device__ inline dv3 closest_point_fix(const Tri& tr, dv3 p) {
+  const dv3 e0 = sub(tr.v1, tr.v0);
+  const dv3 e1 = sub(tr.v2, tr.v0);
+  const dv3 v0 = sub(tr.v0, p);
+  const double a = dot(e0, e0), b = dot(e0, e1), c = dot(e1, e1), d = dot(e0, v0), e = dot(e1, v0);
+  const double det = a * c - b * b;
+  double s = b * e - c * d;
+  double t = b * d - a * e;
+  if (s + t <= det) {
+    if (s < 0.0) {
+      if (t < 0.0 && d < 0.0) {
+        s = edge_param(-d, a);
+        t = 0.0;
+      } else {
+        s = 0.0;
+        t = edge_param(-e, c);
+      }
+    } else if (t < 0.0) {
+      s = edge_param(-d, a);
+      t = 0.0;
+    } else {
+      s = s / det;
+      t = t / det;
+    }
+  } else if (s < 0.0) {
+    const double tmp0 = b + d, tmp1 = c + e;
+    if (tmp1 > tmp0) {
+      s = edge_param(tmp1 - tmp0, a - 2.0 * b + c);
+      t = 1.0 - s;
+    } else {
+      s = 0.0;
+      t = edge_param(-e, c);
+    }
+  } else if (t < 0.0) {
+    const double tmp0 = b + e, tmp1 = a + d;
+    if (tmp1 > tmp0) {
+      t = edge_param(tmp1 - tmp0, a - 2.0 * b + c);
+      s = 1.0 - t;
+    } else {
+      t = 0.0;
+      s = edge_param(-d, a);
+    }
+  } else {
+    s = edge_param((c + e) - (b + d), a - 2.0 * b + c);
+    t = 1.0 - s;
+  }
+  return add(add(tr.v0, smul(s, e0)), smul(t, e1));
 }
 
 // ---- integrator -------------------------------------------------------------

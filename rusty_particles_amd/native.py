@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
     L.rp_stl_read.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), _fp, _fp]
     L.rp_engine_pairs.argtypes = [_eng, _i64p, _i64p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.rp_engine_cells.argtypes = [_eng, C.c_int32, _i64p, C.POINTER(C.c_uint32), _i64p, _dp]
+    L.rp_engine_history.argtypes = [_eng, _i64p, _i64p, _dp, C.c_size_t, C.POINTER(C.c_size_t)]
     L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
     L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
     L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
@@ -74,7 +75,7 @@ def lib() -> C.CDLL:
     for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_set_mesh_motion", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
-                 "rp_stl_read", "rp_engine_pairs", "rp_engine_cells", "rp_engine_stats", "rp_engine_set_timing",
+                 "rp_stl_read", "rp_engine_pairs", "rp_engine_cells", "rp_engine_history", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
                  "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph",
                  "rp_engine_autotune", "rp_engine_set_tiling", "rp_engine_get_tiling"):
@@ -245,6 +246,19 @@ class Engine:
         _check(self._L.rp_engine_pairs(self._h, _ptr(offs, C.c_int64), _ptr(js, C.c_int64), js.size,
                                        C.byref(tot)))
         return offs, js[: tot.value]
+
+    def history(self):
+        """Physics-fix tangential history (rp_engine_history): (offsets[n+1],
+        partner API index, displacement[total, 3]), per particle by ascending partner."""
+        tot = C.c_size_t(0)
+        _check(self._L.rp_engine_history(self._h, None, None, None, 0, C.byref(tot)))
+        n = self.count()
+        offs = np.zeros(n + 1, dtype=np.int64)
+        part = np.zeros(max(tot.value, 1), dtype=np.int64)
+        xi = np.zeros((max(tot.value, 1), 3))
+        _check(self._L.rp_engine_history(self._h, _ptr(offs, C.c_int64), _ptr(part, C.c_int64), _ptr(xi),
+                                         part.size, C.byref(tot)))
+        return offs, part[:tot.value], xi[:tot.value]
 
     def cells(self, rebuild: bool = True) -> dict:
         """Broad-phase cells (rp_engine_cells): per particle in API order the

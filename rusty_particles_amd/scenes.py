@@ -502,7 +502,8 @@ def jkr_ensemble_workload(ninst: int = 64, seed: int = 0xD3E50006, nmin: int = 5
     return p, [int(c) for c in counts], mats, x, r, m, v
 
 
-def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, length: float = 0.03):
+def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, length: float = 0.03,
+                       press: float = 15e-6):
     """The capacity run's spheres, materials and drum (drum_workload: r ~
     U[0.4, 0.8] mm, JKR gamma 0.01 + Coulomb, E=5e7, 16,384-triangle drum
     turning at 30 rpm) at a size the oracle follows: a 6 cm drum whose lower
@@ -510,7 +511,7 @@ def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 
     chain share a radius (drawn per chain) and touch their neighbours with
     overlaps U[0.05, 3] um -- JKR contacts with a root, the range a settled
     bed has and the surrogate serves; chains are 2.2 r_max apart (inside each
-    other's JKR cut-off, without a root); the lowest chains press 1 um into
+    other's JKR cut-off, without a root); the outer chains press 15 um into
     the drum's side.  Returns (params, x, r, m, meshes); mesh 0 turns with
     DRUM_OMEGA about DRUM_ORIGIN."""
     rng = np.random.default_rng(seed)
@@ -525,11 +526,12 @@ def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 
     for q in order:
         rc = rng.uniform(rmin, rmax)
         rho = math.hypot(cx[q], cy[q])
-        if rho > radius - rc + 1e-6:
+        if rho > radius - rc + press:
             continue
         if rho > 1e-9 and rho > radius - rc - 0.5 * pitch:
-            # press the chain onto the side: 1 um of overlap with the wall
-            s_ = (radius - rc + 1e-6) / rho
+            # press the chain onto the side: `press` into the circle, more
+            # than the 9 um the 128-sided polygon's facets lie inside it
+            s_ = (radius - rc + press) / rho
             px, py = cx[q] * s_, cy[q] * s_
         else:
             px, py = cx[q], cy[q]
