@@ -404,6 +404,25 @@ rp_status rp_mlp_infer(const float* params, const float* x_scaled, size_t rows, 
 rp_status rp_mlp_gradient(const float* params, const float* x_scaled, const float* t_scaled, size_t rows,
                           float* grad, double* loss);
 
+/* Inference that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * network and kernel as rp_mlp_infer, but every pointer is device memory the
+ * caller owns: params_dev (RP_MLP_PARAMS floats), x_scaled_dev (rows x 4),
+ * y_scaled_dev (rows x 2).  The packed network (BF16 weights in MFMA
+ * fragment order, FP32 biases) is written into a static device array of
+ * RP_MLP_NOALLOC_SCRATCH bytes that each GPU holds until the process ends;
+ * the work is enqueued on `stream` (a hipStream_t, NULL = the null stream)
+ * and the call returns without waiting.  rows above RP_MLP_NOALLOC_MAX_ROWS
+ * are refused (RP_ERR_INVALID).  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race.
+ * The first call on a device looks the array's address up
+ * (hipGetSymbolAddress); make it outside a stream capture. */
+#define RP_MLP_NOALLOC_SCRATCH 16384
+#define RP_MLP_NOALLOC_MAX_ROWS (1 << 28)
+rp_status rp_mlp_infer_noalloc(const float* params_dev, const float* x_scaled_dev, size_t rows,
+                               float* y_scaled_dev, void* stream);
+
 /* One 16x16x16 BF16 MFMA tile C = A B (row-major 16x16 FP32 in and out; A and
  * B rounded to BF16) through the library's product kernel. */
 rp_status rp_mfma_probe(const float* a16x16, const float* b16x16, float* c16x16);

@@ -169,6 +169,7 @@ EXPORTS = (
     "rp_jkr_samples",
     "rp_jkr_envelope_scale",
     "rp_mlp_infer",
+    "rp_mlp_infer_noalloc",
     "rp_mlp_gradient",
     "rp_mfma_probe",
     "rp_surrogate_last_error",

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rp_abi.h"
@@ -603,11 +604,99 @@ void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsign
   hipLaunchKernelGGL(k_mlp_counted, dim3(blocks), dim3(256), 0, stream, net, x, y, counter, cap);
 }
 
+// ---- inference without allocation (rp_mlp_infer_noalloc) ---------------------
+// The packed network -- BF16 weights in MFMA fragment order (the layout of
+// mlp_upload) and FP32 biases -- lives in this file-scope device array
+// instead of a hipMalloc'd buffer.  Each GPU holds its own copy at its own
+// address, kept until the process ends.  Host code never passes `scratch`
+// itself to a kernel (its host-side name is not a device address): it uses
+// the per-device address from hipGetSymbolAddress (noalloc_scratch below).
+constexpr int kPackedW = 16 * 64 + 64 * 64 + 64 * 16;          // BF16 weights
+constexpr int kPackedB = 64 + 64 + 16;                          // FP32 biases
+constexpr size_t kPackedBytes = (size_t)kPackedW * 2 + (size_t)kPackedB * 4;  // 12,864 B
+static_assert(kPackedBytes <= RP_MLP_NOALLOC_SCRATCH, "the packed network must fit the scratch array");
+static __device__ __align__(256) unsigned char scratch[RP_MLP_NOALLOC_SCRATCH];
+
+// Packs the FP32 parameter vector (device memory, RP_MLP_PARAMS floats) into
+// the scratch layout: element idx of the fragment-order weights as
+// mlp_upload writes it (zero padding of W1's rows 4..15 and W3's columns
+// 2..15), then the biases.
+__global__ void __launch_bounds__(256) k_pack_net(const float* __restrict__ p, uint16_t* __restrict__ w,
+                                                  float* __restrict__ b) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < kPackedW) {
+    int j = idx, base = 0;
+    const int part = j < 16 * 64 ? 0 : (j < 16 * 64 + 64 * 64 ? 1 : 2);
+    if (part == 1) base = 16 * 64;
+    if (part == 2) base = 16 * 64 + 64 * 64;
+    j -= base;
+    const int q = j & 3, rest = j >> 2, l = rest & 63, t = rest >> 6;
+    const int li = l & 15, kq = 4 * (l >> 4);
+    float v;
+    if (part == 0) {  // W1 tile nt = t: row kq + q, column 16 t + li
+      const int row = kq + q;
+      v = row < kMlpIn ? p[kOffW1 + row * kMlpHid + 16 * t + li] : 0.0f;
+    } else if (part == 1) {  // W2 tile t = 4 kt + nt
+      const int row = (t >> 2) * 16 + kq + q;
+      v = p[kOffW2 + row * kMlpHid + (t & 3) * 16 + li];
+    } else {  // W3 tile kt = t: row 16 t + kq + q, column li
+      const int row = t * 16 + kq + q;
+      v = li < kMlpOut ? p[kOffW3 + row * kMlpOut + li] : 0.0f;
+    }
+    w[idx] = f2bf(v);
+  } else if (idx < kPackedW + kPackedB) {
+    const int j = idx - kPackedW;
+    b[j] = j < 64 ? p[kOffB1 + j] : (j < 128 ? p[kOffB2 + j - 64] : (j - 128 < kMlpOut ? p[kOffB3 + j - 128] : 0.0f));
+  }
+}
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;  // device address of `scratch`, per device
+
+// The device address of `scratch` on device `dev` (current): looked up once
+// per device with hipGetSymbolAddress, then reused.  nullptr on failure.
+void* noalloc_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_dev.size()) g_scratch_dev.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_dev[(size_t)dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    g_scratch_dev[(size_t)dev] = ptr;
+  }
+  return g_scratch_dev[(size_t)dev];
+}
+
 }  // namespace rp
 
 using namespace rp;
 
 extern "C" {
+
+rp_status rp_mlp_infer_noalloc(const float* params_dev, const float* x_scaled_dev, size_t rows,
+                               float* y_scaled_dev, void* stream) {
+  if (!params_dev || (rows && (!x_scaled_dev || !y_scaled_dev))) return sfail(RP_ERR_INVALID, "NULL argument");
+  if (rows > (size_t)RP_MLP_NOALLOC_MAX_ROWS)
+    return sfail(RP_ERR_INVALID, "rp_mlp_infer_noalloc: rows exceeds RP_MLP_NOALLOC_MAX_ROWS (" +
+                                     std::to_string((long long)RP_MLP_NOALLOC_MAX_ROWS) + ")");
+  int dev = 0;
+  SHIP(hipGetDevice(&dev));
+  char* base = (char*)noalloc_scratch(dev);
+  if (!base) return sfail(RP_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed");
+  uint16_t* w = (uint16_t*)base;
+  float* b = (float*)(base + (size_t)kPackedW * 2);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pack_net, dim3((kPackedW + kPackedB + 255) / 256), dim3(256), 0, s, params_dev, w, b);
+  MlpDev net{};
+  net.w1 = w;
+  net.w2 = w + 16 * 64;
+  net.w3 = w + 16 * 64 + 64 * 64;
+  net.b1 = b;
+  net.b2 = b + 64;
+  net.b3 = b + 128;
+  mlp_infer(net, x_scaled_dev, y_scaled_dev, (int)rows, s);
+  SHIP(hipGetLastError());
+  return RP_OK;
+}
 
 const char* rp_surrogate_last_error(void) { return s_err.c_str(); }
 
