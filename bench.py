@@ -47,11 +47,21 @@ on a fresh engine from the same initial state after the same warm-up (so
 each times the same simulated window), value = the median region;
 contacts/s = the contacts of the last step times steps/s.
 
+--workload hopper-bed: the hopper configuration's 1M spheres packed in a
+cone of the same half-angle, outlet and triangle count (top radius 0.25 m so
+the bed fits; random sequential addition, radii then grown 8 %): a
+contact-bearing window of the same workload.
+
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
 HIP events on the engine's stream), and a CPU baseline: the oracle (a C++
 restatement of the reference's step_cpu, OpenMP over the host cores) timed
-on a bounded sample of the same workload.
+on a bounded sample of the same workload.  The default hopper line also
+carries, in the same JSON object: "contact_window" (the hopper-bed workload,
+same steps and warm-up: contacts/s and its force kernel's roofline) and
+"surrogate_batched" (the jkr-batch workload with the exact, shortcut and
+surrogate+batched forms and the surrogate's force errors); --no-extras
+leaves them out.
 """
 from __future__ import annotations
 
@@ -86,6 +96,9 @@ def build_workload(name: str, n: int | None):
     if name == "hopper":
         p, x, r, m, meshes = scenes.hopper_workload(n or 1_000_000)
         return p, x, r, m, None, meshes, "Hertz+Mindlin hopper"
+    if name == "hopper-bed":
+        p, x, r, m, meshes = scenes.hopper_bed_workload(n or 1_000_000)
+        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper, bed packed in the cone"
     if name == "box":
         p, x, r, m, meshes = scenes.box_settle_workload(n or 10_000)
         return p, x, r, m, None, meshes, "Hertz+Coulomb box"
@@ -102,13 +115,33 @@ def build_workload(name: str, n: int | None):
 
 
 def traffic_of(workload: str):
-    """HBM bytes per launch of the workload's force kernel from its committed
-    counter profile (FETCH_SIZE / WRITE_SIZE scaled by the copy probe), or None."""
-    tfile = os.path.join(ROOT, "profiles", f"{workload}_force_traffic.json")
+    """(HBM bytes per launch of the workload's force kernel, source): from its
+    committed counter profile (rocprofv3 FETCH_SIZE / WRITE_SIZE of an earlier
+    run, scaled by the copy probe in the same run), not measured live; None
+    when there is no profile."""
+    name = f"{workload}_force_traffic.json"
+    tfile = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(tfile):
-        return None
+        return None, None
     with open(tfile) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        return json.load(fh).get("bytes_per_launch"), f"profiles/{name} (rocprofv3 counters, earlier run)"
+
+
+def cpu_info() -> dict:
+    """Host CPU model and logical CPU count, and the threads the baseline may
+    use: the GPU box allots 16 host cores per GPU (its OMP_NUM_THREADS)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ncpu = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+    return {"cpu_model": model, "host_cpus": ncpu, "threads": max(1, min(ncpu, share))}
 
 
 def run_batch(args) -> int:
@@ -260,7 +293,7 @@ def run_batch(args) -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = args.cpu_threads or cpu_info()["threads"]
         sample = list(range(0, len(counts), max(len(counts) // 4, 1)))[:4]
         csec, cwork, csteps = 0.0, 0, []
         for k in sample:
@@ -286,7 +319,8 @@ def run_batch(args) -> int:
                                f"FP64", "particles": n, "instances": len(counts),
                    "form": "batched" if main == "plain" else f"{main}+batched"},
         "roofline": {"bound": "hbm", "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)},
+                     "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)[0],
+                     "traffic_source": traffic_of(args.workload)[1]},
         "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "repeats": len(R["repeat_values"]), "repeat_values": R["repeat_values"],
@@ -300,8 +334,7 @@ def run_batch(args) -> int:
         out["forms"] = {f: {k: q for k, q in res.items() if k not in ("stats", "wall")} for f, res in results.items()}
     if surr_info:
         out["surrogate"] = surr_info
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 def mesh_motions(workload: str):
@@ -311,7 +344,7 @@ def mesh_motions(workload: str):
 
 
 def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0,
-                 motions=()):
+                 motions=(), bits: int = 64):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
     largest contact cutoff (the cell-size heuristic of simulation.rs:64).
     Times at least `steps` steps, more when the warm-up step says that
@@ -323,7 +356,7 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, s
     from rusty_particles_amd import abi
     lib = oracle.lib()
     lib.rpo_set_threads(threads)
-    o = oracle.OracleSimulation(p.dt, None, None, params=abi.RpParams.from_buffer_copy(p))
+    o = oracle.OracleSimulation(p.dt, None, None, bits=bits, params=abi.RpParams.from_buffer_copy(p))
     o.set_state(x, r, m, v=v)
     for v9, n3 in meshes:
         o.add_mesh_arrays(v9, n3)
@@ -342,59 +375,12 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, s
     return x.shape[0] * steps / dt, dt, steps
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="steps per timed region (5000; drum 1000)")
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="hopper")
-    ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="least CPU-baseline steps")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0,
-                    help="CPU-baseline sample length: as many steps as fit, going by the warm-up step")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch every kernel on its own (no HIP-graph step loop), e.g. under a profiler")
-    ap.add_argument("--no-compare", action="store_true",
-                    help="batch workloads: skip the one-engine-per-instance comparison")
-    ap.add_argument("--timing-steps", type=int, default=100)
-    ap.add_argument("--repeats", type=int, default=3,
-                    help="timed regions of --steps steps, each on a fresh engine from the initial state; "
-                         "value = the median")
-    ap.add_argument("--jkr-forms", default=None,
-                    help="JKR solvers to time on the JKR workloads, the last one is the reported value "
-                         "(default exact,shortcut,surrogate; drum shortcut,surrogate)")
-    ap.add_argument("--train-samples", type=int, default=1_000_000)
-    ap.add_argument("--train-epochs", type=int, default=40)
-    ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
-    ap.add_argument("--autotune", type=int, default=None,
-                    help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
-                         "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
-                         "start at rest and speed up, so an early tuning picks a skin too small for later)")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
-                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
-    args = ap.parse_args()
-    if args.autotune is None:
-        args.autotune = 0 if args.workload in ("hopper", "box", "drum") else 20
-    if args.steps is None:
-        args.steps = 1000 if args.workload == "drum" else 5000
-    if args.jkr_forms is None:
-        args.jkr_forms = "shortcut,surrogate" if args.workload == "drum" else "exact,shortcut,surrogate"
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not "
-                                   "supported (use --gpus 1)"}))
-        return 2
-
+def run_single(args) -> dict:
+    """One workload (single engine): the forms, the timed regions, the kernel
+    roofline and the CPU baseline; returns the JSON object."""
     import numpy as np
 
     from rusty_particles_amd import abi, native
-    if native.device_count() < 1:
-        raise SystemExit("bench.py: no HIP device visible")
-    if args.workload in ("batch", "jkr-batch"):
-        return run_batch(args)
-
     t0 = time.perf_counter()
     p, x, r, m, v, meshes, desc = build_workload(args.workload, args.n)
     n = x.shape[0]
@@ -511,25 +497,54 @@ def main() -> int:
     ms = native.probe_copy(nbytes)
     copy_gbs = 2 * nbytes / ms / 1e6
     log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
-    traffic = traffic_of(args.workload)
+    traffic, traffic_src = traffic_of(args.workload)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        info = cpu_info()
+        threads = args.cpu_threads or info["threads"]
         cell = 2.0 * float(r.max()) * (1.5 if p.normal_model in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 1.0)
         # the drum's 16M spheres x 16,384 triangles (brute-force wall loop) take minutes per CPU step:
         # its sample is the bed's first 20,000 sites (bottom layers) in the same drum
         ns = n if args.workload != "drum" else min(n, 20_000)
         what = f"the full {n}-particle workload" if ns == n else f"the first {ns} of the {n} particles"
-        log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:g}s of {what}")
+        log(f"cpu baseline: oracle, {threads} threads ({info['host_cpus']} host CPUs, {info['cpu_model']}), "
+            f"~{args.cpu_seconds:g}s of {what}")
         cval, csec, csteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
                                           args.cpu_steps, threads, cell, args.cpu_seconds,
                                           mesh_motions(args.workload))
+        log(f"cpu baseline f64 {cval:.4g} particle-steps/s ({csec:.1f}s)")
+        # the reference's own precision (f32), the same sample
+        fval, fsec, fsteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
+                                          args.cpu_steps, threads, cell, 0.5 * args.cpu_seconds,
+                                          mesh_motions(args.workload), bits=32)
+        log(f"cpu baseline f32 {fval:.4g} particle-steps/s ({fsec:.1f}s)")
+        # as shipped: the reference's 1 m grid cell (simulation.rs:66), every
+        # walk appending all particles of up to 27 cells -- O(N^2) per step,
+        # so a sample of particles' gathers is timed and scaled to the workload
+        import oracle
+        oracle.lib().rpo_set_threads(threads)
+        o = oracle.OracleSimulation(p.dt, None, None, params=abi.RpParams.from_buffer_copy(p))
+        o.set_state(x[:ns], r[:ns], m[:ns], v=None if v is None else v[:ns])
+        nsamp = 2000 if ns <= 100_000 else 64
+        gsec = o.time_gather(nsamp, 1.0)
+        del o
+        aval = nsamp / gsec if gsec > 0 else None
+        log(f"cpu baseline as shipped (1 m cells): {nsamp} gathers in {gsec:.2f}s -> {aval:.4g} particle-steps/s")
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{csteps} step(s) of {what} after 1 warm-up step, "
-                         f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff), brute-force "
-                         f"{ntri}-triangle wall loop as in simulation.rs:465-523; {csec:.1f}s"}
-        log(f"cpu baseline {cval:.4g} particle-steps/s ({csec:.1f}s)")
+               "sample": f"{csteps} step(s) of {what} after 1 warm-up step, FP64, "
+                         f"reference grid cell {cell * 1e3:.2f} mm (largest contact cutoff, the cell-size "
+                         f"heuristic of simulation.rs:64), brute-force {ntri}-triangle wall loop as in "
+                         f"simulation.rs:465-523; {csec:.1f}s",
+               "cpu_model": info["cpu_model"], "host_cpus": info["host_cpus"],
+               "threads_note": "the GPU box allots 16 host cores per GPU (OMP_NUM_THREADS)",
+               "f32_value": fval, "f32_sample": f"{fsteps} step(s), the same sample in f32 (the reference's "
+                                                f"precision); {fsec:.1f}s",
+               "as_shipped_value": aval,
+               "as_shipped_sample": f"the gathers of {nsamp} particles spread over {what} with the reference's "
+                                    f"1 m grid cell (simulation.rs:66; each walk appends every particle of up to 27 "
+                                    f"cells), {gsec:.2f}s, scaled to particle-steps/s; walls and integration "
+                                    f"not included"}
 
     out = {
         "metric": "particle-steps/s on 1 GPU, plain vs surrogate+batched at 1e-3 force err; HBM GB/s",
@@ -547,7 +562,7 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {n} spheres, {ntri}-triangle STL, {desc}, dt={p.dt:g}, FP64",
                    "particles": n, "triangles": ntri, "form": main_form},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src},
         "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "repeats": len(R["repeat_values"]),
@@ -563,8 +578,113 @@ def main() -> int:
         out["forms"] = {f: {k: v for k, v in res.items() if k not in ("stats", "wall")} for f, res in results.items()}
     if surr_info:
         out["surrogate"] = surr_info
+    return out
+
+
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="steps per timed region (5000; drum 1000)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="hopper")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="least CPU-baseline steps")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU-baseline sample length: as many steps as fit, going by the warm-up step")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel on its own (no HIP-graph step loop), e.g. under a profiler")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch workloads: skip the one-engine-per-instance comparison")
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed regions of --steps steps, each on a fresh engine from the initial state; "
+                         "value = the median")
+    ap.add_argument("--jkr-forms", default=None,
+                    help="JKR solvers to time on the JKR workloads, the last one is the reported value "
+                         "(default exact,shortcut,surrogate; drum shortcut,surrogate)")
+    ap.add_argument("--train-samples", type=int, default=1_000_000)
+    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
+                         "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
+                         "start at rest and speed up, so an early tuning picks a skin too small for later)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="hopper: leave out the contact window and the surrogate+batched forms")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run the library's 256 MiB copy kernel so a counter profile of this run can "
+                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
+    args = ap.parse_args()
+    if args.autotune is None:
+        args.autotune = 0 if args.workload in ("hopper", "box", "drum") else 20
+    if args.steps is None:
+        args.steps = 1000 if args.workload == "drum" else 5000
+    if args.jkr_forms is None:
+        args.jkr_forms = "shortcut,surrogate" if args.workload == "drum" else "exact,shortcut,surrogate"
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not "
+                                   "supported (use --gpus 1)"}))
+        return 2
+
+    import numpy as np
+
+    from rusty_particles_amd import abi, native
+    if native.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+    if args.workload in ("batch", "jkr-batch"):
+        print(json.dumps(run_batch(args)), flush=True)
+        return 0
+    out = run_single(args)
+    if args.workload == "hopper" and not args.no_extras:
+        out["contact_window"] = contact_window(args)
+        out["surrogate_batched"] = surrogate_batched(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def contact_window(args) -> dict:
+    """The hopper-bed workload (the hopper's spheres packed in the cone) with
+    the same steps, warm-up and repeats: a window in which contacts and wall
+    contacts carry force.  Summary of its JSON object."""
+    a = argparse.Namespace(**vars(args))
+    a.workload, a.n, a.no_cpu, a.autotune = "hopper-bed", None, True, 0
+    log("contact window: hopper-bed workload")
+    r = run_single(a)
+    return {"workload": r["config"]["workload"], "particles": r["config"]["particles"], "value": r["value"],
+            "unit": "particle-steps/s", "ms_per_step": r["ms_per_step"], "contacts_per_step": r["contacts_per_step"],
+            "wall_contacts_per_step": r["wall_contacts_per_step"], "contacts_per_s": r["contacts_per_s"],
+            "force_kernel_avg_us": r["force_kernel_avg_us"], "roofline": r["roofline"],
+            "repeat_values": r["repeat_values"]}
+
+
+def surrogate_batched(args) -> dict:
+    """The jkr-batch workload (64 periodic JKR packings, surface-energy sweep)
+    in its three forms -- the reference's exact solve, the shortcut (identical
+    results) and the surrogate+batched form -- with the same steps and
+    warm-up, one timed region each, no autotuning (the same settings for every
+    form); the surrogate's force errors from check mode."""
+    a = argparse.Namespace(**vars(args))
+    a.workload, a.instances, a.jkr_forms, a.autotune = "jkr-batch", 64, "exact,shortcut,surrogate", 0
+    a.no_compare, a.no_cpu, a.repeats, a.timing_steps = True, True, 1, 50
+    log("surrogate+batched: jkr-batch workload")
+    r = run_batch(a)
+    f = r["forms"]
+    s = r.get("surrogate", {})
+    return {"workload": r["config"]["workload"], "particles": r["config"]["particles"],
+            "forms": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
+                          "force_kernel_avg_us": v["force_kernel_avg_us"],
+                          **({"surrogate_kernels_avg_us": v["surrogate_kernels_avg_us"]}
+                             if "surrogate_kernels_avg_us" in v else {})} for k, v in f.items()},
+            "surrogate_vs_exact": f["surrogate"]["value"] / f["exact"]["value"],
+            "surrogate_vs_shortcut": f["surrogate"]["value"] / f["shortcut"]["value"],
+            "force_rel_err_max": s.get("force_rel_err_max"), "force_rel_err_mean": s.get("force_rel_err_mean"),
+            "served_per_step": s.get("served_per_step"), "fallback_per_step": s.get("fallback_per_step"),
+            "train_s": s.get("train_s")}
+
 
 
 if __name__ == "__main__":

@@ -341,8 +341,8 @@ rp_status rp_engine_set_tiling(rp_engine* e, int32_t tiling);
 rp_status rp_engine_get_tiling(rp_engine* e, int32_t* tiling);
 
 /* Step loop in HIP graphs (on by default; no counterpart in the reference):
- * rp_engine_step replays a captured loop of 20 launches (two rebuild-chain
- * slots, the force kernels, the surrogate's kernels when active) where the
+ * rp_engine_step replays a captured loop of 10 launches (a rebuild-chain
+ * slot, the force kernels, the surrogate's kernels when active) where the
  * step count allows; the results are the same either way.  Timing runs
  * (rp_engine_set_timing) launch every kernel individually. */
 rp_status rp_engine_set_graph(rp_engine* e, int on);

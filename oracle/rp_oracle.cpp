@@ -146,6 +146,7 @@ class SimBase {
   virtual void cell_ids(int64_t* out) const = 0;
   virtual size_t history_total() const = 0;
   virtual void history(int64_t* offsets, int64_t* partner, double* xi3) const = 0;
+  virtual double time_gather(size_t nsample, double cell) const = 0;
 };
 
 template <class R>
@@ -597,6 +598,68 @@ class Sim : public SimBase {
 
  public:
   void set_fast(bool on) override { fast_ = on; }
+  // Seconds the reference's per-particle gather (simulation.rs:317-461: the
+  // grid walk of grid.rs:48-87 over cells of size `cell`, the pair test and
+  // the normal law of each contact) takes for nsample particles spread evenly
+  // over the current state, OpenMP over the sample; the state is unchanged.
+  // With the reference's 1 m cell each walk appends every particle of up to
+  // 27 cells, so a step costs O(N^2) and a sample is all that can be timed.
+  double time_gather(size_t nsample, double cell) const override {
+    const int64_t n = (int64_t)ps_.size();
+    if (n == 0 || nsample == 0) return 0.0;
+    Grid<R> g;
+    g.init(R(cell), bmin_, bmax_);
+    for (int64_t i = 0; i < n; ++i) g.insert(ps_[i].x, i);
+    const V3<R> bsize = bmax_ - bmin_;
+    const bool adhesive = normal_ == RP_NORMAL_JKR || normal_ == RP_NORMAL_SJKR;
+    const R estar = effective_youngs(pm_.E, pm_.nu, pm_.E, pm_.nu);
+    const R gamma = (pm_.gamma + pm_.gamma) * R(0.5);
+    const int64_t stride = std::max<int64_t>(1, n / (int64_t)nsample);
+    const int64_t ns = std::min<int64_t>((int64_t)nsample, n);
+    double sink = 0.0;
+    const double t0 = omp_get_wtime();
+#pragma omp parallel reduction(+ : sink)
+    {
+      std::vector<int64_t> nb;
+      uint64_t fails = 0;
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t q = 0; q < ns; ++q) {
+        const int64_t i = q * stride;
+        const Particle<R>& p = ps_[i];
+        g.neighbours(p.x, periodic_, nb);
+        V3<R> f = v3<R>(R(0), R(0), R(0));
+        for (int64_t j : nb) {
+          if (i == j) continue;
+          const Particle<R>& o = ps_[j];
+          V3<R> d = p.x - o.x;
+          if (periodic_[0]) {
+            if (d.x > bsize.x * R(0.5)) d.x -= bsize.x;
+            else if (d.x < -bsize.x * R(0.5)) d.x += bsize.x;
+          }
+          if (periodic_[1]) {
+            if (d.y > bsize.y * R(0.5)) d.y -= bsize.y;
+            else if (d.y < -bsize.y * R(0.5)) d.y += bsize.y;
+          }
+          if (periodic_[2]) {
+            if (d.z > bsize.z * R(0.5)) d.z -= bsize.z;
+            else if (d.z < -bsize.z * R(0.5)) d.z += bsize.z;
+          }
+          const R rsum = p.r + o.r;
+          const R cutoff = rsum + (adhesive ? rsum * R(0.5) : R(0));
+          const R dsq = length_sq(d);
+          if (!(dsq < cutoff * cutoff)) continue;
+          const R dist = std::sqrt(dsq);
+          const V3<R> nrm = dist > R(1e-6) ? d / dist : v3<R>(R(0), R(1), R(0));
+          f = f + normal_force(rsum - dist, nrm, p.v - o.v, estar, effective_radius(p.r, o.r),
+                               effective_mass(p.m, o.m), gamma, false, &fails);
+        }
+        sink += (double)f.x + (double)nb.size();
+      }
+    }
+    const double t = omp_get_wtime() - t0;
+    return sink == -1.0 ? -t : t;  // keeps the loop's result observable
+  }
+
   size_t history_total() const override {
     size_t t = 0;
     for (const auto& q : ps_) t += q.hist.size();
@@ -934,6 +997,9 @@ int rpo_overlap_flag(void* s) { return static_cast<SimBase*>(s)->overlap_flag();
 uint64_t rpo_jkr_failures(void* s) { return static_cast<SimBase*>(s)->jkr_failures(); }
 void rpo_cell_ids(void* s, int64_t* out) { static_cast<SimBase*>(s)->cell_ids(out); }
 size_t rpo_history_total(void* s) { return static_cast<SimBase*>(s)->history_total(); }
+double rpo_time_gather(void* s, size_t nsample, double cell) {
+  return static_cast<SimBase*>(s)->time_gather(nsample, cell);
+}
 void rpo_history(void* s, int64_t* offsets, int64_t* partner, double* xi3) {
   static_cast<SimBase*>(s)->history(offsets, partner, xi3);
 }

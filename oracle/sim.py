@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
     L.rpo_jkr_failures.restype = C.c_uint64
     L.rpo_set_fast.argtypes = [C.c_void_p, C.c_int]
     L.rpo_cell_ids.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.rpo_time_gather.argtypes = [C.c_void_p, C.c_size_t, C.c_double]
+    L.rpo_time_gather.restype = C.c_double
     L.rpo_history_total.argtypes = [C.c_void_p]
     L.rpo_history_total.restype = C.c_size_t
     L.rpo_history.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp]
@@ -244,6 +246,11 @@ class OracleSimulation:
         xi = np.zeros((max(tot, 1), 3))
         self._L.rpo_history(self._h, _p(offs, C.c_int64), _p(part, C.c_int64), _p(xi))
         return offs, part[:tot], xi[:tot]
+
+    def time_gather(self, nsample, cell=1.0):
+        """Seconds of the reference's literal gather (grid cell `cell`) for
+        nsample particles spread over the current state (state unchanged)."""
+        return float(self._L.rpo_time_gather(self._h, int(nsample), float(cell)))
 
     def set_fast(self, on=True):
         """Fast candidate enumeration: the same visits in the same order as the

@@ -87,8 +87,12 @@ constexpr int kScanChunk = kBlock * kScanItems;  // items per scan workgroup
 constexpr int kMaxGrid = 2048;                   // grid cap of the grid-stride rebuild kernels
 constexpr uint64_t kRebuildInterval = 10;        // steps between launches of the rebuild chain
 constexpr long long kNoStall = 0x7fffffffffffffffLL;  // no stalled step recorded
-constexpr uint64_t kGraphSteps = 2 * kRebuildInterval;  // launches per captured step loop (two rebuild chains,
-                                                        // so buffer and request parities repeat)
+// Launches per captured step loop: one rebuild chain and kRebuildInterval
+// force launches (an even number, so the state buffers are where they were at
+// the capture); the rebuild-request parity alternates from one replay to the
+// next, and each parity has its own executable (graph_exec).  Ten launches
+// let a 20-step call replay one loop.
+constexpr uint64_t kGraphSteps = kRebuildInterval;
 #ifndef RP_DEAL_BATCH
 #define RP_DEAL_BATCH 4
 #endif
@@ -2526,7 +2530,35 @@ void mark_end(rp_engine* e) {
   (void)hipEventRecord(e->ev_pool[e->ev_marks.back().idx + 1], e->stream);
 }
 
+// Per-instance envelope table of a batched engine's surrogate: built here,
+// before any launch of a step call, never while a stream capture records
+// (surr_dev only reads it: an allocation or synchronous copy inside a
+// capture would invalidate it).
+rp_status ensure_surr_inst(rp_engine* e) {
+  if (e->jkr_solver != RP_JKR_SURROGATE || e->ninst <= 1 || e->surr_inst_valid) return RP_OK;
+  const rp_jkr_envelope& v = e->surr_env;
+  const MlpScale sc = make_scale(v);
+  auto inside = [](double x, double lo, double hi) { return x >= lo * (1.0 - 1e-12) && x <= hi * (1.0 + 1e-12); };
+  std::vector<float4> tab((size_t)e->ninst);
+  for (int q = 0; q < e->ninst; ++q) {
+    const PairConsts c = material_consts(e->inst_particle[q], e->inst_wall[q]).pp;
+    tab[q].x = inside(c.estar, v.estar_lo, v.estar_hi) && inside(c.gamma, v.gamma_lo, v.gamma_hi) ? 1.f : 0.f;
+    tab[q].y = scale_in((float)std::log(c.estar), sc.in_lo[2], sc.in_hi[2]);
+    tab[q].z = scale_in((float)std::log(std::max(c.gamma, 1e-300)), sc.in_lo[3], sc.in_hi[3]);
+    tab[q].w = 0.f;
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  free_ptr(e->d_surr_inst);
+  e->d_surr_inst = nullptr;
+  HIP_TRY(hipMalloc(&e->d_surr_inst, tab.size() * sizeof(float4)));
+  HIP_TRY(hipMemcpy(e->d_surr_inst, tab.data(), tab.size() * sizeof(float4), hipMemcpyHostToDevice));
+  e->surr_inst_valid = true;
+  e->gen++;  // captured loops hold the table's address
+  return RP_OK;
+}
+
 // Device view of the surrogate state (null pointers when it is not in use).
+// Reads only: ensure_surr_inst has built the per-instance table.
 SurrDev surr_dev(rp_engine* e) {
   SurrDev sd{};
   if (e->jkr_solver != RP_JKR_SURROGATE) return sd;
@@ -2543,23 +2575,8 @@ SurrDev surr_dev(rp_engine* e) {
   sd.f_lne = scale_in((float)std::log(pp.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
   sd.f_lng = scale_in((float)std::log(std::max(pp.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
   if (e->ninst > 1) {
-    if (!e->surr_inst_valid) {
-      std::vector<float4> tab((size_t)e->ninst);
-      for (int q = 0; q < e->ninst; ++q) {
-        const PairConsts c = material_consts(e->inst_particle[q], e->inst_wall[q]).pp;
-        tab[q].x = inside(c.estar, v.estar_lo, v.estar_hi) && inside(c.gamma, v.gamma_lo, v.gamma_hi) ? 1.f : 0.f;
-        tab[q].y = scale_in((float)std::log(c.estar), sd.sc.in_lo[2], sd.sc.in_hi[2]);
-        tab[q].z = scale_in((float)std::log(std::max(c.gamma, 1e-300)), sd.sc.in_lo[3], sd.sc.in_hi[3]);
-        tab[q].w = 0.f;
-      }
-      free_ptr(e->d_surr_inst);
-      e->d_surr_inst = nullptr;
-      if (hipMalloc(&e->d_surr_inst, tab.size() * sizeof(float4)) == hipSuccess &&
-          hipMemcpy(e->d_surr_inst, tab.data(), tab.size() * sizeof(float4), hipMemcpyHostToDevice) == hipSuccess)
-        e->surr_inst_valid = true;
-    }
     sd.inst = e->d_surr_inst;
-    if (!e->surr_inst_valid) sd.pp_ok = 0;
+    if (!e->surr_inst_valid) sd.pp_ok = 0;  // not built: no instance is served
   }
   sd.base = e->d_surr_base;
   sd.rowmask = e->d_surr_rowmask;
@@ -3083,6 +3100,7 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
   if (nsteps == 0) return RP_OK;
   rp_status st = prepare(e);
   if (st) return st;
+  if ((st = ensure_surr_inst(e))) return st;
   const int gcap = (int)((e->cap + kBlock - 1) / kBlock);
   uint64_t remaining = nsteps;
   int stalled = 0;  // rounds that ended with stalled launches

@@ -297,7 +297,7 @@ class Engine:
         return {v: k for k, v in abi.TILINGS.items()}[t.value]
 
     def set_graph(self, on: bool) -> None:
-        """Replay captured 20-launch step loops (default on)."""
+        """Replay captured 10-launch step loops (default on)."""
         _check(self._L.rp_engine_set_graph(self._h, 1 if on else 0))
 
     def set_timing(self, on: bool) -> None:

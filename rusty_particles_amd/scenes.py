@@ -294,6 +294,41 @@ def hopper_test_workload(n: int = 20_000, seed: int = 0xD3E50012, clear: float =
     return p, x, r, m, [(hv9, hn3)]
 
 
+def hopper_bed_workload(n: int = 1_000_000, seed: int = 0xD3E50022, inflate: float = 1.08):
+    """The hopper configuration with its spheres packed in the cone instead of
+    poured above it -- a contact-bearing window of the same workload: n
+    spheres (r ~ U[0.5, 1.5] mm, rho 2500, Hertz + Mindlin, E=5e7, nu=0.25,
+    e=0.6, mu=0.4, dt=2e-6) placed by random sequential addition in a cone of
+    the same half-angle (30 deg), outlet (8 mm) and 4,096 triangles whose top
+    radius is 0.25 m (the 75 mm cone of hopper_workload holds ~36k such
+    spheres), centres 0.97 r from the wall, radii then grown by 8 % so that
+    near neighbours and the wall are in contact (about one contact per sphere;
+    overlaps below 8 % of a radius sum).  Returns (params, x, r, m, meshes)."""
+    rmin, rmax = 0.5e-3 / inflate, 1.5e-3 / inflate
+    top = 0.25
+    hv9, hn3 = hopper_mesh(HOPPER_OUTLET, HOPPER_HALF_ANGLE, top, y0=0.0, nseg=64, nring=32)
+    y_top = float(hv9[:, 1::3].max())
+    t = math.tan(math.radians(HOPPER_HALF_ANGLE))
+    mean_vol = 4.0 / 3.0 * np.pi * (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin))
+    need = n * mean_vol / 0.24  # bed height for RSA at solid fraction ~0.24
+    h = 0.01
+    while np.pi * h / 3.0 * (HOPPER_OUTLET ** 2 + HOPPER_OUTLET * (HOPPER_OUTLET + h * t)
+                             + (HOPPER_OUTLET + h * t) ** 2) < need:
+        h += 0.002
+    if h > y_top:
+        raise RuntimeError(f"a bed of {n} spheres does not fit the cone")
+    _lib().rps_set_progress(1 if n >= 500_000 else 0)
+    x, r = rsa_cone(n, rmin, rmax, 0.0, h, HOPPER_OUTLET, HOPPER_HALF_ANGLE, 0.97, seed)
+    _lib().rps_set_progress(0)
+    r = r * inflate
+    m = sphere_mass(r, 2500.0)
+    p = abi.default_params(2e-6, (-top - 0.01, -0.05, -top - 0.01), (top + 0.01, y_top + 0.01, top + 0.01))
+    p.normal_model = abi.RP_NORMAL_HERTZ
+    p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
+    p.particle = abi.RpMaterial(5e7, 0.25, 2500.0, 0.4, 0.6, 0.0)
+    return p, x, r, m, [(hv9, hn3)]
+
+
 def box_settle_workload(n: int = 10_000, seed: int = 0xD3E50001):
     """10,000 spheres r=5 mm, rho 2500, jittered lattice (spacing 11 mm, +-0.4 mm)
     in the box [-0.15, 0.15]^3 closed by a 12-triangle STL box, Hertz + Coulomb,
