@@ -144,6 +144,29 @@ def cpu_info() -> dict:
     return {"cpu_model": model, "host_cpus": ncpu, "threads": max(1, min(ncpu, share))}
 
 
+# The surrogate network as its MFMA tiles execute it: 4 inputs padded to 16,
+# two 64-wide hidden layers, 2 outputs padded to 16 (multiply + add = 2 flops).
+MLP_FLOPS_PER_ROW = 2 * (16 * 64 + 64 * 64 + 64 * 16)
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def surrogate_kernels(tm: dict, rows_per_step: float = None) -> dict:
+    """Average durations of the surrogate's collect and network launches from
+    an engine timing dict; with the rows the network ran on per step (the
+    contacts routed to the surrogate), its achieved BF16 TFLOP/s and the
+    fraction of the dense MFMA peak."""
+    (c_ms, c_n), (n_ms, n_n) = tm["collect"], tm["network"]
+    if not n_n:
+        return {}
+    out = {"collect_avg_us": c_ms / max(c_n, 1) * 1e3, "network_avg_us": n_ms / n_n * 1e3}
+    out["surrogate_kernels_avg_us"] = out["collect_avg_us"] + out["network_avg_us"]
+    if rows_per_step:
+        tflops = rows_per_step * MLP_FLOPS_PER_ROW / (out["network_avg_us"] * 1e-6) / 1e12
+        out.update({"network_rows_per_step": rows_per_step, "network_tflops": tflops,
+                    "network_mfma_frac": tflops / BF16_PEAK_TFLOPS})
+    return out
+
+
 def run_batch(args) -> int:
     """Batched ensembles: --workload batch (sJKR material sweep, plain form)
     and --workload jkr-batch (JKR surface-energy sweep, one line per JKR
@@ -235,8 +258,7 @@ def run_batch(args) -> int:
                "achieved_gbs": FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9, "contacts_per_step": st["contacts"],
                "contacts_per_s": st["contacts"] * args.steps / wall,
                "repeat_values": [n * args.steps / w for w in walls], "wall": wall, "stats": st}
-        if tm["surrogate"][1]:
-            res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
+        res.update(surrogate_kernels(tm))
         log(f"[{form}] batched: {args.steps} steps in {wall:.3f}s, {value:.4g} particle-steps/s, force kernel "
             f"{avg_s * 1e6:.1f} us")
         if form == "surrogate":
@@ -251,6 +273,7 @@ def run_batch(args) -> int:
                 "force_rel_err_max": max(q["err_max"] for q in errs),
                 "force_rel_err_mean": float(np.mean([q["err_mean"] for q in errs])),
             })
+            res.update(surrogate_kernels(tm, surr_info["served_per_step"] + surr_info["fallback_per_step"]))
             log(f"[surrogate] check: {surr_info}")
         e.close()
         results[form] = res
@@ -465,11 +488,11 @@ def run_single(args) -> dict:
                "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st,
                "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None,
                "tiling": eng.tiling()}
-        if tm["surrogate"][1]:
-            res["surrogate_kernels_avg_us"] = tm["surrogate"][0] / tm["surrogate"][1] * 1e3
+        res.update(surrogate_kernels(tm))
         log(f"[{form}] force kernel: {f_cnt} launches, avg {avg_s * 1e6:.1f} us, {achieved:.1f} GB/s algorithmic; "
             f"rebuild chain avg {tm['rebuild'][0] / max(tm['rebuild'][1], 1) * 1e3:.1f} us"
-            + (f"; collect+MLP avg {res['surrogate_kernels_avg_us']:.1f} us" if tm["surrogate"][1] else ""))
+            + (f"; collect avg {res['collect_avg_us']:.1f} us, network avg {res['network_avg_us']:.1f} us"
+               if "network_avg_us" in res else ""))
         if form == "surrogate":
             eng.set_jkr_solver("surrogate", net, check=True)
             errs = []
@@ -485,6 +508,7 @@ def run_single(args) -> dict:
                 "network_force_rel_err_mean": float(np.mean([e["raw_mean"] for e in errs])),
                 "rows_capacity": errs[-1]["rows_capacity"],
             })
+            res.update(surrogate_kernels(tm, surr_info["served_per_step"] + surr_info["fallback_per_step"]))
             log(f"[surrogate] check: {surr_info}")
         results[form] = res
     main_form = forms[-1]
@@ -677,8 +701,9 @@ def surrogate_batched(args) -> dict:
     return {"workload": r["config"]["workload"], "particles": r["config"]["particles"],
             "forms": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                           "force_kernel_avg_us": v["force_kernel_avg_us"],
-                          **({"surrogate_kernels_avg_us": v["surrogate_kernels_avg_us"]}
-                             if "surrogate_kernels_avg_us" in v else {})} for k, v in f.items()},
+                          **{q: v[q] for q in ("collect_avg_us", "network_avg_us", "network_rows_per_step",
+                                               "network_tflops", "network_mfma_frac") if q in v}}
+                      for k, v in f.items()},
             "surrogate_vs_exact": f["surrogate"]["value"] / f["exact"]["value"],
             "surrogate_vs_shortcut": f["surrogate"]["value"] / f["shortcut"]["value"],
             "force_rel_err_max": s.get("force_rel_err_max"), "force_rel_err_mean": s.get("force_rel_err_mean"),

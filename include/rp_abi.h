@@ -348,16 +348,17 @@ rp_status rp_engine_get_tiling(rp_engine* e, int32_t* tiling);
 rp_status rp_engine_set_graph(rp_engine* e, int on);
 
 /* Turns per-kernel timing on or off (off by default).  When on, every
- * force-kernel launch, every rebuild chain, every first-half kernel and every
- * surrogate collect + inference pair is bracketed by HIP events on the
- * engine's stream. */
+ * force-kernel launch, every rebuild chain, every first-half kernel, every
+ * surrogate feature collection and every surrogate inference launch is
+ * bracketed by HIP events on the engine's stream. */
 rp_status rp_engine_set_timing(rp_engine* e, int on);
 
 /* Synchronises, then returns the summed event times in milliseconds and the
  * number of timed launches since the last call, in arrays of RP_TIMING_KINDS:
  * index 0 = force kernel, 1 = rebuild chain (including its no-op launches),
- * 2 = first half kernel, 3 = JKR surrogate (collect + network). */
-#define RP_TIMING_KINDS 4
+ * 2 = first half kernel, 3 = JKR surrogate feature collection, 4 = JKR
+ * surrogate network inference (BF16 MFMA). */
+#define RP_TIMING_KINDS 5
 rp_status rp_engine_timing(rp_engine* e, double* ms, uint64_t* count);
 
 /* Diagnostics, no counterpart in the reference: copies `bytes` bytes

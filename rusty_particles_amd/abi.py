@@ -74,7 +74,7 @@ class RpParams(C.Structure):
 
 
 RP_MLP_PARAMS = 4610
-RP_TIMING_KINDS = 4
+RP_TIMING_KINDS = 5
 RP_JKR_EXACT, RP_JKR_SHORTCUT, RP_JKR_SURROGATE = 0, 1, 2
 RP_TILING_LANE, RP_TILING_WAVE = 0, 1
 TILINGS = {"lane": RP_TILING_LANE, "wave": RP_TILING_WAVE}

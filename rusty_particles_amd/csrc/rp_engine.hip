@@ -2492,7 +2492,8 @@ rp_status ensure_pairs(rp_engine* e) {
   return RP_OK;
 }
 
-// kind: 0 = force kernel, 1 = rebuild chain, 2 = first half (k_begin)
+// kind: 0 = force kernel, 1 = rebuild chain, 2 = first half (k_begin),
+// 3 = surrogate feature collection, 4 = surrogate network
 hipEvent_t next_event(rp_engine* e) {
   if (e->ev_used == e->ev_pool.size()) {
     hipEvent_t ev;
@@ -2606,12 +2607,24 @@ bool surr_active(const rp_engine* e) { return e->jkr_solver == RP_JKR_SURROGATE 
 
 // Feature rows of the surrogate-served contacts of the next force launch,
 // then the network on them.
+// The surrogate's two launches before a force launch: feature rows of the
+// served contacts, then the network on them.  Timing mode brackets each.
 void launch_surrogate(rp_engine* e) {
   const SurrDev sd = surr_dev(e);
   const int g = (int)((e->cap + kBlock - 1) / kBlock);
+  mark_begin(e, 3);
   hipLaunchKernelGGL(k_jkr_collect, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, label_off(e),
                      e->kp, e->dyn[e->cur], e->stat[0], e->d_nbr_count, e->d_nbr, sd);
-  mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream);
+  mark_end(e);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (e->timing) {  // timed by its own dispatch, like the force kernel
+    const size_t idx = e->ev_used;
+    a = next_event(e);
+    b = next_event(e);
+    if (a && b) e->ev_marks.push_back(rp_engine::Mark{4, idx, 1});
+    else a = b = nullptr;
+  }
+  mlp_infer_counted(e->surr_net.dev(), sd.x, sd.y, sd.counter, sd.rows_cap, e->stream, a, b);
 }
 
 template <int NORMAL, int TANG, bool ADV, bool REC, int JS, bool BATCH, bool DEAL, bool HIST>
@@ -3134,11 +3147,7 @@ rp_status rp_engine_step(rp_engine* e, uint64_t nsteps) {
       }
       const int slot = (int)(e->chain_no & 1);
       const bool last = final_chunk && s + 1 == chunk;
-      if (surr_active(e) && !(last && e->record)) {
-        mark_begin(e, 3);
-        launch_surrogate(e);
-        mark_end(e);
-      }
+      if (surr_active(e) && !(last && e->record)) launch_surrogate(e);
       mark_begin(e, 0);
       if (last) {
         if (e->record) launch_force<false, true>(e, slot);

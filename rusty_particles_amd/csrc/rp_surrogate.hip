@@ -1,18 +1,23 @@
 // rp_surrogate.hip — BF16 MFMA training and inference of the JKR surrogate
 // (see rp_surrogate.hpp for the model).
 //
-// Matrix products use v_mfma_f32_16x16x16_bf16: one wave computes a 16x16
-// FP32 tile of op(A) * op(B) from BF16 operands.  Operand layout per lane l
-// (64 lanes): A supplies A[l % 16][k0 + 4 (l / 16) + q], B supplies
-// B[k0 + 4 (l / 16) + q][l % 16], q = 0..3, and the accumulator holds
-// D[4 (l / 16) + q][l % 16].  rp_mfma_probe exposes one tile so a test can
-// check this layout against a host product.
+// Matrix products: one wave computes a 16x16 FP32 tile of op(A) * op(B)
+// from BF16 operands.  v_mfma_f32_16x16x16_bf16 (training, and inference's
+// first layer, whose 4 inputs are padded to K = 16): per lane l (64 lanes), A
+// supplies A[l % 16][k0 + 4 (l / 16) + q], B supplies B[k0 + 4 (l / 16) + q][l % 16],
+// q = 0..3.  gfx950's v_mfma_f32_16x16x32_bf16 (inference's 64-wide layers)
+// takes K = 32 per instruction: A[l % 16][k0 + 8 (l / 16) + q] and
+// B[k0 + 8 (l / 16) + q][l % 16], q = 0..7.  Both accumulate
+// D[4 (l / 16) + q][l % 16], q = 0..3.  rp_mfma_probe exposes one 16x16x16
+// tile so a test can check that layout against a host product; the 16x16x32
+// layout is checked by the inference tests against a host forward pass.
 //
 // Training (all on the GPU): samples (d, R*, E*, gamma) drawn over the served
 // envelope (sqrt(d - d_min) uniform, the others log-uniform) are labelled by the FP64
 // Newton solve of the reference (rp::jkr, physics.rs:132-219); mini-batch
 // Adam on the mean squared error of the scaled outputs; forward and backward
 // products on MFMA with FP32 accumulation, FP32 master weights.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +34,7 @@
 namespace rp {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // round to nearest even
@@ -114,36 +120,45 @@ __device__ inline float tanh_fast(float x) {
 // One lane's B operands (weights in fragment order, see mlp_upload) and
 // biases, loaded once per workgroup and reused by every tile it runs.
 struct MlpLane {
-  bf16x4 w1[4], w2[16], w3[4];
+  bf16x4 w1[4];          // layer 1, K = 16: one 16x16x16 tile per output tile
+  bf16x8 w2[8], w3[2];   // layers 2 and 3, K = 64: two 16x16x32 tiles along K
   float b1[4], b2[4], b3;
 };
 
 __device__ inline void mlp_lane_load(const MlpDev& net, MlpLane& L) {
   const int lane = threadIdx.x & 63, li = lane & 15;
   const uint2* w1 = reinterpret_cast<const uint2*>(net.w1);
-  const uint2* w2 = reinterpret_cast<const uint2*>(net.w2);
-  const uint2* w3 = reinterpret_cast<const uint2*>(net.w3);
+  const uint4* w2 = reinterpret_cast<const uint4*>(net.w2);
+  const uint4* w3 = reinterpret_cast<const uint4*>(net.w3);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     L.w1[t] = __builtin_bit_cast(bf16x4, w1[t * 64 + lane]);
-    L.w3[t] = __builtin_bit_cast(bf16x4, w3[t * 64 + lane]);
     L.b1[t] = net.b1[t * 16 + li];
     L.b2[t] = net.b2[t * 16 + li];
   }
 #pragma unroll
-  for (int t = 0; t < 16; ++t) L.w2[t] = __builtin_bit_cast(bf16x4, w2[t * 64 + lane]);
+  for (int t = 0; t < 8; ++t) L.w2[t] = __builtin_bit_cast(bf16x8, w2[t * 64 + lane]);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) L.w3[t] = __builtin_bit_cast(bf16x8, w3[t * 64 + lane]);
   L.b3 = net.b3[li];
 }
 
+// Hidden activations of one wave's 16 rows, BF16 (the precision the next
+// layer's operands have anyway), 8 elements of padding per row so that the
+// 16-byte operand reads of the 16 rows spread over the LDS banks.
+constexpr int kHidStride = kMlpHid + 8;
+
 // Layer outputs go through LDS to turn the accumulator layout (row = 4(l/16)+q)
-// into the A-operand layout (row = l % 16) of the next layer.
+// into the A-operand layout (row = l % 16) of the next layer; each value is
+// rounded to BF16 once, when it is stored (v_cvt_pk_bf16_f32, round to
+// nearest even).
 // Rows [16 tile, 16 tile + 16) of one wave; every wave of the workgroup
 // calls it together (it contains workgroup barriers).  Waves past the last
 // row compute on zeros and store nothing.
 __device__ inline void mlp_tile(const MlpLane& L, const float* __restrict__ x, float* __restrict__ y, int rows,
-                                int tile, float (*h)[kMlpHid + 1]) {
+                                int tile, __bf16 (*h)[kHidStride]) {
   const int lane = threadIdx.x & 63;
-  const int li = lane & 15, kq = 4 * (lane >> 4);
+  const int li = lane & 15, kq = 4 * (lane >> 4), k8 = 8 * (lane >> 4);
   const int r0 = tile * 16;
   // layer 1: [16 x 16(4)] x W1 [16 x 64]
   bf16x4 a;
@@ -151,43 +166,39 @@ __device__ inline void mlp_tile(const MlpLane& L, const float* __restrict__ x, f
   for (int q = 0; q < 4; ++q) {
     const int k = kq + q, r = r0 + li;
     const float v = (k < kMlpIn && r < rows) ? x[(size_t)r * kMlpIn + k] : 0.0f;
-    a[q] = (short)f2bf(v);
+    a[q] = __builtin_bit_cast(short, (__bf16)v);
   }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, L.w1[nt], acc, 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanh_fast(acc[q] + L.b1[nt]);
+    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = (__bf16)tanh_fast(acc[q] + L.b1[nt]);
   }
   __syncthreads();
-  // layer 2: [16 x 64] x W2 [64 x 64]
+  // layer 2: [16 x 64] x W2 [64 x 64], two K = 32 steps
   f32x4 acc2[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) acc2[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kt = 0; kt < 4; ++kt) {
-    bf16x4 a2;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a2[q] = (short)f2bf(h[li][kt * 16 + kq + q]);
+  for (int kt = 0; kt < 2; ++kt) {
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(&h[li][kt * 32 + k8]);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, L.w2[kt * 4 + nt], acc2[nt], 0, 0, 0);
+      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, L.w2[kt * 4 + nt], acc2[nt], 0, 0, 0);
   }
   __syncthreads();
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = tanh_fast(acc2[nt][q] + L.b2[nt]);
+    for (int q = 0; q < 4; ++q) h[kq + q][nt * 16 + li] = (__bf16)tanh_fast(acc2[nt][q] + L.b2[nt]);
   __syncthreads();
-  // layer 3: [16 x 64] x W3 [64 x 16(2)]
+  // layer 3: [16 x 64] x W3 [64 x 16(2)], two K = 32 steps
   f32x4 acc3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kt = 0; kt < 4; ++kt) {
-    bf16x4 a3;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a3[q] = (short)f2bf(h[li][kt * 16 + kq + q]);
-    acc3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, L.w3[kt], acc3, 0, 0, 0);
+  for (int kt = 0; kt < 2; ++kt) {
+    const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(&h[li][kt * 32 + k8]);
+    acc3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, L.w3[kt], acc3, 0, 0, 0);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -199,7 +210,7 @@ __device__ inline void mlp_tile(const MlpLane& L, const float* __restrict__ x, f
 
 __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __restrict__ x,
                                                    float* __restrict__ y, int rows) {
-  __shared__ float hs[4][16][kMlpHid + 1];
+  __shared__ __align__(16) __bf16 hs[4][16][kHidStride];
   MlpLane L;
   mlp_lane_load(net, L);
   mlp_tile(L, x, y, rows, blockIdx.x * 4 + (threadIdx.x >> 6), hs[threadIdx.x >> 6]);
@@ -212,7 +223,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __re
 // for every wave of a workgroup and all meet the barriers.
 __global__ void __launch_bounds__(256) k_mlp_counted(MlpDev net, const float* __restrict__ x,
                                                      float* __restrict__ y, const unsigned* counter, int cap) {
-  __shared__ float hs[4][16][kMlpHid + 1];
+  __shared__ __align__(16) __bf16 hs[4][16][kHidStride];
   MlpLane L;
   mlp_lane_load(net, L);
   const int range_rows = cap / kSurrRanges;
@@ -557,23 +568,26 @@ int mlp_upload(MlpHost& m, const float* params, hipStream_t stream) {
   if (!m.b && hipMalloc(&m.b, (64 + 64 + 16) * sizeof(float)) != hipSuccess) return RP_ERR_NOMEM;
   std::vector<float> pp;
   pad_params(params, pp);
-  // Fragment order: lane l of tile t holds the MFMA B operand
-  // W[k0 + 4 (l / 16) + q][n0 + l % 16], q = 0..3, at 4 (64 t + l) + q, so
-  // each lane loads its operand of a tile as one 8-byte word.
+  // Fragment order: lane l of W1's tile nt holds the 16x16x16 B operand
+  // W1[4 (l / 16) + q][16 nt + l % 16], q = 0..3, at 4 (64 nt + l) + q; lane
+  // l of W2's tile t = 4 kt + nt holds the 16x16x32 B operand
+  // W2[32 kt + 8 (l / 16) + q][16 nt + l % 16], q = 0..7, at 8 (64 t + l) + q,
+  // and of W3's tile kt W3[32 kt + 8 (l / 16) + q][l % 16]; each lane loads
+  // its operand of a tile as one 8- or 16-byte word.
   std::vector<uint16_t> wb(16 * 64 + 64 * 64 + 64 * 16);
   uint16_t* f1 = wb.data();
   uint16_t* f2 = f1 + 16 * 64;
   uint16_t* f3 = f2 + 64 * 64;
   for (int l = 0; l < 64; ++l) {
-    const int li = l % 16, kq = 4 * (l / 16);
-    for (int q = 0; q < 4; ++q) {
+    const int li = l % 16, kq = 4 * (l / 16), k8 = 8 * (l / 16);
+    for (int q = 0; q < 4; ++q)
       for (int nt = 0; nt < 4; ++nt) f1[4 * (64 * nt + l) + q] = f2bf(pp[kPW1 + (kq + q) * 64 + nt * 16 + li]);
-      for (int kt = 0; kt < 4; ++kt) {
+    for (int q = 0; q < 8; ++q)
+      for (int kt = 0; kt < 2; ++kt) {
         for (int nt = 0; nt < 4; ++nt)
-          f2[4 * (64 * (kt * 4 + nt) + l) + q] = f2bf(pp[kPW2 + (kt * 16 + kq + q) * 64 + nt * 16 + li]);
-        f3[4 * (64 * kt + l) + q] = f2bf(pp[kPW3 + (kt * 16 + kq + q) * 16 + li]);
+          f2[8 * (64 * (kt * 4 + nt) + l) + q] = f2bf(pp[kPW2 + (kt * 32 + k8 + q) * 64 + nt * 16 + li]);
+        f3[8 * (64 * kt + l) + q] = f2bf(pp[kPW3 + (kt * 32 + k8 + q) * 16 + li]);
       }
-    }
   }
   std::vector<float> bb(64 + 64 + 16);
   for (int i = 0; i < 64; ++i) bb[i] = pp[kPB1 + i];
@@ -599,9 +613,12 @@ void mlp_infer(const MlpDev& net, const float* x, float* y, int rows, hipStream_
 }
 
 void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsigned* counter, int cap,
-                       hipStream_t stream) {
+                       hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
   const int blocks = std::max(1, std::min(1024, (cap + 63) / 64));
-  hipLaunchKernelGGL(k_mlp_counted, dim3(blocks), dim3(256), 0, stream, net, x, y, counter, cap);
+  if (start && stop)
+    hipExtLaunchKernelGGL(k_mlp_counted, dim3(blocks), dim3(256), 0, stream, start, stop, 0u, net, x, y, counter, cap);
+  else
+    hipLaunchKernelGGL(k_mlp_counted, dim3(blocks), dim3(256), 0, stream, net, x, y, counter, cap);
 }
 
 // ---- inference without allocation (rp_mlp_infer_noalloc) ---------------------
@@ -630,18 +647,21 @@ __global__ void __launch_bounds__(256) k_pack_net(const float* __restrict__ p, u
     if (part == 1) base = 16 * 64;
     if (part == 2) base = 16 * 64 + 64 * 64;
     j -= base;
-    const int q = j & 3, rest = j >> 2, l = rest & 63, t = rest >> 6;
-    const int li = l & 15, kq = 4 * (l >> 4);
     float v;
-    if (part == 0) {  // W1 tile nt = t: row kq + q, column 16 t + li
-      const int row = kq + q;
-      v = row < kMlpIn ? p[kOffW1 + row * kMlpHid + 16 * t + li] : 0.0f;
-    } else if (part == 1) {  // W2 tile t = 4 kt + nt
-      const int row = (t >> 2) * 16 + kq + q;
-      v = p[kOffW2 + row * kMlpHid + (t & 3) * 16 + li];
-    } else {  // W3 tile kt = t: row 16 t + kq + q, column li
-      const int row = t * 16 + kq + q;
-      v = li < kMlpOut ? p[kOffW3 + row * kMlpOut + li] : 0.0f;
+    if (part == 0) {  // W1 tile nt = t: row 4 (l / 16) + q, column 16 t + l % 16
+      const int q = j & 3, rest = j >> 2, l = rest & 63, t = rest >> 6;
+      const int row = 4 * (l >> 4) + q;
+      v = row < kMlpIn ? p[kOffW1 + row * kMlpHid + 16 * t + (l & 15)] : 0.0f;
+    } else {
+      const int q = j & 7, rest = j >> 3, l = rest & 63, t = rest >> 6;
+      const int li = l & 15, k8 = 8 * (l >> 4);
+      if (part == 1) {  // W2 tile t = 4 kt + nt: row 32 kt + 8 (l / 16) + q, column 16 nt + l % 16
+        const int row = (t >> 2) * 32 + k8 + q;
+        v = p[kOffW2 + row * kMlpHid + (t & 3) * 16 + li];
+      } else {  // W3 tile kt = t: row 32 t + 8 (l / 16) + q, column l % 16
+        const int row = t * 32 + k8 + q;
+        v = li < kMlpOut ? p[kOffW3 + row * kMlpOut + li] : 0.0f;
+      }
     }
     w[idx] = f2bf(v);
   } else if (idx < kPackedW + kPackedB) {

@@ -154,8 +154,9 @@ void mlp_infer(const MlpDev& net, const float* x, float* y, int rows, hipStream_
 
 // The same on the claimed rows of each of the kSurrRanges ranges of
 // [0, cap): counter[r * kSurrCounterStride] rows from r cap / kSurrRanges
-// (counts read on the device).
+// (counts read on the device).  With an event pair, the dispatch records
+// its start and end in them (hipExtLaunchKernel).
 void mlp_infer_counted(const MlpDev& net, const float* x, float* y, const unsigned* counter, int cap,
-                       hipStream_t stream);
+                       hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 }  // namespace rp

@@ -308,7 +308,7 @@ class Engine:
         ms = np.zeros(abi.RP_TIMING_KINDS)
         cnt = (C.c_uint64 * abi.RP_TIMING_KINDS)()
         _check(self._L.rp_engine_timing(self._h, _ptr(ms), cnt))
-        names = ("force", "rebuild", "begin", "surrogate")
+        names = ("force", "rebuild", "begin", "collect", "network")
         return {names[k]: (float(ms[k]), int(cnt[k])) for k in range(abi.RP_TIMING_KINDS)}
 
     def set_jkr_solver(self, solver: str, surrogate=None, check: bool = False) -> None:
