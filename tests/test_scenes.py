@@ -1,4 +1,5 @@
-"""Procedural STL generators: the drum of the 16M-sphere capacity run.
+"""Procedural STL generators: the drum of the 16M-sphere capacity run and
+the 4,096-triangle cone of the hopper configuration.
 
 Checks fixed by geometry, not by the generator's own formulas:
 * every edge belongs to exactly two triangles (a closed surface);
@@ -6,7 +7,12 @@ Checks fixed by geometry, not by the generator's own formulas:
   (centroid . n) * area over a closed surface with inward normals is -3 V,
   and V of the polygonal drum is (nseg / 2) R^2 sin(2 pi / nseg) L;
 * the triangle count of the capacity run (16,384) and the binary and ASCII
-  files read back by the library's reader (mesh.rs:125-143 semantics).
+  files read back by the library's reader (mesh.rs:125-143 semantics);
+* the hopper cone: every vertex on the cone r = r_out + y tan(30 deg), the
+  outlet and top rims, the half-angle between a generator line and the axis,
+  interior edges shared by two triangles and rim edges by one, unit normals
+  perpendicular to their triangle and pointing into the hopper
+  (n = (-cos a cos t, sin a, -cos a sin t) for half-angle a at azimuth t).
 """
 import math
 
@@ -61,3 +67,51 @@ def test_drum_stl_round_trip(tmp_path, fmt):
     (scenes.write_stl_binary if fmt == "binary" else scenes.write_stl_ascii)(path, v9, n3)
     a9, a3 = native.read_stl(path)
     assert np.array_equal(a9, v9) and np.array_equal(a3, n3)
+
+
+def test_hopper_cone_geometry():
+    r_out, alpha, r_top = scenes.HOPPER_OUTLET, scenes.HOPPER_HALF_ANGLE, scenes.HOPPER_TOP
+    v9, n3 = scenes.hopper_mesh(r_out, alpha, r_top, y0=0.0, nseg=64, nring=32)
+    assert v9.shape == (4096, 9) and n3.shape == (4096, 3)
+    ta = math.tan(math.radians(alpha))
+    height = (r_top - r_out) / ta
+    v = v9.reshape(-1, 3).astype(np.float64)
+    rho = np.hypot(v[:, 0], v[:, 2])
+    assert np.allclose(rho, r_out + v[:, 1] * ta, rtol=0, atol=1e-7)
+    assert abs(v[:, 1].min()) < 1e-9 and abs(v[:, 1].max() - height) < 1e-7
+    bottom, top = v[:, 1] < 1e-9, v[:, 1] > height - 1e-7
+    assert np.allclose(rho[bottom], r_out, atol=1e-8) and np.allclose(rho[top], r_top, atol=1e-7)
+    # half-angle of a generator line: outlet vertex to top vertex at the same azimuth
+    a = np.array([r_out, 0.0, 0.0])
+    b = np.array([r_top, height, 0.0])
+    assert math.degrees(math.atan2(b[0] - a[0], b[1] - a[1])) == pytest.approx(alpha, abs=1e-9)
+    # open at both rims, closed in between (the seam at azimuth 0 / 2 pi
+    # matches to float32 rounding: positions compared on a 1 nm grid)
+    q = np.round(v9.astype(np.float64).reshape(-1, 3) * 1e9).astype(np.int64).reshape(-1, 3, 3)
+    keys = {}
+    ids = np.empty(q.shape[:2], np.int64)
+    for t in range(q.shape[0]):
+        for k in range(3):
+            ids[t, k] = keys.setdefault(tuple(q[t, k].tolist()), len(keys))
+    e = np.sort(np.concatenate([ids[:, [0, 1]], ids[:, [1, 2]], ids[:, [2, 0]]]), axis=1)
+    uniq, counts = np.unique(e, axis=0, return_counts=True)
+    yv = np.zeros(len(keys))
+    for key, i in keys.items():
+        yv[i] = key[1] * 1e-9
+    rim = (np.abs(yv[uniq[:, 0]] - yv[uniq[:, 1]]) < 1e-9) & ((yv[uniq[:, 0]] < 1e-9) | (yv[uniq[:, 0]] > height - 1e-7))
+    assert np.all(counts[rim] == 1) and np.all(counts[~rim] == 2)
+    assert rim.sum() == 2 * 64
+    # normals: unit, perpendicular to the triangle, into the hopper
+    nn = n3.astype(np.float64)
+    assert np.allclose(np.linalg.norm(nn, axis=1), 1.0, atol=1e-6)
+    p0, p1, p2 = v9[:, 0:3].astype(np.float64), v9[:, 3:6].astype(np.float64), v9[:, 6:9].astype(np.float64)
+    for d in (p1 - p0, p2 - p0):
+        assert np.abs((nn * d).sum(1)).max() < 1e-5 * np.linalg.norm(d, axis=1).max()
+    cen = (p0 + p1 + p2) / 3.0
+    az = np.arctan2(cen[:, 2], cen[:, 0])
+    ca, sa = math.cos(math.radians(alpha)), math.sin(math.radians(alpha))
+    want = np.stack([-ca * np.cos(az), np.full(az.size, sa), -ca * np.sin(az)], 1)
+    # facets of a 64-gon: each normal is the cone normal at its facet's own
+    # azimuth, within half a segment's angle
+    assert np.all((nn * want).sum(1) > math.cos(math.pi / 64) - 1e-6)
+    assert np.allclose(nn[:, 1], sa, atol=1e-3)
