@@ -471,18 +471,39 @@ __global__ void __launch_bounds__(kBlock) k_keys(Ctl* ctl, int par, KParams P, D
   }
 }
 
+// The kScanItems counts from `base` (a multiple of kScanItems, so 64-byte
+// aligned): four 16-byte loads when all lie below m, zeros past m.
+__device__ inline void load_items(const uint32_t* in, int base, int m, uint32_t* vals) {
+  if (base + kScanItems <= m) {
+    const uint4* p4 = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int q = 0; q < kScanItems / 4; ++q) {
+      const uint4 u = p4[q];
+      vals[4 * q] = u.x;
+      vals[4 * q + 1] = u.y;
+      vals[4 * q + 2] = u.z;
+      vals[4 * q + 3] = u.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) vals[k] = base + k < m ? in[base + k] : 0u;
+  }
+}
+
 // Exclusive scan of cell_count[0..m) into cell_start[0..m], cell_start[m] = total.
 __global__ void __launch_bounds__(kBlock) k_scan_sums(const Ctl* ctl, int par, const uint32_t* in, int m,
                                                       uint32_t* block_sums) {
   if (!ctl->rebuild[par]) return;
   __shared__ uint32_t red[kBlock];
   for (int b = blockIdx.x; b * kScanChunk < m; b += gridDim.x) {
+    // thread t sums the kScanItems items from b*chunk + t*items (64 bytes,
+    // four 16-byte loads), the same items k_scan_apply gives it
+    const int base = b * kScanChunk + threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems];
+    load_items(in, base, m, vals);
     uint32_t s = 0;
-    int base = b * kScanChunk + threadIdx.x;
-    for (int k = 0; k < kScanItems; ++k) {
-      int idx = base + k * kBlock;
-      if (idx < m) s += in[idx];
-    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += vals[k];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
@@ -525,14 +546,12 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const Ctl* ctl, int par, 
   __shared__ uint32_t tsum[kBlock];
   for (int b = blockIdx.x; b * kScanChunk < m; b += gridDim.x) {
     // thread t owns items [b*chunk + t*items, +items): contiguous per thread
-    int base = b * kScanChunk + threadIdx.x * kScanItems;
+    const int base = b * kScanChunk + threadIdx.x * kScanItems;
     uint32_t vals[kScanItems];
+    load_items(in, base, m, vals);
     uint32_t s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-      int idx = base + k;
-      vals[k] = idx < m ? in[idx] : 0;
-      s += vals[k];
-    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += vals[k];
     tsum[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < kBlock; off <<= 1) {
@@ -542,11 +561,24 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const Ctl* ctl, int par, 
       __syncthreads();
     }
     uint32_t run = block_offsets[b] + tsum[threadIdx.x] - s;
-    for (int k = 0; k < kScanItems; ++k) {
-      int idx = base + k;
-      if (idx < m) out[idx] = run;
-      if (idx == m - 1) out[m] = run + vals[k];
-      run += vals[k];
+    if (base + kScanItems <= m) {
+      uint32_t ex[kScanItems];
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k) {
+        ex[k] = run;
+        run += vals[k];
+      }
+      uint4* o4 = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+      for (int q = 0; q < kScanItems / 4; ++q) o4[q] = make_uint4(ex[4 * q], ex[4 * q + 1], ex[4 * q + 2], ex[4 * q + 3]);
+      if (base + kScanItems == m) out[m] = run;
+    } else {
+      for (int k = 0; k < kScanItems; ++k) {
+        int idx = base + k;
+        if (idx < m) out[idx] = run;
+        if (idx == m - 1) out[m] = run + vals[k];
+        run += vals[k];
+      }
     }
     __syncthreads();
   }
@@ -575,6 +607,9 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   if (!ctl->rebuild[par]) return;
   const int n = ctl->n;
   const int cap = P.cap;
+  // a and tau are read only by the first half of the next call (k_begin
+  // clears at_rest once it has used them): moved only while they are live
+  const bool keep_at = ctl->at_rest != 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t c = key[i];
     if (c >= P.ncells_all) continue;
@@ -590,8 +625,10 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
       D.rx[d + o] = S.rx[i + o];
       D.rv[d + o] = S.rv[i + o];
       D.rw[d + o] = S.rw[i + o];
-      D.a[d + o] = S.a[i + o];
-      D.tau[d + o] = S.tau[i + o];
+      if (keep_at) {
+        D.a[d + o] = S.a[i + o];
+        D.tau[d + o] = S.tau[i + o];
+      }
     }
     reinterpret_cast<double4*>(D.xr)[d] = reinterpret_cast<const double4*>(S.xr)[i];
     U.r[d] = T.r[i];
@@ -603,6 +640,80 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   }
 }
 
+// Candidates of slot i in the slots [s0, s1) of the re-ordered scratch
+// buffer D (one or more cells with consecutive numbers): appends every j with
+// |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin to i's list (the radius
+// comes with the position record); loads issued four at a time.
+__device__ inline int build_range(const KParams& P, const Dyn& D, uint32_t s0, uint32_t s1, int i, dv3 xi,
+                                  double ri, int* nbr, int cnt) {
+  for (uint32_t j0 = s0; j0 < s1; j0 += 4) {
+    dv3 xq[4];
+    double rq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ldxr(D.xr, (int)min(j0 + (uint32_t)q, s1 - 1u), xq[q], rq[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t j = j0 + (uint32_t)q;
+      if (j >= s1 || (int)j == i) continue;
+      dv3 d = sub(xi, xq[q]);
+      if (P.periodic[0]) {
+        if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
+        else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
+      }
+      if (P.periodic[1]) {
+        if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
+        else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
+      }
+      if (P.periodic[2]) {
+        if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
+        else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
+      }
+      const double lim = P.margin_factor * (ri + rq[q]) + P.skin;
+      if (dot(d, d) < lim * lim) {
+        if (cnt < P.nbr_cap) nbr[cnt * P.cap + i] = (int)j;
+        ++cnt;
+      }
+    }
+  }
+  return cnt;
+}
+
+// Cell coordinate c of the search box along axis K, wrapped on a periodic
+// axis the box does not cover whole.
+template <int K>
+__device__ inline int box_wrap(int c, const KParams& P, const bool* all) {
+  return (P.periodic[K] && !all[K]) ? (c % P.gn[K] + P.gn[K]) % P.gn[K] : c;
+}
+
+// The search box lo..hi of slot i, axis A0 innermost: consecutive cell
+// numbers along a row are merged into one slot range (two cell_start loads
+// per range instead of per cell).  The set of candidates is the same in any
+// walk order; the list is sorted by stable id afterwards.
+template <int A0, int A1, int A2>
+__device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t* cell_start, uint32_t ibase, int i,
+                                 dv3 xi, double ri, const int* lo, const int* hi, const bool* all, int* nbr) {
+  int cnt = 0;
+  for (int c2 = lo[A2]; c2 <= hi[A2]; ++c2) {
+    const uint32_t t2 = ibase + cell_term<A2>(box_wrap<A2>(c2, P, all), P);
+    for (int c1 = lo[A1]; c1 <= hi[A1]; ++c1) {
+      const uint32_t t21 = t2 + cell_term<A1>(box_wrap<A1>(c1, P, all), P);
+      uint32_t r0 = 0, r1 = 0;  // current run of cells [r0, r1)
+      for (int c0 = lo[A0]; c0 <= hi[A0]; ++c0) {
+        const uint32_t c = t21 + cell_term<A0>(box_wrap<A0>(c0, P, all), P);
+        if (r1 > r0 && c == r1) {
+          r1 = c + 1u;
+          continue;
+        }
+        if (r1 > r0) cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, nbr, cnt);
+        r0 = c;
+        r1 = c + 1u;
+      }
+      cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, nbr, cnt);
+    }
+  }
+  return cnt;
+}
+
 // Verlet list of slot i (from the re-ordered scratch buffers D, U): every j
 // with |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin; then copies slot
 // i back into the working buffers (W, V) and records the new slot count.
@@ -612,10 +723,12 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
   if (!ctl->rebuild[par]) return;
   const int n = (int)cell_start[P.ncells_all];  // alive particles after compaction
   const int cap = P.cap;
+  const bool keep_at = ctl->at_rest != 0;
   int local_max = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const dv3 xi = ldx(D.xr, i);
-    double ri = U.r[i];
+    dv3 xi;
+    double ri;
+    ldxr(D.xr, i, xi, ri);
     const uint32_t ibase = (uint32_t)U.inst[i] * P.ncells;  // candidates come from i's own instance only
     int ci[3] = {fine_coord(xi.x, 0, P, false), fine_coord(xi.y, 1, P, false), fine_coord(xi.z, 2, P, false)};
     double lim_max = P.margin_factor * (ri + P.r_max) + P.skin;
@@ -635,42 +748,11 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
       }
     }
     int cnt = 0;
-    for (int cz = lo[2]; cz <= hi[2]; ++cz) {
-      int wz = cz;
-      if (P.periodic[2] && !all[2]) wz = (cz % P.gn[2] + P.gn[2]) % P.gn[2];
-      for (int cy = lo[1]; cy <= hi[1]; ++cy) {
-        int wy = cy;
-        if (P.periodic[1] && !all[1]) wy = (cy % P.gn[1] + P.gn[1]) % P.gn[1];
-        for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-          int wx = cx;
-          if (P.periodic[0] && !all[0]) wx = (cx % P.gn[0] + P.gn[0]) % P.gn[0];
-          uint32_t c = ibase + cell_linear(P, wx, wy, wz);
-          uint32_t s0 = cell_start[c], s1 = cell_start[c + 1];
-          for (uint32_t j = s0; j < s1; ++j) {
-            if ((int)j == i) continue;
-            const dv3 xj = ldx(D.xr, (int)j);
-            dv3 d = sub(xi, xj);
-            if (P.periodic[0]) {
-              if (d.x > P.bsize[0] * 0.5) d.x -= P.bsize[0];
-              else if (d.x < -P.bsize[0] * 0.5) d.x += P.bsize[0];
-            }
-            if (P.periodic[1]) {
-              if (d.y > P.bsize[1] * 0.5) d.y -= P.bsize[1];
-              else if (d.y < -P.bsize[1] * 0.5) d.y += P.bsize[1];
-            }
-            if (P.periodic[2]) {
-              if (d.z > P.bsize[2] * 0.5) d.z -= P.bsize[2];
-              else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
-            }
-            double lim = P.margin_factor * (ri + U.r[j]) + P.skin;
-            if (dot(d, d) < lim * lim) {
-              if (cnt < P.nbr_cap) nbr[cnt * cap + i] = (int)j;
-              ++cnt;
-            }
-          }
-        }
-      }
-    }
+    // the grid axis whose neighbouring cells have consecutive numbers walks
+    // innermost, so each row of the search box is one or a few slot ranges
+    if (P.gstride[0] == 1) cnt = build_walk<0, 1, 2>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
+    else if (P.gstride[1] == 1) cnt = build_walk<1, 0, 2>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
+    else cnt = build_walk<2, 0, 1>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
     const int kept = min(cnt, P.nbr_cap);
     // Ascending stable id = ascending API index: the reference's summation
     // order inside one reference cell (grid.rs:42-46 inserts in index order).
@@ -698,8 +780,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
       W.rx[o] = D.rx[o];
       W.rv[o] = D.rv[o];
       W.rw[o] = D.rw[o];
-      W.a[o] = D.a[o];
-      W.tau[o] = D.tau[o];
+      if (keep_at) {  // live only in the at-rest state (see k_rank_permute)
+        W.a[o] = D.a[o];
+        W.tau[o] = D.tau[o];
+      }
     }
     reinterpret_cast<double4*>(W.xr)[i] = reinterpret_cast<const double4*>(D.xr)[i];
     V.r[i] = U.r[i];
