@@ -40,7 +40,12 @@ rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m turning at
 Coulomb (mu 0.5), E=5e7, nu=0.3, e=0.5, dt=2e-6 s, 1,000 steps per timed
 region; a jittered-lattice bed filling 30 % of the drum.  Forms: shortcut,
 then surrogate (the value).  CPU baseline sample: the bed's first 20,000
-spheres in the same drum.
+spheres in the same drum.  --workload drum-fix: the capacity run as
+specified, JKR + Mindlin with tangential history, in the physics-fix mode
+(real shear modulus, history table carried through re-sorts), with the bed
+in contact (scenes.drum_chain_workload: chains of touching spheres along the
+axis filling 30 % of a 0.99 m drum); its roofline counts 64 B per stored
+history entry on top of the 312 B per particle.
 
 Timing: --steps steps per timed region, --repeats regions (default 3), each
 on a fresh engine from the same initial state after the same warm-up (so
@@ -80,9 +85,10 @@ HBM_PEAK_GBS = 8000.0
 # (6 x 24 B), radius and mass (16 B), id/flags (8 B) = 168 B; write x, v, w and
 # residuals = 144 B.  Neighbour reads are reuse (cache), not counted.
 FORCE_BYTES_PER_PARTICLE = 312.0
-# Per directed contact carrying tangential history (Mindlin/LSC with a
-# non-zero shear modulus): 32 B read + 32 B write.  The reference's shear
-# modulus is 0 (material.rs:49), so no history is carried and this term is 0.
+# Per directed contact carrying tangential history (the physics-fix mode's
+# Mindlin/LSC with a real shear modulus): partner id and displacement, 32 B
+# read + 32 B write.  In the parity mode the reference's shear modulus is 0
+# (material.rs:49), no history is carried and this term is 0.
 HIST_BYTES_PER_CONTACT = 64.0
 
 
@@ -111,6 +117,12 @@ def build_workload(name: str, n: int | None):
     if name == "drum":
         p, x, r, m, meshes = scenes.drum_workload(n or 16_000_000)
         return p, x, r, m, None, meshes, "JKR+Coulomb drum turning at 30 rpm"
+    if name == "drum-fix":
+        from rusty_particles_amd import abi
+        p, x, r, m, meshes = scenes.drum_chain_workload(n or 16_000_000)
+        p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
+        p.mode = abi.RP_MODE_PHYSICS_FIX
+        return p, x, r, m, None, meshes, "JKR+Mindlin with tangential history (physics-fix mode) drum turning at 30 rpm"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -363,7 +375,7 @@ def run_batch(args) -> int:
 def mesh_motions(workload: str):
     """(mesh, origin, omega) of the workload's moving meshes."""
     from rusty_particles_amd import scenes
-    return [(0, scenes.DRUM_ORIGIN, scenes.DRUM_OMEGA)] if workload == "drum" else []
+    return [(0, scenes.DRUM_ORIGIN, scenes.DRUM_OMEGA)] if workload in ("drum", "drum-fix") else []
 
 
 def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0,
@@ -480,12 +492,15 @@ def run_single(args) -> dict:
         eng.set_timing(False)
         f_ms, f_cnt = tm["force"]
         avg_s = f_ms / max(f_cnt, 1) * 1e-3
-        achieved = FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9
+        # history tables (physics-fix mode): each stored entry read and written once
+        alg_bytes = FORCE_BYTES_PER_PARTICLE * n + HIST_BYTES_PER_CONTACT * st.get("history_contacts", 0)
+        achieved = alg_bytes / avg_s / 1e9
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
                "achieved_gbs": achieved, "contacts_per_step": st["contacts"],
                "contacts_per_s": st["contacts"] * args.steps / wall,
                "repeat_values": [n * args.steps / w for w in walls],
-               "jkr_failures_per_step": st["jkr_failures"], "wall": wall, "stats": st,
+               "jkr_failures_per_step": st["jkr_failures"], "history_contacts_per_step": st.get("history_contacts", 0),
+               "wall": wall, "stats": st,
                "autotuned_skin_m": tuned[0] if tuned else None, "autotuned_cell_m": tuned[1] if tuned else None,
                "tiling": eng.tiling()}
         res.update(surrogate_kernels(tm))
@@ -530,7 +545,7 @@ def run_single(args) -> dict:
         cell = 2.0 * float(r.max()) * (1.5 if p.normal_model in (abi.RP_NORMAL_JKR, abi.RP_NORMAL_SJKR) else 1.0)
         # the drum's 16M spheres x 16,384 triangles (brute-force wall loop) take minutes per CPU step:
         # its sample is the bed's first 20,000 sites (bottom layers) in the same drum
-        ns = n if args.workload != "drum" else min(n, 20_000)
+        ns = n if args.workload not in ("drum", "drum-fix") else min(n, 20_000)
         what = f"the full {n}-particle workload" if ns == n else f"the first {ns} of the {n} particles"
         log(f"cpu baseline: oracle, {threads} threads ({info['host_cpus']} host CPUs, {info['cpu_model']}), "
             f"~{args.cpu_seconds:g}s of {what}")
@@ -596,6 +611,7 @@ def run_single(args) -> dict:
         "wall_contacts_per_step": st["wall_contacts"],
         "force_kernel_avg_us": avg_s * 1e6,
         "jkr_failures_per_step": st["jkr_failures"],
+        "history_contacts_per_step": R["history_contacts_per_step"],
         "autotuned_skin_m": R["autotuned_skin_m"], "autotuned_cell_m": R["autotuned_cell_m"], "tiling": R["tiling"],
     }
     if len(results) > 1 or main_form != "plain":
@@ -644,11 +660,11 @@ def main() -> int:
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.autotune is None:
-        args.autotune = 0 if args.workload in ("hopper", "box", "drum") else 20
+        args.autotune = 0 if args.workload in ("hopper", "box", "drum", "drum-fix") else 20
     if args.steps is None:
-        args.steps = 1000 if args.workload == "drum" else 5000
+        args.steps = 1000 if args.workload in ("drum", "drum-fix") else 5000
     if args.jkr_forms is None:
-        args.jkr_forms = "shortcut,surrogate" if args.workload == "drum" else "exact,shortcut,surrogate"
+        args.jkr_forms = "shortcut,surrogate" if args.workload in ("drum", "drum-fix") else "exact,shortcut,surrogate"
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one domain on one GPU; multi-GPU runs are not "
                                    "supported (use --gpus 1)"}))

@@ -537,6 +537,81 @@ def jkr_ensemble_workload(ninst: int = 64, seed: int = 0xD3E50006, nmin: int = 5
     return p, [int(c) for c in counts], mats, x, r, m, v
 
 
+def drum_chain_workload(n: int = 16_000_000, seed: int = 0xD3E50025, radius: float = 0.25, fill: float = 0.30,
+                        press: float = 2e-6):
+    """The capacity run with its bed in contact, for the physics-fix mode's
+    Mindlin history: n spheres (r ~ U[0.4, 0.8] mm, rho 2500, JKR gamma 0.01,
+    E=5e7, nu=0.3, e=0.5, mu=0.5, dt=2e-6 s) in the 16,384-triangle drum of
+    radius `radius` turning at 30 rpm (DRUM_OMEGA, mesh 0), arranged as in
+    drum_test_workload: chains along the drum's axis, one radius per chain,
+    neighbours touching with overlaps U[0.05, 3] um (JKR contacts with a
+    root), chains 2.2 r_max apart, the outer ones pressed `press` into the
+    plane of the side facet they face (2 um: a Hertz wall contact that
+    accelerates a 0.4 mm sphere by ~7e3 m/s^2; 15 um would launch it off the
+    wall at ~1 m/s).  The chains fill the lowest `fill` of the drum's cross-section; the
+    drum's length is set so that they hold n spheres.  Returns (params, x, r,
+    m, meshes)."""
+    rng = np.random.default_rng(seed)
+    rmin, rmax = 0.4e-3, 0.8e-3
+    pitch = 2.2 * rmax
+    g = np.arange(-radius, radius + 1e-12, pitch)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    cx, cy = X.ravel(), Y.ravel()
+    order = np.lexsort((cx, cy))  # lowest chains first
+    cx, cy = cx[order], cy[order]
+    rc = rng.uniform(rmin, rmax, cx.size)
+    rho = np.hypot(cx, cy)
+    keep = rho <= radius - rc + press
+    cx, cy, rc, rho = cx[keep], cy[keep], rc[keep], rho[keep]
+    # the chains below the height that bounds `fill` of the disc's area
+    seg = np.cumsum(np.full(cx.size, pitch * pitch))
+    nch = max(1, int(np.searchsorted(seg, fill * math.pi * radius * radius)))
+    cx, cy, rc, rho = cx[:nch], cy[:nch], rc[:nch], rho[:nch]
+    # outer chains pressed `press` into the plane of the drum_mesh facet
+    # they face (facet k spans the angles 2 pi k / 128 .. 2 pi (k + 1) / 128;
+    # its plane lies R cos(pi / 128) from the axis, up to 75 um inside the
+    # circle at R = 0.25 m)
+    nseg = 128
+    th = np.mod(np.arctan2(cy, cx), 2.0 * math.pi)
+    phi = (np.floor(th / (2.0 * math.pi / nseg)) + 0.5) * (2.0 * math.pi / nseg)
+    to_plane = (radius * math.cos(math.pi / nseg) - rc + press) / np.cos(th - phi)
+    side = (rho > 1e-9) & (rho > radius - rc - 0.5 * pitch)
+    sc = np.where(side, to_plane / np.maximum(rho, 1e-300), 1.0)
+    px, py = cx * sc, cy * sc
+    # length: chains of 2 r_c - 1.5 um per sphere hold n spheres
+    length = float(n / np.sum(1.0 / (2.0 * rc - 1.525e-6))) + 4e-4
+    for _ in range(20):
+        per = np.floor((length - 2e-4 - 2.0 * rc) / (2.0 * rc - 0.05e-6)).astype(np.int64) + 1
+        per = np.maximum(per, 0)
+        if per.sum() >= n:
+            break
+        length *= 1.01
+    # positions: z_0 = -L/2 + r_c + 1e-4, then steps of 2 r_c - U[0.05, 3] um
+    # while the sphere stays 1e-4 m inside the end caps
+    cid = np.repeat(np.arange(nch), per)
+    step = 2.0 * rc[cid] - rng.uniform(0.05e-6, 3e-6, cid.size)
+    start = np.concatenate([[0], np.cumsum(per)[:-1]])
+    cs = np.cumsum(step)
+    z0 = -0.5 * length + rc[cid] + 1e-4
+    z = z0 + cs - step - np.repeat(cs[start] - step[start], per)
+    ok = z + rc[cid] < 0.5 * length - 1e-4
+    sel = np.nonzero(ok)[0][:n]
+    if sel.size < n:
+        raise RuntimeError(f"the drum holds only {sel.size} of {n} chained spheres")
+    cid = cid[sel]
+    x = np.stack([px[cid], py[cid], z[sel]], 1)
+    r = rc[cid]
+    m = sphere_mass(r, 2500.0)
+    v9, n3 = drum_mesh(radius, length)
+    pad = 0.005
+    p = abi.default_params(2e-6, (-radius - pad, -radius - pad, -0.5 * length - pad),
+                           (radius + pad, radius + pad, 0.5 * length + pad))
+    p.normal_model = abi.RP_NORMAL_JKR
+    p.tangential_model = abi.RP_TANGENTIAL_COULOMB
+    p.particle = abi.RpMaterial(5e7, 0.3, 2500.0, 0.5, 0.5, 0.01)
+    return p, x, r, m, [(v9, n3)]
+
+
 def drum_test_workload(n: int = 20_000, seed: int = 0xD3E50015, radius: float = 0.03, length: float = 0.03,
                        press: float = 15e-6):
     """The capacity run's spheres, materials and drum (drum_workload: r ~

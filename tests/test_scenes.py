@@ -115,3 +115,22 @@ def test_hopper_cone_geometry():
     # azimuth, within half a segment's angle
     assert np.all((nn * want).sum(1) > math.cos(math.pi / 64) - 1e-6)
     assert np.allclose(nn[:, 1], sa, atol=1e-3)
+
+
+def test_drum_chain_bed():
+    """The chained bed of the physics-fix capacity run: n spheres, every one
+    inside the drum (centre within R - r + 15 um of the axis, 1e-4 m inside the
+    end caps), consecutive spheres of a chain overlapping by 0.05 ... 3 um, the
+    bed in the lowest 30 % of the cross-section."""
+    p, x, r, m, meshes = scenes.drum_chain_workload(50_000, radius=0.05)
+    assert x.shape == (50_000, 3) and r.min() >= 0.4e-3 and r.max() <= 0.8e-3
+    half = 0.5 * (p.bounds_max[2] - p.bounds_min[2]) - 0.005
+    assert np.all(np.hypot(x[:, 0], x[:, 1]) <= 0.05 - r + 2e-6 + 1e-12)
+    assert np.all(np.abs(x[:, 2]) + r <= half - 1e-4 + 1e-12)
+    same = (r[1:] == r[:-1]) & (x[1:, 0] == x[:-1, 0]) & (x[1:, 1] == x[:-1, 1])
+    ov = (r[1:] + r[:-1] - np.linalg.norm(np.diff(x, axis=0), axis=1))[same]
+    assert same.sum() > 40_000 and ov.min() >= 0.05e-6 - 1e-12 and ov.max() <= 3e-6 + 1e-12
+    # the chains' cross-section: about 30 % of the disc, the lowest part
+    area = np.unique(np.round(x[:, :2] * 1e7).astype(np.int64), axis=0).shape[0] * (2.2 * 0.8e-3) ** 2
+    assert 0.2 < area / (math.pi * 0.05 ** 2) < 0.35
+    assert x[:, 1].max() < 0.0
