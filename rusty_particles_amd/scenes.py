@@ -294,38 +294,124 @@ def hopper_test_workload(n: int = 20_000, seed: int = 0xD3E50012, clear: float =
     return p, x, r, m, [(hv9, hn3)]
 
 
-def hopper_bed_workload(n: int = 1_000_000, seed: int = 0xD3E50022, inflate: float = 1.08):
-    """The hopper configuration with its spheres packed in the cone instead of
-    poured above it -- a contact-bearing window of the same workload: n
-    spheres (r ~ U[0.5, 1.5] mm, rho 2500, Hertz + Mindlin, E=5e7, nu=0.25,
-    e=0.6, mu=0.4, dt=2e-6) placed by random sequential addition in a cone of
-    the same half-angle (30 deg), outlet (8 mm) and 4,096 triangles whose top
-    radius is 0.25 m (the 75 mm cone of hopper_workload holds ~36k such
-    spheres), centres 0.97 r from the wall, radii then grown by 8 % so that
-    near neighbours and the wall are in contact (about one contact per sphere;
-    overlaps below 8 % of a radius sum).  Returns (params, x, r, m, meshes)."""
-    rmin, rmax = 0.5e-3 / inflate, 1.5e-3 / inflate
-    top = 0.25
-    hv9, hn3 = hopper_mesh(HOPPER_OUTLET, HOPPER_HALF_ANGLE, top, y0=0.0, nseg=64, nring=32)
+def _hertz_kn(e_star: float, r_star):
+    """Hertz stiffness 4/3 E* sqrt(R*) (F = kn delta^1.5, physics.rs:86-129)."""
+    return 4.0 / 3.0 * e_star * np.sqrt(r_star)
+
+
+def _e_star(e1: float, nu1: float, e2: float, nu2: float) -> float:
+    """Effective modulus 1 / ((1 - nu1^2)/E1 + (1 - nu2^2)/E2) (material.rs:25-37)."""
+    return 1.0 / ((1.0 - nu1 * nu1) / e1 + (1.0 - nu2 * nu2) / e2)
+
+
+def hopper_bed_workload(n: int = 1_000_000, seed: int = 0xD3E50022, top: float = 0.3):
+    """The hopper configuration's n spheres (r ~ U[0.5, 1.5] mm, rho 2500,
+    Hertz + Mindlin, E=5e7, nu=0.25, e=0.6, mu=0.4, dt=2e-6 s, gravity -Y)
+    resting in a cone of the same half-angle (30 deg), outlet (8 mm) and
+    triangle count (4,096) whose top radius is `top` (the 75 mm cone of
+    hopper_workload holds ~36k such spheres): a contact-bearing window of the
+    same workload.
+
+    The bed is a set of vertical columns on a square grid of pitch 2.2 r_max
+    (one radius per column), each standing on the cone facet below it: the
+    lowest sphere touches the facet's plane (the plane through the apex and
+    the facet's two generators), pressed into it by the Hertz overlap that
+    carries twice the column's weight (the facet's normal has a vertical
+    component of 1/2), and each sphere above rests on the one below with the
+    static Hertz overlap of the weight above it.  Columns whose lowest contact
+    would lie in the outlet are left out.  The columns fill the cone from the
+    bottom up to a common height that gives n spheres.  Every sphere but the
+    top of its column has two contacts and every column one wall contact; the
+    walls' normal force is not vertical, so the columns' feet slide towards
+    the axis and the columns lean from the start (a slowly moving bed, no
+    violent relaxation).  Returns (params, x, r, m, meshes)."""
+    rng = np.random.default_rng(seed)
+    rmin, rmax = 0.5e-3, 1.5e-3
+    e_mod, nu, rho, grav = 5e7, 0.25, 2500.0, 9.81
+    wall = abi.default_params(1.0, (0, 0, 0), (1, 1, 1)).wall
+    nseg = 64
+    hv9, hn3 = hopper_mesh(HOPPER_OUTLET, HOPPER_HALF_ANGLE, top, y0=0.0, nseg=nseg, nring=32)
     y_top = float(hv9[:, 1::3].max())
-    t = math.tan(math.radians(HOPPER_HALF_ANGLE))
-    mean_vol = 4.0 / 3.0 * np.pi * (rmax ** 4 - rmin ** 4) / (4 * (rmax - rmin))
-    need = n * mean_vol / 0.24  # bed height for RSA at solid fraction ~0.24
-    h = 0.01
-    while np.pi * h / 3.0 * (HOPPER_OUTLET ** 2 + HOPPER_OUTLET * (HOPPER_OUTLET + h * t)
-                             + (HOPPER_OUTLET + h * t) ** 2) < need:
-        h += 0.002
-    if h > y_top:
-        raise RuntimeError(f"a bed of {n} spheres does not fit the cone")
-    _lib().rps_set_progress(1 if n >= 500_000 else 0)
-    x, r = rsa_cone(n, rmin, rmax, 0.0, h, HOPPER_OUTLET, HOPPER_HALF_ANGLE, 0.97, seed)
-    _lib().rps_set_progress(0)
-    r = r * inflate
-    m = sphere_mass(r, 2500.0)
+    half = math.radians(HOPPER_HALF_ANGLE)
+    apex_y = -HOPPER_OUTLET / math.tan(half)
+    pitch = 2.2 * rmax
+    g = np.arange(-top, top + 1e-12, pitch) + 0.5 * pitch
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    cx, cz = X.ravel(), Z.ravel()
+    rc = rng.uniform(rmin, rmax, cx.size)
+    mc = sphere_mass(rc, rho)
+    # inward unit normals of the facet planes of segments j - 1, j, j + 1
+    th = np.mod(np.arctan2(cz, cx), 2.0 * math.pi)
+    seg = np.floor(th / (2.0 * math.pi / nseg)).astype(np.int64)
+
+    def plane(jj):
+        t0, t1 = 2.0 * math.pi * jj / nseg, 2.0 * math.pi * (jj + 1) / nseg
+        s, c = math.sin(half), math.cos(half)
+        g0 = np.stack([s * np.cos(t0), np.full(t0.shape, c), s * np.sin(t0)], 1)
+        g1 = np.stack([s * np.cos(t1), np.full(t1.shape, c), s * np.sin(t1)], 1)
+        nn = -np.cross(g0, g1)
+        return nn / np.linalg.norm(nn, axis=1, keepdims=True)
+
+    e_pp = _e_star(e_mod, nu, e_mod, nu)
+    e_pw = _e_star(e_mod, nu, wall.youngs_modulus, wall.poissons_ratio)
+    kn_pp = _hertz_kn(e_pp, 0.5 * rc)
+    kn_pw = _hertz_kn(e_pw, rc)
+
+    def base(count):
+        """Lowest sphere's centre height and its foot's height on the cone."""
+        w = np.maximum(count, 1) * mc * grav
+        press = (2.0 * w / kn_pw) ** (2.0 / 3.0)
+        yb = np.full(cx.size, -np.inf)
+        foot = np.zeros(cx.size)
+        for dj in (-1, 0, 1):
+            nn = plane(seg + dj)
+            y = apex_y + (rc - press - nn[:, 0] * cx - nn[:, 2] * cz) / nn[:, 1]
+            better = y > yb
+            yb = np.where(better, y, yb)
+            foot = np.where(better, y - (rc - press) * nn[:, 1], foot)
+        return yb, foot
+
+    yb, foot = base(np.ones(cx.size))
+    ok = (foot > 0.0) & (np.hypot(cx, cz) + rc < top)
+    cx, cz, rc, mc, seg, kn_pp, kn_pw = cx[ok], cz[ok], rc[ok], mc[ok], seg[ok], kn_pp[ok], kn_pw[ok]
+    yb = yb[ok]
+
+    def counts(h):
+        return np.maximum(np.floor((h - rc - yb) / (2.0 * rc)).astype(np.int64) + 1, 0)
+
+    lo, hi = 0.0, y_top - 2.0 * rmax
+    if counts(hi).sum() < n:
+        raise RuntimeError(f"a bed of {n} spheres does not fit a cone of top radius {top}")
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if counts(mid).sum() < n else (lo, mid)
+    per = counts(hi)
+    # drop the highest spheres beyond n (column tops, highest first)
+    extra = int(per.sum()) - n
+    if extra > 0:
+        tops = yb + 2.0 * rc * (per - 1)
+        for q in np.argsort(-tops)[:extra]:
+            per[q] -= 1
+    keep = per > 0
+    cx, cz, rc, mc, seg, kn_pp, kn_pw, per = (cx[keep], cz[keep], rc[keep], mc[keep], seg[keep], kn_pp[keep],
+                                              kn_pw[keep], per[keep])
+    yb, _ = base(per)
+    cid = np.repeat(np.arange(per.size), per)
+    start = np.concatenate([[0], np.cumsum(per)[:-1]])
+    k = np.arange(cid.size) - start[cid]                 # 0 = lowest sphere of its column
+    above = per[cid] - 1 - k                             # spheres resting on sphere k
+    # contact k (between sphere k and k + 1) carries the weight of the spheres above sphere k
+    delta = (np.maximum(above, 0) * mc[cid] * grav / kn_pp[cid]) ** (2.0 / 3.0)
+    step = 2.0 * rc[cid] - delta
+    cs = np.cumsum(step)
+    y = yb[cid] + (cs - step) - np.repeat((cs - step)[start], per)
+    x = np.stack([cx[cid], y, cz[cid]], 1)
+    r = rc[cid]
+    m = mc[cid]
     p = abi.default_params(2e-6, (-top - 0.01, -0.05, -top - 0.01), (top + 0.01, y_top + 0.01, top + 0.01))
     p.normal_model = abi.RP_NORMAL_HERTZ
     p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
-    p.particle = abi.RpMaterial(5e7, 0.25, 2500.0, 0.4, 0.6, 0.0)
+    p.particle = abi.RpMaterial(e_mod, nu, rho, 0.4, 0.6, 0.0)
     return p, x, r, m, [(hv9, hn3)]
 
 

@@ -12,7 +12,11 @@ Checks fixed by geometry, not by the generator's own formulas:
   outlet and top rims, the half-angle between a generator line and the axis,
   interior edges shared by two triangles and rim edges by one, unit normals
   perpendicular to their triangle and pointing into the hopper
-  (n = (-cos a cos t, sin a, -cos a sin t) for half-angle a at azimuth t).
+  (n = (-cos a cos t, sin a, -cos a sin t) for half-angle a at azimuth t);
+* the contact-bearing hopper bed: its columns in Hertz equilibrium with the
+  weight above each contact, the lowest sphere's overlap with the cone
+  (brute-force distance to the nearest triangle) carrying twice the column's
+  weight, the other spheres clear of the mesh, ~2 contacts per sphere.
 """
 import math
 
@@ -134,3 +138,90 @@ def test_drum_chain_bed():
     area = np.unique(np.round(x[:, :2] * 1e7).astype(np.int64), axis=0).shape[0] * (2.2 * 0.8e-3) ** 2
     assert 0.2 < area / (math.pi * 0.05 ** 2) < 0.35
     assert x[:, 1].max() < 0.0
+
+
+def _point_triangle_distance(p, v9):
+    """Distance from each point of p (k, 3) to the nearest triangle of v9
+    (t, 9): the nearest of the plane's foot (when inside the triangle) and
+    the three edges' nearest points, evaluated for every pair."""
+    a, b, c = (v9[:, 0:3].astype(np.float64), v9[:, 3:6].astype(np.float64), v9[:, 6:9].astype(np.float64))
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    out = np.empty(p.shape[0])
+    for s in range(0, p.shape[0], 64):
+        q = p[s:s + 64, None, :]
+        h = ((q - a) * nrm).sum(-1)
+        foot = q - h[..., None] * nrm
+        inside = np.ones(h.shape, bool)
+        for u, w in ((a, b), (b, c), (c, a)):
+            inside &= (np.cross(w - u, foot - u) * nrm).sum(-1) >= 0.0
+        best = np.where(inside, np.abs(h), np.inf)
+        for u, w in ((a, b), (b, c), (c, a)):
+            e = w - u
+            t = np.clip(((q - u) * e).sum(-1) / (e * e).sum(-1), 0.0, 1.0)
+            best = np.minimum(best, np.linalg.norm(q - (u + t[..., None] * e), axis=-1))
+        out[s:s + 64] = best.min(1)
+    return out
+
+
+def test_hopper_bed_columns_rest_on_the_cone():
+    """The contact-bearing hopper bed: vertical columns standing on the
+    4,096-triangle cone.  Checked against mechanics and geometry computed
+    here: each column's contacts carry exactly the weight above them under
+    Hertz's law (F = 4/3 E* sqrt(R*) delta^1.5 with E* of two equal
+    materials, R* = r/2), the lowest sphere's overlap with the mesh (distance
+    to the nearest triangle, brute force) carries twice the column's weight,
+    every other sphere clears the mesh, and one oracle step finds ~2 directed
+    contacts per sphere and no excessive overlap."""
+    from oracle import OracleSimulation
+    from rusty_particles_amd import abi
+
+    n = 30_000
+    p, x, r, m, meshes = scenes.hopper_bed_workload(n)
+    (v9, n3), = meshes
+    assert x.shape == (n, 3) and r.min() >= 0.5e-3 and r.max() <= 1.5e-3
+    assert v9.shape[0] == 4096
+    col = np.unique(np.round(x[:, [0, 2]] * 1e9).astype(np.int64), axis=0, return_inverse=True)[1].ravel()
+    order = np.lexsort((x[:, 1], col))
+    e_star = 5e7 / (2.0 * (1.0 - 0.25 ** 2))
+    g = 9.81
+    checked = 0
+    lowest = []
+    for c in np.unique(col):
+        idx = order[col[order] == c]
+        assert np.all(r[idx] == r[idx[0]])
+        lowest.append(idx[0])
+        if idx.size > 1:
+            d = np.diff(x[idx, 1])
+            delta = 2.0 * r[idx[0]] - d
+            f = 4.0 / 3.0 * e_star * np.sqrt(0.5 * r[idx[0]]) * delta ** 1.5
+            w = m[idx[0]] * g * np.arange(idx.size - 1, 0, -1)
+            # delta is a difference of heights ~0.1 m apart by ~1e-6: ~1e-9 of it is rounding
+            assert np.allclose(f, w, rtol=1e-6, atol=0.0)
+            checked += idx.size - 1
+    assert checked > 0.9 * n
+    lowest = np.array(lowest)
+    rng = np.random.default_rng(5)
+    pick = rng.choice(lowest, 300, replace=False)
+    dist = _point_triangle_distance(x[pick], v9)
+    press = r[pick] - dist
+    per = np.bincount(col)[col[pick]]
+    e_pw = 1.0 / ((1.0 - 0.25 ** 2) / 5e7 + (1.0 - 0.3 ** 2) / 1e9)
+    f_wall = 4.0 / 3.0 * e_pw * np.sqrt(r[pick]) * press ** 1.5
+    # float32 triangle vertices: the distance is exact to ~1e-8 m
+    expect = 2.0 * per * m[pick] * g
+    assert np.all(press > 0.0)
+    assert np.allclose(press, (expect / (4.0 / 3.0 * e_pw * np.sqrt(r[pick]))) ** (2.0 / 3.0), rtol=0, atol=2e-8)
+    assert np.all(f_wall > 0.0)
+    others = np.setdiff1d(np.arange(n), lowest)
+    far = rng.choice(others, 300, replace=False)
+    assert np.all(_point_triangle_distance(x[far], v9) > r[far])
+    o = OracleSimulation(p.dt, None, None, params=abi.RpParams.from_buffer_copy(p))
+    o.set_state(x, r, m)
+    o.add_mesh_arrays(v9, n3)
+    o.set_fast(True)
+    o.set_record(True)
+    o.step(1)
+    _, js = o.pairs()
+    assert 1.8 < js.size / n < 2.0
+    assert not o.overlap_flag()
