@@ -52,10 +52,11 @@ on a fresh engine from the same initial state after the same warm-up (so
 each times the same simulated window), value = the median region;
 contacts/s = the contacts of the last step times steps/s.
 
---workload hopper-bed: the hopper configuration's 1M spheres packed in a
-cone of the same half-angle, outlet and triangle count (top radius 0.25 m so
-the bed fits; random sequential addition, radii then grown 8 %): a
-contact-bearing window of the same workload.
+--workload hopper-bed: the hopper configuration's 1M spheres resting in a
+cone of the same half-angle, outlet and triangle count (top radius 0.3 m so
+the bed fits): vertical columns in static Hertz equilibrium, each standing on
+a fixed foot sphere that touches the cone (scenes.hopper_bed_workload), ~2
+contacts per sphere: a contact-bearing window of the same workload.
 
 Printed: one JSON line (rank 0) with value = particle-steps/s, the roofline
 of the dominant kernel (the fused contact + integration kernel, timed with
@@ -97,32 +98,33 @@ def log(msg: str) -> None:
 
 
 def build_workload(name: str, n: int | None):
-    """(params, x, r, m, v or None, meshes, description)."""
+    """(params, x, r, m, v or None, meshes, description, fixed flags or None)."""
     from rusty_particles_amd import scenes
     if name == "hopper":
         p, x, r, m, meshes = scenes.hopper_workload(n or 1_000_000)
-        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper"
+        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper", None
     if name == "hopper-bed":
-        p, x, r, m, meshes = scenes.hopper_bed_workload(n or 1_000_000)
-        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper, bed packed in the cone"
+        p, x, r, m, meshes, fixed = scenes.hopper_bed_workload(n or 1_000_000)
+        return p, x, r, m, None, meshes, "Hertz+Mindlin hopper, bed of columns standing in the cone", fixed
     if name == "box":
         p, x, r, m, meshes = scenes.box_settle_workload(n or 10_000)
-        return p, x, r, m, None, meshes, "Hertz+Coulomb box"
+        return p, x, r, m, None, meshes, "Hertz+Coulomb box", None
     if name == "jkr":
         p, x, r, m, v = scenes.jkr_periodic_workload(n or 2_000_000)
-        return p, x, r, m, v, [], "JKR+Coulomb periodic packing"
+        return p, x, r, m, v, [], "JKR+Coulomb periodic packing", None
     if name == "jkr-dense":
         p, x, r, m, v = scenes.jkr_dense_workload(n or 2_000_000)
-        return p, x, r, m, v, [], "JKR+Coulomb compressed FCC crystal"
+        return p, x, r, m, v, [], "JKR+Coulomb compressed FCC crystal", None
     if name == "drum":
         p, x, r, m, meshes = scenes.drum_workload(n or 16_000_000)
-        return p, x, r, m, None, meshes, "JKR+Coulomb drum turning at 30 rpm"
+        return p, x, r, m, None, meshes, "JKR+Coulomb drum turning at 30 rpm", None
     if name == "drum-fix":
         from rusty_particles_amd import abi
         p, x, r, m, meshes = scenes.drum_chain_workload(n or 16_000_000)
         p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
         p.mode = abi.RP_MODE_PHYSICS_FIX
-        return p, x, r, m, None, meshes, "JKR+Mindlin with tangential history (physics-fix mode) drum turning at 30 rpm"
+        return (p, x, r, m, None, meshes, "JKR+Mindlin with tangential history (physics-fix mode) drum turning at 30 rpm",
+                None)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -379,7 +381,7 @@ def mesh_motions(workload: str):
 
 
 def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, seconds: float = 0.0,
-                 motions=(), bits: int = 64):
+                 motions=(), bits: int = 64, fixed=None):
     """The oracle (port of step_cpu) on the same inputs, reference grid cell =
     largest contact cutoff (the cell-size heuristic of simulation.rs:64).
     Times at least `steps` steps, more when the warm-up step says that
@@ -392,7 +394,7 @@ def cpu_baseline(p, x, r, m, v, meshes, steps: int, threads: int, cell: float, s
     lib = oracle.lib()
     lib.rpo_set_threads(threads)
     o = oracle.OracleSimulation(p.dt, None, None, bits=bits, params=abi.RpParams.from_buffer_copy(p))
-    o.set_state(x, r, m, v=v)
+    o.set_state(x, r, m, fixed, v=v)
     for v9, n3 in meshes:
         o.add_mesh_arrays(v9, n3)
     for k, org, om in motions:
@@ -417,7 +419,7 @@ def run_single(args) -> dict:
 
     from rusty_particles_amd import abi, native
     t0 = time.perf_counter()
-    p, x, r, m, v, meshes, desc = build_workload(args.workload, args.n)
+    p, x, r, m, v, meshes, desc, fixed = build_workload(args.workload, args.n)
     n = x.shape[0]
     ntri = sum(v9.shape[0] for v9, _ in meshes)
     log(f"workload {args.workload}: {n} particles, {ntri} triangles, built in {time.perf_counter() - t0:.1f}s")
@@ -444,7 +446,7 @@ def run_single(args) -> dict:
         e = native.Engine(abi.RpParams.from_buffer_copy(p))
         if args.no_graph:
             e.set_graph(False)
-        e.upload(x, r, m, v=v)
+        e.upload(x, r, m, fixed, v=v)
         for v9, n3 in meshes:
             e.add_mesh(v9, n3)
         for k, org, om in mesh_motions(args.workload):
@@ -549,14 +551,15 @@ def run_single(args) -> dict:
         what = f"the full {n}-particle workload" if ns == n else f"the first {ns} of the {n} particles"
         log(f"cpu baseline: oracle, {threads} threads ({info['host_cpus']} host CPUs, {info['cpu_model']}), "
             f"~{args.cpu_seconds:g}s of {what}")
+        fx = None if fixed is None else fixed[:ns]
         cval, csec, csteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
                                           args.cpu_steps, threads, cell, args.cpu_seconds,
-                                          mesh_motions(args.workload))
+                                          mesh_motions(args.workload), fixed=fx)
         log(f"cpu baseline f64 {cval:.4g} particle-steps/s ({csec:.1f}s)")
         # the reference's own precision (f32), the same sample
         fval, fsec, fsteps = cpu_baseline(p, x[:ns], r[:ns], m[:ns], None if v is None else v[:ns], meshes,
                                           args.cpu_steps, threads, cell, 0.5 * args.cpu_seconds,
-                                          mesh_motions(args.workload), bits=32)
+                                          mesh_motions(args.workload), bits=32, fixed=fx)
         log(f"cpu baseline f32 {fval:.4g} particle-steps/s ({fsec:.1f}s)")
         # as shipped: the reference's 1 m grid cell (simulation.rs:66), every
         # walk appending all particles of up to 27 cells -- O(N^2) per step,
@@ -564,7 +567,7 @@ def run_single(args) -> dict:
         import oracle
         oracle.lib().rpo_set_threads(threads)
         o = oracle.OracleSimulation(p.dt, None, None, params=abi.RpParams.from_buffer_copy(p))
-        o.set_state(x[:ns], r[:ns], m[:ns], v=None if v is None else v[:ns])
+        o.set_state(x[:ns], r[:ns], m[:ns], fx, v=None if v is None else v[:ns])
         nsamp = 2000 if ns <= 100_000 else 64
         gsec = o.time_gather(nsamp, 1.0)
         del o

@@ -318,13 +318,15 @@ def hopper_bed_workload(n: int = 1_000_000, seed: int = 0xD3E50022, top: float =
     the facet's two generators), pressed into it by the Hertz overlap that
     carries twice the column's weight (the facet's normal has a vertical
     component of 1/2), and each sphere above rests on the one below with the
-    static Hertz overlap of the weight above it.  Columns whose lowest contact
-    would lie in the outlet are left out.  The columns fill the cone from the
-    bottom up to a common height that gives n spheres.  Every sphere but the
-    top of its column has two contacts and every column one wall contact; the
-    walls' normal force is not vertical, so the columns' feet slide towards
-    the axis and the columns lean from the start (a slowly moving bed, no
-    violent relaxation).  Returns (params, x, r, m, meshes)."""
+    static Hertz overlap of the weight above it.  The lowest sphere of every
+    column is fixed (the reference's `fixed` flag, particle.rs:10): on the
+    slanted facet a free one slides from under its column, which then falls
+    apart (frictionless spheres in the parity mode), while on a fixed foot the
+    column stays in equilibrium.  Columns whose lowest contact would lie in
+    the outlet are left out.  The columns fill the cone from the bottom up to
+    a common height that gives n spheres.  Every sphere but the top of its
+    column has two contacts, and every foot touches the cone.  Returns
+    (params, x, r, m, meshes, fixed)."""
     rng = np.random.default_rng(seed)
     rmin, rmax = 0.5e-3, 1.5e-3
     e_mod, nu, rho, grav = 5e7, 0.25, 2500.0, 9.81
@@ -408,11 +410,12 @@ def hopper_bed_workload(n: int = 1_000_000, seed: int = 0xD3E50022, top: float =
     x = np.stack([cx[cid], y, cz[cid]], 1)
     r = rc[cid]
     m = mc[cid]
+    fixed = (k == 0).astype(np.uint8)
     p = abi.default_params(2e-6, (-top - 0.01, -0.05, -top - 0.01), (top + 0.01, y_top + 0.01, top + 0.01))
     p.normal_model = abi.RP_NORMAL_HERTZ
     p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
     p.particle = abi.RpMaterial(e_mod, nu, rho, 0.4, 0.6, 0.0)
-    return p, x, r, m, [(hv9, hn3)]
+    return p, x, r, m, [(hv9, hn3)], fixed
 
 
 def box_settle_workload(n: int = 10_000, seed: int = 0xD3E50001):

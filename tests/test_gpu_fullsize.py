@@ -1,9 +1,9 @@
 """The benchmark configurations at their full sizes on the GPU, with checks
 that do not need the oracle to follow every step (simulation.rs:173-562).
 
-* 1M spheres of the hopper configuration packed in its 4,096-triangle cone
-  (pair and wall contacts from the first step; the domain lies in one
-  reference cell): the first step against the oracle's fast enumeration,
+* 1M spheres of the hopper configuration standing in columns in its
+  4,096-triangle cone (~2M pair and ~14k wall contacts from the first step;
+  the domain lies in one reference cell): the first step against the oracle's fast enumeration,
   pair lists and state bit for bit, cell ids and sort permutation included;
   then 30 more steps: count conserved, finite state, wall contacts persist.
 * 2M spheres in the periodic JKR crystal (pencil-numbered cells at this
@@ -35,9 +35,9 @@ pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 FIELDS = ("x", "v", "w", "a", "tau", "rx", "rv", "rw")
 
 
-def _oracle(p, x, r, m, v=None, meshes=()):
+def _oracle(p, x, r, m, v=None, meshes=(), fixed=None):
     o = OracleSimulation(p.dt, None, None, params=copy_params(p))
-    o.set_state(x, r, m, v=v)
+    o.set_state(x, r, m, fixed, v=v)
     for v9, n3 in meshes:
         o.add_mesh_arrays(v9, n3)
     o.set_fast(True)
@@ -45,9 +45,9 @@ def _oracle(p, x, r, m, v=None, meshes=()):
     return o
 
 
-def _engine(p, x, r, m, v=None, meshes=(), solver=None):
+def _engine(p, x, r, m, v=None, meshes=(), solver=None, fixed=None):
     e = native.Engine(copy_params(p))
-    e.upload(x, r, m, v=v)
+    e.upload(x, r, m, fixed, v=v)
     for v9, n3 in meshes:
         e.add_mesh(v9, n3)
     if solver:
@@ -55,12 +55,12 @@ def _engine(p, x, r, m, v=None, meshes=(), solver=None):
     return e
 
 
-def _first_step_equal(p, x, r, m, v=None, meshes=()):
+def _first_step_equal(p, x, r, m, v=None, meshes=(), fixed=None):
     """One recorded step on the engine and the oracle: pairs and state bit for
     bit; returns the engine (recording off) and the oracle."""
-    e = _engine(p, x, r, m, v, meshes)
+    e = _engine(p, x, r, m, v, meshes, fixed=fixed)
     e.set_record(True)
-    o = _oracle(p, x, r, m, v, meshes)
+    o = _oracle(p, x, r, m, v, meshes, fixed=fixed)
     e.step(1)
     o.step(1)
     oo, oj = o.pairs()
@@ -79,17 +79,21 @@ def _finite(st):
 
 
 def test_hopper_bed_1m():
-    p, x, r, m, meshes = scenes.hopper_bed_workload(1_000_000)
+    p, x, r, m, meshes, fixed = scenes.hopper_bed_workload(1_000_000)
     n = x.shape[0]
     print(f"case fullsize hopper-bed n={n}", flush=True)
-    e, o, npairs = _first_step_equal(p, x, r, m, meshes=meshes)
+    e, o, npairs = _first_step_equal(p, x, r, m, meshes=meshes, fixed=fixed)
     st = e.stats()
-    assert npairs > 100_000 and st["contacts"] > 100_000 and st["wall_contacts"] > 1_000, st
+    assert npairs > 1_500_000 and st["contacts"] > 1_500_000 and st["wall_contacts"] > 5_000, st
     assert_cells_equal(e, o)
-    e.step(30)
+    e.step(300)
     assert e.count() == n
-    _finite(e.download())
-    assert e.stats()["wall_contacts"] > 1_000
+    s = e.download()
+    _finite(s)
+    assert np.array_equal(s["x"][fixed == 1], x[fixed == 1])  # fixed feet stay put
+    st = e.stats()
+    # the columns stand: their contacts persist
+    assert st["contacts"] > 0.95 * npairs and st["wall_contacts"] > 5_000, st
 
 
 def _momentum(st, m):
