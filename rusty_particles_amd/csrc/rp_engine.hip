@@ -392,6 +392,33 @@ __device__ inline int ref_mask_fast(int ci, int cj, const KParams& P) {
   return m;
 }
 
+// (x - origin) * RN(1/cs): the scaled coordinate ref_coord_fast floors.
+template <int K>
+__device__ inline double ref_scaled(double x, const KParams& P) {
+  return (x - P.ref_origin[K]) * P.ref_inv_cs;
+}
+
+// True when reference cell c along axis K lies inside the grid, the axis has
+// at least 2 cells and |c| is small enough for in_ref_cell's bound.
+template <int K>
+__device__ inline bool ref_window_ok(int c, const KParams& P) {
+  return c >= 0 && c < P.ref_dims[K] && P.ref_dims[K] >= 2 && c < 999990;
+}
+
+// True when x certainly lies in reference cell c along axis K (c passed
+// ref_window_ok): t = ref_scaled - c is within 2e-9 of neither end of
+// [0, 1].  ref_scaled is within 4e-10 of the rounded quotient the reference
+// floors (|q| < 1e6, see ref_coord_fast) and the subtraction adds ~1e-16,
+// so that floor is c.  A neighbour in the owner's cell on all three axes is
+// visible, and the 27-cell walk reaches it exactly once (offset 0; with at
+// least 2 cells per axis the offsets -1 and +1 lead to other cells, also
+// after a periodic wrap).
+template <int K>
+__device__ inline bool in_ref_cell(double x, int c, const KParams& P) {
+  const double t = ref_scaled<K>(x, P) - (double)c;
+  return t >= 2.0e-9 && t <= 1.0 - 2.0e-9;
+}
+
 // XCD-aware block index: consecutive logical blocks land on the same XCD
 // (hardware dispatches block b to XCD b % 8), so each XCD's L2 sees a
 // contiguous range of cell-sorted particles and their neighbours.
@@ -1078,12 +1105,12 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
   // (physics-fix mode: every pair once, as the walk's own cell)
   int mask0 = 2, mask1 = 2, mask2 = 2;
   if (!P.fix) {
-    const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+    const int cj0 = ref_coord_fast<0>(xj.x, P), cj1 = ref_coord_fast<1>(xj.y, P), cj2 = ref_coord_fast<2>(xj.z, P);
     if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
       return;
-    mask0 = ref_mask<0>(o.rc0, cj0, P);
-    mask1 = ref_mask<1>(o.rc1, cj1, P);
-    mask2 = ref_mask<2>(o.rc2, cj2, P);
+    mask0 = ref_mask_fast<0>(o.rc0, cj0, P);
+    mask1 = ref_mask_fast<1>(o.rc1, cj1, P);
+    mask2 = ref_mask_fast<2>(o.rc2, cj2, P);
   }
   const int mult = __popc(mask0) * __popc(mask1) * __popc(mask2);
   if (mult == 0) return;
@@ -1127,11 +1154,12 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
     fn = sjkr(overlap, nrm, vrel, mc.pp, rstar);
   }
   acc.njkr += (unsigned int)(jfail * mult);
-  const double fnmag = len(fn);
-  dv3 ft;
+  dv3 fc;
   if (TANG == RP_TANGENTIAL_COULOMB) {
-    ft = coulomb(fnmag, vrel, nrm, mc.mu);
+    fc = add(fn, coulomb(len(fn), vrel, nrm, mc.mu));
   } else if constexpr (HIST) {
+    const double fnmag = len(fn);
+    dv3 ft;
     // the stored displacement of (i, j): the old table is sorted by partner
     // id and the contacts come in ascending id, so a cursor finds it
     const uint32_t sj = T.sid[j];
@@ -1154,12 +1182,24 @@ __device__ inline void contact_pp(const KParams& P, const MaterialConsts& mc, do
       HW.xi[b + 2 * st] = xi.z;
       ++acc.hn;
     }
+    fc = add(fn, ft);
   } else {
-    // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49)
-    const double kt = TANG == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(rstar * overlap) : 0.0 * rstar;
-    ft = spring_coulomb_g0(fnmag, kt, mc.mu);
+    // Mindlin kt = 8*G*sqrt(R* delta), LSC kt = G*R*, with G* = 0 (material.rs:49).
+    // Where that root is finite and the friction coefficient is finite and
+    // >= 0, kt = +0 and the law returns (-0, -0, -0), which adds nothing to
+    // the normal force (x + -0 == x for every x, zeros included): the sum is
+    // the normal force as it stands, and neither root nor |F_n| is needed.
+    // Otherwise (a negative overlap's NaN root) the law runs as written.
+    const double ro = rstar * overlap;
+    const bool inert = mc.mu >= 0.0 && mc.mu < HUGE_VAL &&
+                       (TANG == RP_TANGENTIAL_LSC || (ro >= 0.0 && ro < HUGE_VAL));
+    if (inert) {
+      fc = fn;
+    } else {
+      const double kt = TANG == RP_TANGENTIAL_MINDLIN ? (8.0 * 0.0) * sqrt(ro) : 0.0 * rstar;
+      fc = add(fn, spring_coulomb_g0(len(fn), kt, mc.mu));
+    }
   }
-  const dv3 fc = add(fn, ft);
   const dv3 tc = cross(ra, fc);
   for (int rep = 0; rep < mult; ++rep) {
     acc.f = add(acc.f, fc);
@@ -1186,6 +1226,8 @@ template <int NORMAL, int JS>
 __device__ inline void contact_wall(const KParams& P, const MaterialConsts& mc, double far_pw, const Own& o,
                                     const Tri& tr, unsigned int ti, Acc& acc, unsigned int& nwall,
                                     const SurrDev& sd) {
+  // out of reach of the triangle's plane: no closest point can be within r
+  if (!(fabs(dot(tr.pn, o.x) - tr.pd) < o.r + tr.ptol)) return;
   const dv3 cp = P.fix ? closest_point_fix(tr, o.x) : closest_point(tr, o.x);
   const dv3 dv = sub(o.x, cp);
   const double dsq = dot(dv, dv);
@@ -1678,11 +1720,12 @@ __device__ inline bool collect_pair(const KParams& P, const Dyn& S, const Own& o
   const double rsum = o.r + rj;
   if (!in_contact(P, dsq, rsum)) return false;
   if (!P.fix) {  // the physics-fix mode has no reference-grid visibility
-    const int cj0 = ref_coord<0>(xj.x, P), cj1 = ref_coord<1>(xj.y, P), cj2 = ref_coord<2>(xj.z, P);
+    // the division-free forms give the same integers (ref_coord_fast)
+    const int cj0 = ref_coord_fast<0>(xj.x, P), cj1 = ref_coord_fast<1>(xj.y, P), cj2 = ref_coord_fast<2>(xj.z, P);
     if (cj0 < 0 || cj0 >= P.ref_dims[0] || cj1 < 0 || cj1 >= P.ref_dims[1] || cj2 < 0 || cj2 >= P.ref_dims[2])
       return false;
-    const int mult = __popc(ref_mask<0>(o.rc0, cj0, P)) * __popc(ref_mask<1>(o.rc1, cj1, P)) *
-                     __popc(ref_mask<2>(o.rc2, cj2, P));
+    const int mult = __popc(ref_mask_fast<0>(o.rc0, cj0, P)) * __popc(ref_mask_fast<1>(o.rc1, cj1, P)) *
+                     __popc(ref_mask_fast<2>(o.rc2, cj2, P));
     if (mult == 0) return false;
   }
   const double overlap = rsum - sqrt(dsq);
@@ -2227,10 +2270,24 @@ rp_status upload_meshes_binned(rp_engine* e) {
       tr.v1 = mk(v[3], v[4], v[5]);
       tr.v2 = mk(v[6], v[7], v[8]);
       tr.n = mk(m.n3[3 * t], m.n3[3 * t + 1], m.n3[3 * t + 2]);
+      // the plane through the vertices (the file normal may be anything)
+      const dv3 c = cross(sub(tr.v1, tr.v0), sub(tr.v2, tr.v0));
+      const double cl = len(c);
+      tr.pn = (cl > 0.0 && std::isfinite(cl)) ? divs(c, cl) : zero3();
+      tr.pd = dot(tr.pn, tr.v0);
+      double vmax = 0.0;
+      for (int q = 0; q < 9; ++q) vmax = std::max(vmax, std::fabs((double)v[q]));
+      // the rounding of pn.x - pd and of any closest point is ~1e-16 of the
+      // coordinates' size; 1e-9 of it leaves a wide margin
+      tr.ptol = 1e-9 * (1.0 + vmax);
+      tr.pad = 0.0;
       tris.push_back(tr);
     }
   }
-  // bin: every cell whose (clamped) box meets the triangle's AABB grown by r_max
+  // bin: every cell whose (clamped) box meets the triangle's AABB grown by
+  // r_max and comes within r_max of the triangle's plane (the edge cells of
+  // each axis hold the particles clamped into them, so they reach to infinity
+  // on their outer side and are kept)
   std::vector<uint32_t> counts((size_t)K.ncells + 1, 0);
   std::vector<std::pair<uint32_t, uint32_t>> entries;  // (cell, tri)
   const double grow = K.r_max * 1.0001 + 1e-12;
@@ -2267,9 +2324,24 @@ rp_status upload_meshes_binned(rp_engine* e) {
     }
     double ncov = (double)(c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1);
     if (ncov > 2.0e8) return fail(RP_ERR_NOMEM, "triangle covers too many grid cells");
+    const double pn[3] = {tr.pn.x, tr.pn.y, tr.pn.z};
+    const bool planar = pn[0] != 0.0 || pn[1] != 0.0 || pn[2] != 0.0;
     for (int z = c0[2]; z <= c1[2]; ++z)
       for (int y = c0[1]; y <= c1[1]; ++y)
         for (int x = c0[0]; x <= c1[0]; ++x) {
+          if (planar) {
+            const int cc[3] = {x, y, z};
+            double sdist = -tr.pd, reach = 0.0;
+            bool open = false;
+            for (int k = 0; k < 3; ++k) {
+              if (pn[k] == 0.0) continue;
+              if (cc[k] == 0 || cc[k] == K.gn[k] - 1) open = true;
+              const double w = 1.0 / K.ginv[k];
+              sdist += pn[k] * (K.go[k] + (cc[k] + 0.5) * w);
+              reach += std::fabs(pn[k]) * 0.5 * w;
+            }
+            if (!open && std::fabs(sdist) > (reach + grow) * (1.0 + 1e-9) + 2.0 * tr.ptol) continue;
+          }
           uint32_t c = cell_linear(K, x, y, z);
           entries.emplace_back(c, (uint32_t)t);
         }

@@ -267,8 +267,16 @@ __device__ inline double mindlin_kt_fix(double gstar, double rstar, double ov) {
 
 // ---- triangle ---------------------------------------------------------------
 
+// A triangle as the STL gives it (vertices, file normal), plus the unit
+// normal pn and offset pd of its plane (pn.x = pd on the plane; pn = 0 for a
+// degenerate triangle) and a rounding allowance ptol for tests against it.
+// Every closest point -- the reference's clamped ones of regions 2-6 too --
+// is v0 + s e0 + t e1 and lies in that plane, so a centre farther than r from
+// the plane has no contact with the triangle.
 struct Tri {
   dv3 v0, v1, v2, n;
+  dv3 pn;
+  double pd, ptol, pad;
 };
 
 // mesh.rs:45-117, with the reference's clamps in regions 2-6 (mesh.rs:67,71,75,78,95,105).
