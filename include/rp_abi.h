@@ -126,7 +126,8 @@ typedef struct rp_params {
  * 8,192 particles, else row-major. */
 enum { RP_CELL_ORDER_AUTO = 0, RP_CELL_ORDER_ROW = 1, RP_CELL_ORDER_PENCIL = 2 };
 
-/* Counters of the most recent rp_engine_step call (integers only). */
+/* Counters of the last step of the most recent rp_engine_step call
+ * (integers only). */
 typedef struct rp_stats {
   uint64_t steps;              /* steps taken since create */
   uint64_t rebuilds;           /* neighbour-list rebuilds since create */

@@ -1444,12 +1444,16 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
         const dv3 d = pair_delta(P, xo, xq[q]);
         const double dsq = dot(d, d);
         const double rsum = s_own[wv][3][p] + rq[q];
-        if (!in_contact(P, dsq, rsum)) continue;
         // distance past (r_sum - threshold)(1 + 1e-6): the overlap is below
         // the threshold by far more than rounding, so no square root is
         // needed; nearer pairs take the exact test
         const double tz = (rsum - zt) * (1.0 + 1e-6);
-        bool zero = dsq > tz * tz;
+        // Only the call's last launch (not ADVANCE) keeps the counters the
+        // host reads; the others need just the entries with force, so a pair
+        // past the zero-force distance is dropped whether in the cutoff or not.
+        if (ADVANCE && dsq > tz * tz) continue;
+        if (!in_contact(P, dsq, rsum)) continue;
+        bool zero = !ADVANCE && dsq > tz * tz;
         if (!zero) {
           const double ov = rsum - sqrt(dsq);
           zero = NORMAL == RP_NORMAL_JKR ? ov < zt - 1e-8 * fabs(ov) : ov < zt;
@@ -1458,6 +1462,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
           atomicOr(&s_def[wv][p], 1ull << k);
           continue;
         }
+        if (ADVANCE) continue;  // not counted (above)
         unsigned int mult = 1u;  // physics-fix mode: every pair once
         if (!P.fix) {
           const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
@@ -1525,6 +1530,13 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
               const dv3 d = pair_delta(P, o.x, xq[q]);
               const double dsq = dot(d, d);
               const double rsum = o.r + rq[q];
+              if (ADVANCE && zon) {
+                // not counted (the call's last launch keeps the counters):
+                // a pair past the zero-force distance with rounding to spare
+                // has no force, whether in the cutoff or not
+                const double tz = (rsum - zthr) * (1.0 + 1e-6);
+                if (dsq > tz * tz) continue;
+              }
               if (!(jj[q] >= 0 && in_contact(P, dsq, rsum))) continue;
               const double ov = rsum - sqrt(dsq);
               const bool zero = zon && (NORMAL == RP_NORMAL_JKR ? ov < zthr - 1e-8 * fabs(ov) : ov < zthr);
@@ -1532,6 +1544,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
                 defer |= 1ull << (base + q);
                 continue;
               }
+              if (ADVANCE) continue;  // counters: the call's last launch only
               unsigned int mult = 1u;  // physics-fix mode: every pair once
               if (!P.fix) {
                 const int cj0 = ref_coord_fast<0>(xq[q].x, P), cj1 = ref_coord_fast<1>(xq[q].y, P),
