@@ -667,12 +667,25 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   }
 }
 
+// Overlap below which a contact of slot i has zero force in every law the
+// zero-force pass handles (JKR with a surface energy: no root; sJKR: past
+// its cut-off), -inf for the other laws: the threshold of that pass.
+__device__ inline double zero_threshold(const KParams& P, const Stat& U, int i) {
+  const InstConsts* ic = P.ninst > 1 ? P.itab + U.inst[i] : nullptr;
+  if (P.normal == RP_NORMAL_JKR) {
+    const double g = ic ? ic->mc.pp.gamma : P.mc.pp.gamma;
+    return g > 0.0 ? (ic ? ic->jkr_far_pp : P.jkr_far_pp) : -HUGE_VAL;
+  }
+  if (P.normal == RP_NORMAL_SJKR) return ic ? ic->sjkr_zero_pp : P.sjkr_zero_pp;
+  return -HUGE_VAL;
+}
+
 // Candidates of slot i in the slots [s0, s1) of the re-ordered scratch
 // buffer D (one or more cells with consecutive numbers): appends every j with
 // |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin to i's list (the radius
 // comes with the position record); loads issued four at a time.
 __device__ inline int build_range(const KParams& P, const Dyn& D, uint32_t s0, uint32_t s1, int i, dv3 xi,
-                                  double ri, int* nbr, int cnt) {
+                                  double ri, double zthr, int* nbr, int cnt) {
   for (uint32_t j0 = s0; j0 < s1; j0 += 4) {
     dv3 xq[4];
     double rq[4];
@@ -696,8 +709,14 @@ __device__ inline int build_range(const KParams& P, const Dyn& D, uint32_t s0, u
         else if (d.z < -P.bsize[2] * 0.5) d.z += P.bsize[2];
       }
       const double lim = P.margin_factor * (ri + rq[q]) + P.skin;
-      if (dot(d, d) < lim * lim) {
-        if (cnt < P.nbr_cap) nbr[cnt * P.cap + i] = (int)j;
+      const double dsq = dot(d, d);
+      if (dsq < lim * lim) {
+        // far (bit 31): beyond the zero-force distance with rounding to
+        // spare by more than the skin, so no step before the next rebuild
+        // can give the pair a force (zero_far)
+        const double lz = (ri + rq[q] - zthr) * (1.0 + 1e-6) + P.skin;
+        const bool far = dsq > lz * lz * (1.0 + 1e-9);
+        if (cnt < P.nbr_cap) nbr[cnt * P.cap + i] = (int)(j | (far ? 0x80000000u : 0u));
         ++cnt;
       }
     }
@@ -718,7 +737,8 @@ __device__ inline int box_wrap(int c, const KParams& P, const bool* all) {
 // walk order; the list is sorted by stable id afterwards.
 template <int A0, int A1, int A2>
 __device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t* cell_start, uint32_t ibase, int i,
-                                 dv3 xi, double ri, const int* lo, const int* hi, const bool* all, int* nbr) {
+                                 dv3 xi, double ri, double zthr, const int* lo, const int* hi, const bool* all,
+                                 int* nbr) {
   int cnt = 0;
   for (int c2 = lo[A2]; c2 <= hi[A2]; ++c2) {
     const uint32_t t2 = ibase + cell_term<A2>(box_wrap<A2>(c2, P, all), P);
@@ -731,11 +751,11 @@ __device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t*
           r1 = c + 1u;
           continue;
         }
-        if (r1 > r0) cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, nbr, cnt);
+        if (r1 > r0) cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, zthr, nbr, cnt);
         r0 = c;
         r1 = c + 1u;
       }
-      cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, nbr, cnt);
+      cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, zthr, nbr, cnt);
     }
   }
   return cnt;
@@ -775,27 +795,39 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
       }
     }
     int cnt = 0;
+    const double zthr = zero_threshold(P, U, i);
     // the grid axis whose neighbouring cells have consecutive numbers walks
     // innermost, so each row of the search box is one or a few slot ranges
-    if (P.gstride[0] == 1) cnt = build_walk<0, 1, 2>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
-    else if (P.gstride[1] == 1) cnt = build_walk<1, 0, 2>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
-    else cnt = build_walk<2, 0, 1>(P, D, cell_start, ibase, i, xi, ri, lo, hi, all, nbr);
+    if (P.gstride[0] == 1) cnt = build_walk<0, 1, 2>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
+    else if (P.gstride[1] == 1) cnt = build_walk<1, 0, 2>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
+    else cnt = build_walk<2, 0, 1>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
     const int kept = min(cnt, P.nbr_cap);
-    // Ascending stable id = ascending API index: the reference's summation
-    // order inside one reference cell (grid.rs:42-46 inserts in index order).
+    // Near entries first, each part by ascending stable id = ascending API
+    // index: the reference's summation order inside one reference cell
+    // (grid.rs:42-46 inserts in index order) for every entry that can carry
+    // a force; the far part only ever adds zeros, in any order.
+    auto key = [&](int e) {
+      return ((unsigned long long)((uint32_t)e >> 31) << 32) | (unsigned long long)U.sid[e & 0x7fffffff];
+    };
     for (int a = 1; a < kept; ++a) {
       int ja = nbr[a * cap + i];
-      uint32_t ka = U.sid[ja];
+      const unsigned long long ka = key(ja);
       int b = a - 1;
       while (b >= 0) {
         int jb = nbr[b * cap + i];
-        if (U.sid[jb] <= ka) break;
+        if (key(jb) <= ka) break;
         nbr[(b + 1) * cap + i] = jb;
         --b;
       }
       nbr[(b + 1) * cap + i] = ja;
     }
+    int near = kept;
+    for (int a = kept - 1; a >= 0 && nbr[a * cap + i] < 0; --a) {
+      nbr[a * cap + i] &= 0x7fffffff;
+      near = a;
+    }
     nbr_count[i] = kept;
+    nbr_count[cap + i] = near;
     local_max = max(local_max, cnt);
     xb[i] = (float)xi.x;
     xb[i + cap] = (float)xi.y;
@@ -1351,7 +1383,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
   bool far = false;
   uint8_t flags = 0;
   Own o{};
-  int cnt = 0;
+  int cnt = 0, cnt_f = 0;  // list length; entries that can carry a force
   if (i < n) {
     flags = T.flags[i];
     ldxr(S.xr, i, o.x, o.r);
@@ -1359,6 +1391,10 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
     o.w = ld3s(S.w, i, cap);
     o.m = T.m[i];
     cnt = nbr_count[i];
+    // an advancing launch needs only the near part of the list: the far
+    // entries have zero force until the next rebuild (k_build), and only
+    // the call's last launch counts them
+    cnt_f = (ADVANCE && kSplit) ? nbr_count[cap + i] : cnt;
   }
   const bool live = i < n && !(flags & FLAG_DEAD);
   if (live) {
@@ -1395,7 +1431,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
         zthr = BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp;
       }
     }
-    const int ka = live ? min(cnt, 64) : 0;
+    const int ka = live ? min(cnt_f, 64) : 0;
     int incl = ka;
     for (int sh = 1; sh < 64; sh <<= 1) {
       const int v = __shfl_up(incl, sh, 64);
@@ -1513,7 +1549,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
           const double zthr = NORMAL == RP_NORMAL_JKR ? far_pp
                                                       : (BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp);
           const bool zon = NORMAL == RP_NORMAL_SJKR || mc.pp.gamma > 0.0;
-          const int ka = min(cnt, 64);
+          const int ka = min(cnt_f, 64);
           for (int base = 0; base < ka; base += kNbrBatch) {
             int jj[kNbrBatch];
             dv3 xq[kNbrBatch];
@@ -1569,7 +1605,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
           contact_pp<NORMAL, TANG, RECORD, JS, HIST>(P, mc, far_pp, S, T, o, nbr[k * cap + i], k, acc, ctl, prow, sd,
                                                      HR, HW, i);
         }
-        for (int k = 64; k < cnt; ++k) {
+        for (int k = 64; k < cnt_f; ++k) {
           const int j = nbr[k * cap + i];
           dv3 xj;
           double rj;
@@ -3060,7 +3096,8 @@ rp_status upload_impl(rp_engine* e, size_t n, const double* x3, const double* v3
   HIP_TRY(hipMalloc(&e->d_xb, (size_t)cap * 3 * sizeof(float)));
   HIP_TRY(hipMalloc(&e->d_key, (size_t)cap * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_tmp, (size_t)cap * sizeof(uint32_t)));
-  HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * sizeof(int)));
+  // list length of slot i at [i], its near part (k_build) at [cap + i]
+  HIP_TRY(hipMalloc(&e->d_nbr_count, (size_t)cap * 2 * sizeof(int)));
   HIP_TRY(hipMalloc(&e->d_bstats, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMemset(e->d_bstats, 0, ((size_t)cap / kForceBlock + 1) * sizeof(BlockStats)));
   HIP_TRY(hipMalloc(&e->d_surr_base, (size_t)cap * sizeof(int)));
