@@ -319,8 +319,9 @@ rp_status rp_engine_cells(rp_engine* e, int32_t rebuild, int64_t* ref_cell, uint
  * times each candidate Verlet skin (the current one, then 0.2 .. 0.005 of
  * the largest radius, never below the automatic skin's motion bound nor the
  * fall of a particle released at rest over 1,000 steps) and broad-phase cell
- * (automatic or half the largest cutoff + skin) over chunks of `steps` steps
- * until it has rebuilt its lists 3 times (at most 10 chunks), keeps the
+ * (automatic or half the largest cutoff + skin) over calls of `steps` steps
+ * (at least 100) until it has rebuilt its lists 3 times (at most 10 calls),
+ * keeps the
  * fastest in the engine's parameters (the current one unless beaten by 3 %)
  * and returns it (0 = automatic).  Where the force kernel has a zero-force
  * pass (JKR with the shortcut or surrogate, sJKR; Coulomb or spring

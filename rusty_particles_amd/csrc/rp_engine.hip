@@ -3803,10 +3803,11 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   // candidates: the current setting first, then skins as a fraction of the
   // largest radius (never below the motion bound the automatic skin uses,
   // nor the fall bound below)
-  // with automatic or half-cutoff cells.  Each runs in chunks of `steps`
-  // until it has rebuilt its lists 3 times or run 10 chunks, so a small skin
-  // pays for its frequent rebuilds; another setting must beat the current one
-  // by 3 % to replace it.
+  // with automatic or half-cutoff cells.  Each runs in calls of `steps`
+  // steps (at least 100, so that the at-rest start and end of a call weigh
+  // little) until it has rebuilt its lists 3 times or run 10 calls, so a
+  // small skin pays for its frequent rebuilds; another setting must beat the
+  // current one by 3 % to replace it.
   // Speeds grow after tuning under gravity: a skin must also hold the fall of
   // a particle released at rest over 1,000 steps, |g| (1000 dt)^2 (twice the
   // distance, as the rebuild check allows half the skin).
@@ -3825,6 +3826,7 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   }
   double best_t = HUGE_VAL, best_skin = skin0, best_cell = cell0;
   e->timing = false;
+  steps = std::max<uint32_t>(steps, 100u);
   for (size_t q = 0; q < cand.size() && st == RP_OK; ++q) {
     if ((st = restore())) break;
     e->prm.skin = cand[q].first;
