@@ -667,17 +667,33 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(Ctl* ctl, int par, KPar
   }
 }
 
-// Overlap below which a contact of slot i has zero force in every law the
-// zero-force pass handles (JKR with a surface energy: no root; sJKR: past
-// its cut-off), -inf for the other laws: the threshold of that pass.
-__device__ inline double zero_threshold(const KParams& P, const Stat& U, int i) {
-  const InstConsts* ic = P.ninst > 1 ? P.itab + U.inst[i] : nullptr;
-  if (P.normal == RP_NORMAL_JKR) {
-    const double g = ic ? ic->mc.pp.gamma : P.mc.pp.gamma;
-    return g > 0.0 ? (ic ? ic->jkr_far_pp : P.jkr_far_pp) : -HUGE_VAL;
+// Overlap below which a particle-particle contact of a particle of radius r
+// has zero force in every law the zero-force pass handles, -inf for the
+// other laws (and for JKR without surface energy).  Its contacts have
+// R* = r r_j / (r + r_j) <= rs = r r_max / (r + r_max), and both thresholds
+// fall as R* grows -- JKR's no-root bound t d_min - 1e-3 R* - 1e-8 (R* + t)
+// with t = R*^(1/3) jc^(2/3) (jkr_no_root), sJKR's cut-off
+// -sqrt(3 pi^2 gamma^2 R* / E*^2) (physics.rs:256-260) -- so the value at rs,
+// pushed down by 1e-12 (JKR) or 1e-9 (sJKR) of itself for rounding, bounds
+// every contact of the particle: a contact below it has zero force.
+__device__ inline double zero_threshold_r(const KParams& P, const PairConsts& c, double r, int normal) {
+  const double rs = r * P.r_max / (r + P.r_max);
+  if (normal == RP_NORMAL_JKR) {
+    if (!(c.gamma > 0.0)) return -HUGE_VAL;
+    const double t = cbrt(rs) * c.jc23;
+    return (kJkrDMin * t - 1e-3 * rs - 1e-8 * (rs + t)) * (1.0 + 1e-12);
   }
-  if (P.normal == RP_NORMAL_SJKR) return ic ? ic->sjkr_zero_pp : P.sjkr_zero_pp;
+  if (normal == RP_NORMAL_SJKR) {
+    if (!(c.gamma > 0.0)) return 0.0;
+    return -sqrt(c.sterm * rs / c.estar_sq) * (1.0 + 1e-9);
+  }
   return -HUGE_VAL;
+}
+
+// The threshold of zero_threshold_r for slot i of the re-ordered buffers.
+__device__ inline double zero_threshold(const KParams& P, const Stat& U, int i, double r) {
+  const InstConsts* ic = P.ninst > 1 ? P.itab + U.inst[i] : nullptr;
+  return zero_threshold_r(P, ic ? ic->mc.pp : P.mc.pp, r, P.normal);
 }
 
 // Candidates of slot i in the slots [s0, s1) of the re-ordered scratch
@@ -795,7 +811,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
       }
     }
     int cnt = 0;
-    const double zthr = zero_threshold(P, U, i);
+    const double zthr = zero_threshold(P, U, i, ri);
     // the grid axis whose neighbouring cells have consecutive numbers walks
     // innermost, so each row of the search box is one or a few slot ranges
     if (P.gstride[0] == 1) cnt = build_walk<0, 1, 2>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
@@ -1424,12 +1440,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
     double zthr = -HUGE_VAL;
     if (live) {
       const InstConsts* ic = BATCH ? P.itab + T.inst[i] : nullptr;
-      if (NORMAL == RP_NORMAL_JKR) {
-        const double g = BATCH ? ic->mc.pp.gamma : P.mc.pp.gamma;
-        if (g > 0.0) zthr = BATCH ? ic->jkr_far_pp : P.jkr_far_pp;
-      } else {
-        zthr = BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp;
-      }
+      zthr = zero_threshold_r(P, BATCH ? ic->mc.pp : P.mc.pp, o.r, NORMAL);
     }
     const int ka = live ? min(cnt_f, 64) : 0;
     int incl = ka;
@@ -1546,8 +1557,7 @@ __global__ void __launch_bounds__(kForceBlock) RP_FORCE_ATTR k_force(Ctl* ctl, i
         if constexpr (!DEAL) {
           // The zero-force pass in one lane per particle (the tiling the
           // autotuner may pick instead of the wave-dealt one above).
-          const double zthr = NORMAL == RP_NORMAL_JKR ? far_pp
-                                                      : (BATCH ? ic->sjkr_zero_pp : P.sjkr_zero_pp);
+          const double zthr = zero_threshold_r(P, mc.pp, o.r, NORMAL);
           const bool zon = NORMAL == RP_NORMAL_SJKR || mc.pp.gamma > 0.0;
           const int ka = min(cnt_f, 64);
           for (int base = 0; base < ka; base += kNbrBatch) {

@@ -74,6 +74,7 @@ struct PairConsts {
   double spull;     // (1.5*pi)*gamma              physics.rs:252
   double sterm;     // (3*(pi*pi))*(gamma*gamma)   physics.rs:256
   double estar_sq;  // E*·E*                       physics.rs:256
+  double jc23;      // jc^(2/3): the JKR length t = R*^(1/3) jc^(2/3)
 };
 
 // PairConsts from E* and the pair's surface energy (same operation order as
@@ -93,6 +94,7 @@ __host__ __device__ inline PairConsts pair_consts(double estar, double gamma) {
   c.spull = 1.5 * pi * gamma;
   c.sterm = 3.0 * (pi * pi) * (gamma * gamma);
   c.estar_sq = estar * estar;
+  c.jc23 = pow(c.jc, 2.0 / 3.0);
   return c;
 }
 
