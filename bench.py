@@ -32,7 +32,9 @@ time (each with its own launches) are timed beside it.
 --workload jkr-batch: 64 periodic JKR packings (N_k ~ U{5000..40000}, 1.5M
 spheres, r lognormal median 50 um, surface energy ~U[0.02, 0.08] J/m^2) in
 one batched engine, once per JKR solver (exact, shortcut, surrogate); value
-= the surrogate+batched form.
+= the surrogate+batched form.  With --physics-fix (batch workloads): the
+physics-fix mode with Mindlin springs and their tangential history, and a
+friction coefficient drawn per instance on top of the sweep.
 
 --workload drum: the capacity run, 16,000,000 spheres (r ~ U[0.4, 0.8] mm,
 rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m turning at
@@ -200,6 +202,15 @@ def run_batch(args) -> int:
         v = None
         forms = ["plain"]
         desc = "sJKR+Coulomb, material sweep"
+    if args.physics_fix:
+        # Mindlin springs with their history; the sweep adds friction, which
+        # the parity mode's inert spring (shear modulus 0) would ignore
+        p.mode = abi.RP_MODE_PHYSICS_FIX
+        p.tangential_model = abi.RP_TANGENTIAL_MINDLIN
+        rng = np.random.default_rng(0x5EED)
+        mats = [abi.RpMaterial(q.youngs_modulus, q.poissons_ratio, q.density, float(rng.uniform(0.1, 0.9)),
+                               q.restitution_coefficient, q.surface_energy) for q in mats]
+        desc = desc.replace("+Coulomb", "+Mindlin with history (physics-fix mode)") + ", friction sweep"
     n = int(sum(counts))
     offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     log(f"workload {args.workload}: {len(counts)} instances, {n} particles, built in {time.perf_counter() - t0:.1f}s")
@@ -268,8 +279,10 @@ def run_batch(args) -> int:
         tm = e.timing()
         e.set_timing(False)
         avg_s = tm["force"][0] / max(tm["force"][1], 1) * 1e-3
+        alg_bytes = FORCE_BYTES_PER_PARTICLE * n + HIST_BYTES_PER_CONTACT * st.get("history_contacts", 0)
         res = {"value": value, "ms_per_step": wall / args.steps * 1e3, "force_kernel_avg_us": avg_s * 1e6,
-               "achieved_gbs": FORCE_BYTES_PER_PARTICLE * n / avg_s / 1e9, "contacts_per_step": st["contacts"],
+               "achieved_gbs": alg_bytes / avg_s / 1e9, "contacts_per_step": st["contacts"],
+               "history_contacts_per_step": st.get("history_contacts", 0),
                "contacts_per_s": st["contacts"] * args.steps / wall,
                "repeat_values": [n * args.steps / w for w in walls], "wall": wall, "stats": st}
         res.update(surrogate_kernels(tm))
@@ -328,6 +341,7 @@ def run_batch(args) -> int:
     copy_gbs = 2 * nbytes / ms / 1e6
     log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
 
+    wname = args.workload + ("-fix" if args.physics_fix else "")
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or cpu_info()["threads"]
@@ -352,16 +366,17 @@ def run_batch(args) -> int:
         "value": R["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {len(counts)} instances, {n} spheres in all, {desc}, dt={p.dt:g}, "
+        "config": {"workload": f"{wname}: {len(counts)} instances, {n} spheres in all, {desc}, dt={p.dt:g}, "
                                f"FP64", "particles": n, "instances": len(counts),
                    "form": "batched" if main == "plain" else f"{main}+batched"},
         "roofline": {"bound": "hbm", "achieved": R["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(args.workload)[0],
-                     "traffic_source": traffic_of(args.workload)[1]},
+                     "frac": R["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_of(wname)[0],
+                     "traffic_source": traffic_of(wname)[1]},
         "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "repeats": len(R["repeat_values"]), "repeat_values": R["repeat_values"],
         "contacts_per_step": R["contacts_per_step"], "contacts_per_s": R["contacts_per_s"],
+        "history_contacts_per_step": R["history_contacts_per_step"],
         "force_kernel_avg_us": R["force_kernel_avg_us"],
         "one_engine_per_instance_value": sep_value,
     }
@@ -652,6 +667,9 @@ def main() -> int:
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
+    ap.add_argument("--physics-fix", action="store_true",
+                    help="batch workloads: the physics-fix mode with Mindlin springs and their tangential "
+                         "history, friction drawn per instance (the autotuner is off: it refuses a history)")
     ap.add_argument("--autotune", type=int, default=None,
                     help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
                          "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
@@ -663,7 +681,7 @@ def main() -> int:
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.autotune is None:
-        args.autotune = 0 if args.workload in ("hopper", "box", "drum", "drum-fix") else 20
+        args.autotune = 0 if args.workload in ("hopper", "box", "drum", "drum-fix") or args.physics_fix else 20
     if args.steps is None:
         args.steps = 1000 if args.workload in ("drum", "drum-fix") else 5000
     if args.jkr_forms is None:
@@ -712,6 +730,7 @@ def surrogate_batched(args) -> dict:
     form); the surrogate's force errors from check mode."""
     a = argparse.Namespace(**vars(args))
     a.workload, a.instances, a.jkr_forms, a.autotune = "jkr-batch", 64, "exact,shortcut,surrogate", 0
+    a.physics_fix = False
     a.no_compare, a.no_cpu, a.repeats, a.timing_steps = True, True, 1, 50
     log("surrogate+batched: jkr-batch workload")
     r = run_batch(a)

@@ -83,8 +83,9 @@ enum {
  *     parameters (mesh.rs:63-113 clamps unnormalised ones to [0, a] / [0, c]);
  *   - every pair inside the cutoff acts once: no reference-grid visibility
  *     or double visits (grid.rs:48-87 depends on a 1 m cell).
- * Walls keep Coulomb friction (simulation.rs:512-514).  Single engines only
- * (not rp_engine_upload_batch). */
+ * Walls keep Coulomb friction (simulation.rs:512-514).  Batched engines
+ * (rp_engine_upload_batch) keep a history per instance, keyed by the stable
+ * ids that are unique across the batch. */
 enum { RP_MODE_PARITY = 0, RP_MODE_PHYSICS_FIX = 1 };
 
 /* Material, field order as Material::new (src/material.rs:12) and the
@@ -289,12 +290,27 @@ rp_status rp_engine_pairs(rp_engine* e, int64_t* offsets, int64_t* js,
 /* Tangential history of the physics-fix mode (no counterpart in the
  * reference, whose Contact map is internal, src/contact.rs:3-18): per
  * particle in API order, its loaded directed contacts -- partner's API index
- * (ascending) and stored displacement xi (3 doubles) -- as the next step
+ * (ascending; in a batched engine the index over all instances, as in the
+ * downloads) and stored displacement xi (3 doubles) -- as the next step
  * will read them.  offsets has n+1 entries; partner and xi3 receive up to cap
  * entries (xi3: 3 per entry); *total receives the count.  Empty when no
  * history is kept (parity mode or Coulomb friction). */
 rp_status rp_engine_history(rp_engine* e, int64_t* offsets, int64_t* partner, double* xi3, size_t cap,
                             size_t* total);
+
+/* Loads a tangential history in the layout rp_engine_history returns (no
+ * counterpart in the reference, whose contacts map never leaves its
+ * Simulation, src/simulation.rs:30): the particles' loaded contacts, per particle in API order by
+ * ascending partner index, as the next step will read them.  Lets a caller
+ * that re-uploads a state (an edit between runs, an ensemble formed from
+ * simulations that have run) keep the history it read before.  Replaces the
+ * current tables; entries with partner -1 (a removed particle) are skipped.
+ * RP_ERR_INVALID for offsets that do not run from 0 to total without
+ * decreasing, a partner out of range or not ascending, or more entries for a
+ * particle than its neighbour list can hold; RP_ERR_STATE for entries when no
+ * history is kept (parity mode or Coulomb friction). */
+rp_status rp_engine_set_history(rp_engine* e, const int64_t* offsets, const int64_t* partner,
+                                const double* xi3, size_t total);
 
 /* Cells of the broad phase, for checking them (no counterpart in the
  * reference, whose Grid keeps them private, src/grid.rs:30-46).  With

@@ -161,6 +161,7 @@ EXPORTS = (
     "rp_engine_pairs",
     "rp_engine_cells",
     "rp_engine_history",
+    "rp_engine_set_history",
     "rp_engine_stats",
     "rp_engine_set_timing",
     "rp_engine_timing",

@@ -2643,8 +2643,6 @@ rp_status read_ctl(rp_engine* e) {
 rp_status prepare(rp_engine* e) {
   if (!e->have_particles) return fail(RP_ERR_STATE, "no particles uploaded");
   if (!e->geometry_dirty) return RP_OK;
-  if (e->prm.mode == RP_MODE_PHYSICS_FIX && e->ninst > 1)
-    return fail(RP_ERR_INVALID, "the physics-fix mode runs single engines, not batched ensembles");
   const bool hist = e->prm.mode == RP_MODE_PHYSICS_FIX && e->prm.tangential_model != RP_TANGENTIAL_COULOMB;
   if (hist != e->hist_on) {
     e->hist_on = hist;
@@ -2898,24 +2896,38 @@ void launch_force_s(rp_engine* e, int flag_slot) {
   }
 }
 
-// Batched engines (several instances) use the per-instance material table;
-// they have no pair recording and no physics-fix history.
+// Batched engines (several instances): the JKR solver variants with the
+// per-instance material table.
+template <int NORMAL, int TANG, bool ADV, bool HIST>
+void launch_force_b(rp_engine* e, int flag_slot) {
+  if constexpr (NORMAL == RP_NORMAL_JKR) {
+    if (e->jkr_solver == RP_JKR_SURROGATE) {
+      if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, false, 3, true, HIST>(e, flag_slot);
+      else launch_force_t<NORMAL, TANG, ADV, false, 2, true, HIST>(e, flag_slot);
+      return;
+    }
+    if (e->jkr_solver == RP_JKR_SHORTCUT) {
+      launch_force_t<NORMAL, TANG, ADV, false, 1, true, HIST>(e, flag_slot);
+      return;
+    }
+  }
+  launch_force_t<NORMAL, TANG, ADV, false, 0, true, HIST>(e, flag_slot);
+}
+
+// Batched engines have no pair recording; in the physics-fix mode they keep
+// the tangential history as single engines do (stable ids are unique across
+// instances, so they key the tables unchanged).
 template <int NORMAL, int TANG, bool ADV, bool REC>
 void launch_force_j(rp_engine* e, int flag_slot) {
   if constexpr (!REC) {
     if (e->ninst > 1) {
-      if constexpr (NORMAL == RP_NORMAL_JKR) {
-        if (e->jkr_solver == RP_JKR_SURROGATE) {
-          if (e->surr_check) launch_force_t<NORMAL, TANG, ADV, false, 3, true, false>(e, flag_slot);
-          else launch_force_t<NORMAL, TANG, ADV, false, 2, true, false>(e, flag_slot);
-          return;
-        }
-        if (e->jkr_solver == RP_JKR_SHORTCUT) {
-          launch_force_t<NORMAL, TANG, ADV, false, 1, true, false>(e, flag_slot);
+      if constexpr (TANG != RP_TANGENTIAL_COULOMB) {
+        if (e->hist_on) {
+          launch_force_b<NORMAL, TANG, ADV, true>(e, flag_slot);
           return;
         }
       }
-      launch_force_t<NORMAL, TANG, ADV, false, 0, true, false>(e, flag_slot);
+      launch_force_b<NORMAL, TANG, ADV, false>(e, flag_slot);
       return;
     }
   }
@@ -3216,8 +3228,6 @@ rp_status rp_engine_upload_batch(rp_engine* e, int32_t ninst, const size_t* coun
   if (ninst < 1 || ninst > 65535) return fail(RP_ERR_INVALID, "ninst must lie in [1, 65535]");
   if (!counts || !particle) return fail(RP_ERR_INVALID, "counts and particle materials are required");
   if (e->record && ninst > 1) return fail(RP_ERR_STATE, "pair recording is not available for batched engines");
-  if (e->prm.mode == RP_MODE_PHYSICS_FIX && ninst > 1)
-    return fail(RP_ERR_INVALID, "the physics-fix mode runs single engines, not batched ensembles");
   size_t n = 0;
   for (int q = 0; q < ninst; ++q) n += counts[q];
   std::vector<uint16_t> inst(n);
@@ -3576,6 +3586,63 @@ rp_status rp_engine_history(rp_engine* e, int64_t* offsets, int64_t* partner, do
     }
   }
   offsets[order.size()] = (int64_t)o;
+  return RP_OK;
+}
+
+rp_status rp_engine_set_history(rp_engine* e, const int64_t* offsets, const int64_t* partner, const double* xi3,
+                                size_t total) {
+  if (!e || !offsets) return fail(RP_ERR_INVALID, "NULL argument");
+  if (!e->have_particles) return fail(RP_ERR_STATE, "no particles uploaded");
+  if (total > 0 && (!partner || !xi3)) return fail(RP_ERR_INVALID, "partner and xi3 are required");
+  // sizes the lists (and with them the tables) and sorts the slots, as the
+  // first step would; a later step finds the engine prepared
+  rp_status st = prepare(e);
+  if (st) return st;
+  std::vector<int> order;
+  Dyn cur;
+  Stat cst;
+  int ns;
+  if ((st = slot_order(e, order, cur, cst, ns))) return st;
+  const size_t n = order.size();
+  if (offsets[0] != 0 || offsets[n] < 0 || (size_t)offsets[n] != total)
+    return fail(RP_ERR_INVALID, "offsets must run from 0 to total");
+  for (size_t q = 0; q < n; ++q)
+    if (offsets[q + 1] < offsets[q]) return fail(RP_ERR_INVALID, "offsets must not decrease");
+  if (!e->hist_on) {
+    if (total > 0) return fail(RP_ERR_STATE, "no tangential history is kept (parity mode or Coulomb friction)");
+    return RP_OK;
+  }
+  const size_t c = (size_t)e->cap, h = (size_t)e->hcap;
+  std::vector<uint32_t> sid((size_t)std::max(ns, 1));
+  if (ns > 0) HIP_TRY(hipMemcpy(sid.data(), cst.sid, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<int> cnt(c, 0);
+  std::vector<uint32_t> key(h * c, 0u);
+  std::vector<double> xi(3 * h * c, 0.0);
+  for (size_t q = 0; q < n; ++q) {
+    const size_t i = (size_t)order[q];
+    size_t k = 0;
+    int64_t prev = -1;
+    for (int64_t o = offsets[q]; o < offsets[q + 1]; ++o) {
+      const int64_t pj = partner[o];
+      if (pj < 0) continue;  // a partner removed since the history was read
+      if ((size_t)pj >= n || (size_t)pj == q) return fail(RP_ERR_INVALID, "history partner out of range");
+      if (pj <= prev) return fail(RP_ERR_INVALID, "history partners must ascend");
+      prev = pj;
+      if (k >= h) return fail(RP_ERR_INVALID, "more history entries than the neighbour lists hold");
+      // API order is ascending stable id, so the keys ascend as the kernel's cursor needs
+      const size_t b = k * c + i;
+      key[b] = sid[(size_t)order[(size_t)pj]];
+      xi[b] = xi3[3 * o];
+      xi[b + h * c] = xi3[3 * o + 1];
+      xi[b + 2 * h * c] = xi3[3 * o + 2];
+      ++k;
+    }
+    cnt[i] = (int)k;
+  }
+  const Hist& H = e->hist[e->cur];
+  HIP_TRY(hipMemcpy(H.cnt, cnt.data(), c * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(H.key, key.data(), key.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(H.xi, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice));
   return RP_OK;
 }
 

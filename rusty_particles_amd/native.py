@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
     L.rp_engine_pairs.argtypes = [_eng, _i64p, _i64p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.rp_engine_cells.argtypes = [_eng, C.c_int32, _i64p, C.POINTER(C.c_uint32), _i64p, _dp]
     L.rp_engine_history.argtypes = [_eng, _i64p, _i64p, _dp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rp_engine_set_history.argtypes = [_eng, _i64p, _i64p, _dp, C.c_size_t]
     L.rp_engine_stats.argtypes = [_eng, C.POINTER(abi.RpStats)]
     L.rp_engine_set_timing.argtypes = [_eng, C.c_int]
     L.rp_engine_timing.argtypes = [_eng, _dp, C.POINTER(C.c_uint64)]
@@ -75,7 +76,7 @@ def lib() -> C.CDLL:
     for name in ("rp_engine_create", "rp_engine_set_params", "rp_engine_upload", "rp_engine_upload_full",
                  "rp_engine_add_mesh", "rp_engine_add_mesh_stl", "rp_engine_set_mesh_motion", "rp_engine_step", "rp_engine_synchronize",
                  "rp_engine_count", "rp_engine_download", "rp_engine_download_aux", "rp_engine_set_record",
-                 "rp_stl_read", "rp_engine_pairs", "rp_engine_cells", "rp_engine_history", "rp_engine_stats", "rp_engine_set_timing",
+                 "rp_stl_read", "rp_engine_pairs", "rp_engine_cells", "rp_engine_history", "rp_engine_set_history", "rp_engine_stats", "rp_engine_set_timing",
                  "rp_engine_timing", "rp_probe_copy", "rp_engine_set_jkr_solver", "rp_engine_jkr_stats",
                  "rp_engine_upload_batch", "rp_engine_instance_counts", "rp_engine_set_graph",
                  "rp_engine_autotune", "rp_engine_set_tiling", "rp_engine_get_tiling"):
@@ -259,6 +260,19 @@ class Engine:
         _check(self._L.rp_engine_history(self._h, _ptr(offs, C.c_int64), _ptr(part, C.c_int64), _ptr(xi),
                                          part.size, C.byref(tot)))
         return offs, part[:tot.value], xi[:tot.value]
+
+    def set_history(self, offsets, partner, xi) -> None:
+        """Loads a tangential history in the layout of history()
+        (rp_engine_set_history), e.g. one read before a re-upload."""
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        part = np.ascontiguousarray(partner, dtype=np.int64).reshape(-1)
+        x = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 3)
+        if offs.size != self.count() + 1:
+            raise ValueError("offsets needs one entry per particle plus one")
+        if x.shape[0] != part.size:
+            raise ValueError("partner and xi differ in length")
+        _check(self._L.rp_engine_set_history(self._h, _ptr(offs, C.c_int64), _ptr(part, C.c_int64), _ptr(x),
+                                             part.size))
 
     def cells(self, rebuild: bool = True) -> dict:
         """Broad-phase cells (rp_engine_cells): per particle in API order the

@@ -72,6 +72,9 @@ class Simulation:
         self._record = False
         self._jkr = ("exact", None, False)  # solver, surrogate, check
         self._ovl_seen = 0  # overlap_steps of the engine already warned about
+        # physics-fix tangential history read with the host state (offsets,
+        # partner, xi), loaded again after a re-upload
+        self._hist: tuple[np.ndarray, np.ndarray, np.ndarray] | None = None
         # bumped by every change of state, parameters or meshes made through
         # this object (an Ensemble re-uploads a member whose version moved)
         self._version = 0
@@ -90,6 +93,7 @@ class Simulation:
         h["radius"] = np.append(h["radius"], float(radius))
         h["mass"] = np.append(h["mass"], float(mass))
         h["fixed"] = np.append(h["fixed"], np.uint8(1 if fixed else 0))
+        self._extend_history(1)
         self._engine_valid = False
         self._version += 1
 
@@ -106,6 +110,7 @@ class Simulation:
         h["mass"] = np.concatenate([h["mass"], np.broadcast_to(np.asarray(masses, dtype=np.float64), (n,))])
         f = np.zeros(n, dtype=np.uint8) if fixed is None else np.asarray(fixed, dtype=np.uint8).reshape(n)
         h["fixed"] = np.concatenate([h["fixed"], f])
+        self._extend_history(n)
         self._engine_valid = False
         self._version += 1
 
@@ -181,6 +186,8 @@ class Simulation:
             h = self._h
             eng.upload(h["x"], h["radius"], h["mass"], h["fixed"], v=h["v"], w=h["w"], a=h["a"], tau=h["tau"],
                        rx=h["rx"], rv=h["rv"], rw=h["rw"])
+            if self._keeps_history() and self._hist is not None and self._hist[1].size:
+                eng.set_history(*self._hist)
             self._engine_valid = True
             self._ovl_seen = 0  # the upload starts the engine's counters afresh
         return eng
@@ -223,6 +230,16 @@ class Simulation:
         """Extension: broad-phase cell size and Verlet skin (0 = automatic)."""
         self._params.cell_size = float(cell_size)
         self._params.skin = float(skin)
+        self._params_changed()
+
+    def set_physics_fix(self, on=True):
+        """Extension (no counterpart in the reference): the physics-fix mode
+        (RP_MODE_PHYSICS_FIX) -- the shear modulus from E and nu, Mindlin's
+        stiffness only under compression, a tangential history per contact
+        that persists across steps and re-uploads, Eberly's closest point,
+        every pair inside the cutoff once.  Off (the default): the
+        reference's arithmetic, quirks included."""
+        self._params.mode = abi.RP_MODE_PHYSICS_FIX if on else abi.RP_MODE_PARITY
         self._params_changed()
 
     def set_jkr_solver(self, solver="exact", surrogate=None, check=False):
@@ -380,7 +397,18 @@ class Simulation:
             return
         st = self._engine.download()
         self._h = {k: st[k] for k in (*_FIELDS3, "radius", "mass", "fixed")}
+        self._hist = self._engine.history() if self._keeps_history() else None
         self._host_valid = True
+
+    def _keeps_history(self) -> bool:
+        p = self._params
+        return p.mode == abi.RP_MODE_PHYSICS_FIX and p.tangential_model != abi.RP_TANGENTIAL_COULOMB
+
+    def _extend_history(self, n: int) -> None:
+        """n particles appended: they start without history."""
+        if self._hist is not None:
+            offs, part, xi = self._hist
+            self._hist = (np.concatenate([offs, np.full(n, offs[-1], dtype=np.int64)]), part, xi)
 
 
 class Ensemble:
@@ -393,7 +421,9 @@ class Ensemble:
     alone with the surrogate, to its force-error bound).  After run()/step()
     every member's getters return its new state.  A member changed between
     two ensemble steps (a setter, add_particle, or a step of its own) makes
-    the next ensemble step upload every member's current state again."""
+    the next ensemble step upload every member's current state again.  In the
+    physics-fix mode each member's tangential history goes with its state,
+    into the ensemble and back."""
 
     def __init__(self, sims):
         self.sims = list(sims)
@@ -458,6 +488,23 @@ class Ensemble:
         solver, net, check = self.sims[0]._jkr
         if solver != "exact":
             eng.set_jkr_solver(solver, net, check=check)
+        if self.sims[0]._keeps_history():
+            # the members' tables, partners shifted to the batch's indices
+            offs, part, xi = [np.zeros(1, dtype=np.int64)], [], []
+            ne = npart = 0
+            for s, h in zip(self.sims, hs):
+                n = h["x"].shape[0]
+                if s._hist is not None:
+                    o_, p_, x_ = s._hist
+                    offs.append(o_[1:] + ne)
+                    part.append(np.where(p_ >= 0, p_ + npart, -1))
+                    xi.append(x_)
+                    ne += int(o_[-1])
+                else:
+                    offs.append(np.full(n, ne, dtype=np.int64))
+                npart += n
+            if ne:
+                eng.set_history(np.concatenate(offs), np.concatenate(part), np.concatenate(xi))
         self._engine = eng
 
     def _steps(self, n: int) -> None:
@@ -467,10 +514,17 @@ class Ensemble:
         # members see their new state through their own getters
         counts = self._engine.instance_counts()
         st = self._engine.download()
+        hist = self._engine.history() if self.sims[0]._keeps_history() else None
         o = 0
         for s, c in zip(self.sims, counts):
+            c = int(c)
             s._h = {k: np.array(st[k][o:o + c]) for k in (*_FIELDS3, "radius", "mass", "fixed")}
+            s._hist = None
+            if hist is not None:
+                a, b = int(hist[0][o]), int(hist[0][o + c])
+                p_ = hist[1][a:b]
+                s._hist = (hist[0][o:o + c + 1] - a, np.where(p_ >= 0, p_ - o, -1), np.array(hist[2][a:b]))
             s._host_valid = True
             s._engine_valid = False
-            o += int(c)
+            o += c
         self._versions = [s._version for s in self.sims]

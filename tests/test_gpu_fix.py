@@ -9,7 +9,15 @@
 * Wall contacts through Eberly's closest point in every branch.
 * The capacity run as the benchmark specifies it: JKR (surrogate with exact
   fallback) + Mindlin with history in the turning drum, at oracle size.
-* What the mode refuses: batched ensembles, the autotuner with a history.
+* Batched ensembles: each instance (its own friction, restitution and
+  surface energy) evolves as a single engine given it alone, history tables
+  included, bit for bit -- with the exact solve, the shortcut and the
+  surrogate.
+* A history read with the state and loaded again after a re-upload
+  (rp_engine_set_history): the run continues bit for bit as if never
+  interrupted -- through the engine, through Simulation (an edit between
+  runs) and through an Ensemble formed from members that have run.
+* What the mode refuses: the autotuner with a history; malformed histories.
 """
 import math
 
@@ -193,13 +201,246 @@ def test_capacity_drum_jkr_mindlin(drum_fix, solver):
     assert e.stats()["history_contacts"] > 100
 
 
+def _batched_vs_single(p, inst, meshes, chunks, motions=(), solver=None, net=None):
+    """Steps the instances in one batched engine and each alone; asserts
+    equal states and history tables (partners shifted by the instance's
+    offset) and returns the number of history entries."""
+    e = native.Engine(copy_params(p))
+    cat = lambda key: np.concatenate([sc[key] for sc, _ in inst])
+    e.upload_batch([sc["x"].shape[0] for sc, _ in inst], [m for _, m in inst], cat("x"), cat("radius"),
+                   cat("mass"), cat("fixed"), v=cat("v"), w=cat("w"))
+    for v9, n3 in meshes:
+        e.add_mesh(v9, n3)
+    for k, org, om in motions:
+        e.set_mesh_motion(k, org, om)
+    if solver:
+        e.set_jkr_solver(solver, net)
+    for c in chunks:
+        e.step(c)
+    got = e.download()
+    go, gp, gx = e.history()
+    assert e.stats()["history_contacts"] == gp.size
+    off = nh = 0
+    for q, (sc, mat) in enumerate(inst):
+        pq = copy_params(p)
+        pq.particle = mat
+        s1 = native.Engine(pq)
+        s1.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+        for v9, n3 in meshes:
+            s1.add_mesh(v9, n3)
+        for k, org, om in motions:
+            s1.set_mesh_motion(k, org, om)
+        if solver:
+            s1.set_jkr_solver(solver, net)
+        for c in chunks:
+            s1.step(c)
+        ref = s1.download()
+        ro, rp, rx = s1.history()
+        n = ref["x"].shape[0]
+        for key in FIELDS:
+            assert np.array_equal(got[key][off:off + n], ref[key]), (q, key)
+        a, b = go[off], go[off + n]
+        assert np.array_equal(go[off:off + n + 1] - a, ro), q
+        assert np.array_equal(gp[a:b] - off, rp), q
+        assert np.array_equal(gx[a:b], rx), q
+        nh += rp.size
+        off += n
+    assert off == got["x"].shape[0]
+    return e, nh
+
+
+@pytest.mark.parametrize("normal,tangential,solver,gamma", [
+    (abi.RP_NORMAL_JKR, abi.RP_TANGENTIAL_MINDLIN, "shortcut", 0.05),
+    (abi.RP_NORMAL_JKR, abi.RP_TANGENTIAL_MINDLIN, "exact", 0.05),
+    (abi.RP_NORMAL_HERTZ, abi.RP_TANGENTIAL_LSC, None, 0.0),
+    (abi.RP_NORMAL_SJKR, abi.RP_TANGENTIAL_MINDLIN, None, 0.05),
+], ids=["jkr-mindlin-shortcut", "jkr-mindlin-exact", "hertz-lsc", "sjkr-mindlin"])
+def test_batched_instances_keep_their_history(normal, tangential, solver, gamma):
+    """A friction / restitution / surface-energy sweep in one batched engine:
+    every instance ends where a single engine puts it, history included."""
+    rng = np.random.default_rng(31 + normal)
+    inst = []
+    for q in range(4):
+        c = tuple(int(v) for v in rng.integers(4, 8, 3))
+        sc = dense_scene(counts=c, spacing=0.0095, seed=300 + q, vel=0.3, omega=30.0,
+                         centre=tuple(rng.uniform(-0.01, 0.01, 3)))
+        mat = abi.RpMaterial(1e7, 0.3, 2500.0, float(rng.uniform(0.1, 0.8)), float(rng.uniform(0.2, 0.9)),
+                             gamma * float(rng.uniform(0.5, 1.5)))
+        inst.append((sc, mat))
+    p = params(2e-6, (-0.06, -0.06, -0.06), (0.06, 0.06, 0.06), normal=normal, tangential=tangential,
+               gamma=gamma)
+    p.mode = abi.RP_MODE_PHYSICS_FIX
+    p.skin = 2e-5  # thin: the lists are rebuilt (and the particles re-sorted) several times
+    meshes = [scenes.box_mesh((-0.045, -0.045, -0.045), (0.045, 0.045, 0.045))]
+    print(f"case fix batched normal={normal} tangential={tangential} solver={solver}", flush=True)
+    e, nh = _batched_vs_single(p, inst, meshes, (1, 60, 139), solver=solver)
+    assert nh > 100
+    assert e.stats()["rebuilds"] >= 3
+
+
+def test_batched_drum_surrogate_keeps_history(drum_fix):
+    """The capacity configuration's drum as two instances of different
+    friction and restitution, with the surrogate: each ends where a single
+    engine with the same network puts it, history included."""
+    p, sc, meshes, net = drum_fix
+    sc = dict(sc, fixed=np.zeros(sc["x"].shape[0], np.uint8), v=np.zeros_like(sc["x"]),
+              w=np.zeros_like(sc["x"]))
+    m0 = p.particle
+    inst = [(sc, abi.RpMaterial(m0.youngs_modulus, m0.poissons_ratio, m0.density, mu, e, m0.surface_energy))
+            for mu, e in ((0.2, 0.4), (0.7, 0.8))]
+    motions = [(0, scenes.DRUM_ORIGIN, scenes.DRUM_OMEGA)]
+    print("case fix batched drum surrogate", flush=True)
+    e, nh = _batched_vs_single(p, inst, meshes, (50, 100), motions, solver="surrogate", net=net)
+    assert e.jkr_stats()["served"] > 0
+    assert nh > 100
+
+
+def _jkr_mindlin_fix():
+    p = params(2e-6, (-0.06, -0.06, -0.06), (0.06, 0.06, 0.06), normal=abi.RP_NORMAL_JKR,
+               tangential=abi.RP_TANGENTIAL_MINDLIN, gamma=0.05)
+    p.mode = abi.RP_MODE_PHYSICS_FIX
+    p.skin = 2e-5
+    return p
+
+
+def test_set_history_continues_a_run():
+    """A second engine given the first's full state and history carries on
+    exactly as the first does."""
+    sc = dense_scene(counts=(8, 8, 8), spacing=0.0095, seed=41, vel=0.3, omega=30.0)
+    p = _jkr_mindlin_fix()
+    meshes = [scenes.box_mesh((-0.045, -0.045, -0.045), (0.045, 0.045, 0.045))]
+    a = native.Engine(copy_params(p))
+    a.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+    for v9, n3 in meshes:
+        a.add_mesh(v9, n3)
+    a.set_jkr_solver("shortcut")
+    a.step(100)
+    st, hist = a.download(), a.history()
+    assert hist[1].size > 100
+    b = native.Engine(copy_params(p))
+    b.upload(st["x"], st["radius"], st["mass"], st["fixed"], v=st["v"], w=st["w"], a=st["a"], tau=st["tau"],
+             rx=st["rx"], rv=st["rv"], rw=st["rw"])
+    for v9, n3 in meshes:
+        b.add_mesh(v9, n3)
+    b.set_jkr_solver("shortcut")
+    b.set_history(*hist)
+    for h0, h1 in zip(hist, b.history()):
+        assert np.array_equal(h0, h1)
+    a.step(150)
+    b.step(150)
+    sa, sb = a.download(), b.download()
+    for k in FIELDS:
+        assert np.array_equal(sa[k], sb[k]), k
+    for h0, h1 in zip(a.history(), b.history()):
+        assert np.array_equal(h0, h1)
+    # without the history the run departs
+    c = native.Engine(copy_params(p))
+    c.upload(st["x"], st["radius"], st["mass"], st["fixed"], v=st["v"], w=st["w"], a=st["a"], tau=st["tau"],
+             rx=st["rx"], rv=st["rv"], rw=st["rw"])
+    for v9, n3 in meshes:
+        c.add_mesh(v9, n3)
+    c.set_jkr_solver("shortcut")
+    c.step(150)
+    assert not np.array_equal(c.download()["w"], sa["w"])
+
+
+def _fix_sim(seed, mu, gamma, counts):
+    from rusty_particles_amd import Simulation
+    sc = dense_scene(counts=counts, spacing=0.0095, seed=seed, vel=0.3, omega=30.0)
+    s = Simulation.create(2e-6, -0.06, -0.06, -0.06, 0.06, 0.06, 0.06, 0)
+    s.set_contact_models("jkr", "mindlin")
+    s.set_physics_fix(True)
+    s.set_engine_tuning(skin=2e-5)
+    s.set_particle_material(1e7, 0.3, 2500.0, mu, 0.5, gamma)
+    s.add_particles(sc["x"], sc["radius"], sc["mass"])
+    for i in range(sc["x"].shape[0]):
+        s.set_particle_velocity(i, *sc["v"][i])
+        s.set_particle_angular_velocity(i, *sc["w"][i])
+    return s
+
+
+def _same(s0, s1):
+    a, b = s0.state(), s1.state()
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_simulation_keeps_history_across_an_edit():
+    """An edit between two runs re-uploads the state; the history read with
+    it goes back too, so a no-op edit changes nothing, and an added particle
+    joins without history."""
+    s0, s1 = _fix_sim(5, 0.5, 0.05, (6, 6, 6)), _fix_sim(5, 0.5, 0.05, (6, 6, 6))
+    s0.run(2e-4)
+    s1.run(2e-4)
+    s1.set_particle_velocity(3, *s1.get_particle_velocity(3))  # re-upload, same state
+    assert s1._hist is not None and s1._hist[1].size > 50
+    s0.run(3e-4)
+    s1.run(3e-4)
+    _same(s0, s1)
+    s0.add_particle(0.05, 0.05, 0.05, 0.005, 1e-3)
+    s1.add_particle(0.05, 0.05, 0.05, 0.005, 1e-3)
+    s0.run(1e-4)
+    s1.run(1e-4)
+    _same(s0, s1)
+
+
+def test_ensemble_carries_member_history():
+    """Members that ran alone form an ensemble, run in it and then alone
+    again: every member ends where it would had it always run alone."""
+    from rusty_particles_amd import Ensemble
+    spec = [(1, 0.3, 0.05, (5, 6, 7)), (2, 0.7, 0.03, (8, 4, 5)), (3, 0.1, 0.08, (4, 4, 6))]
+    alone = [_fix_sim(*a) for a in spec]
+    together = [_fix_sim(*a) for a in spec]
+    for s in alone[:2] + together[:2]:  # the third member first runs in the ensemble
+        s.run(1e-4)
+    for s in alone:
+        s.run(2e-4)
+    ens = Ensemble(together)
+    ens.run(2e-4)
+    for a, b in zip(alone, together):
+        _same(a, b)
+        assert b._hist is not None
+    assert sum(b._hist[1].size for b in together) > 100
+    for s in alone + together:
+        s.run(1e-4)
+    for a, b in zip(alone, together):
+        _same(a, b)
+
+
+def test_set_history_refuses_malformed_tables():
+    sc = dense_scene(counts=(4, 4, 4), spacing=0.0095, seed=2, vel=0.3)
+    e = native.Engine(copy_params(_jkr_mindlin_fix()))
+    e.upload(sc["x"], sc["radius"], sc["mass"], v=sc["v"])
+    n = sc["x"].shape[0]
+    offs = np.zeros(n + 1, dtype=np.int64)
+    offs[1:] = 2
+    good = (offs, np.array([1, 2]), np.zeros((2, 3)))
+    e.set_history(*good)
+    bad = [
+        (offs + 1, good[1], good[2]),                       # does not start at 0
+        (offs, np.array([2, 1]), good[2]),                  # partners not ascending
+        (offs, np.array([0, 2]), good[2]),                  # the particle itself
+        (offs, np.array([1, n]), good[2]),                  # out of range
+    ]
+    for b in bad:
+        with pytest.raises(native.EngineError):
+            e.set_history(*b)
+    q = _jkr_mindlin_fix()
+    q.mode = abi.RP_MODE_PARITY
+    e2 = native.Engine(copy_params(q))
+    e2.upload(sc["x"], sc["radius"], sc["mass"], v=sc["v"])
+    with pytest.raises(native.EngineError):
+        e2.set_history(*good)
+    e2.set_history(np.zeros(n + 1, dtype=np.int64), np.zeros(0, dtype=np.int64), np.zeros((0, 3)))
+    e.step(3)  # still usable
+    assert np.isfinite(e.download()["x"]).all()
+
+
 def test_refusals():
     p = params(1e-5, (-0.05,) * 3, (0.05,) * 3, tangential=abi.RP_TANGENTIAL_MINDLIN)
     p.mode = abi.RP_MODE_PHYSICS_FIX
     sc = dense_scene(counts=(4, 4, 4), seed=2)
     e = native.Engine(copy_params(p))
-    with pytest.raises(native.EngineError):
-        e.upload_batch([32, 32], [p.particle, p.particle], sc["x"], sc["radius"], sc["mass"])
     e.upload(sc["x"], sc["radius"], sc["mass"])
     with pytest.raises(native.EngineError):
         e.autotune(5)
