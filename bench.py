@@ -34,7 +34,9 @@ spheres, r lognormal median 50 um, surface energy ~U[0.02, 0.08] J/m^2) in
 one batched engine, once per JKR solver (exact, shortcut, surrogate); value
 = the surrogate+batched form.  With --physics-fix (batch workloads): the
 physics-fix mode with Mindlin springs and their tangential history, and a
-friction coefficient drawn per instance on top of the sweep.
+friction coefficient drawn per instance on top of the sweep; on the single
+workloads it switches the mode only (the hopper's Hertz + Mindlin then keeps
+a real tangential history).
 
 --workload drum: the capacity run, 16,000,000 spheres (r ~ U[0.4, 0.8] mm,
 rho 2500) in a closed 16,384-triangle STL drum of radius 0.25 m turning at
@@ -435,6 +437,12 @@ def run_single(args) -> dict:
     from rusty_particles_amd import abi, native
     t0 = time.perf_counter()
     p, x, r, m, v, meshes, desc, fixed = build_workload(args.workload, args.n)
+    wname = args.workload
+    if args.physics_fix and p.mode != abi.RP_MODE_PHYSICS_FIX:
+        p.mode = abi.RP_MODE_PHYSICS_FIX
+        wname += "-fix"
+        desc += (", physics-fix mode (Mindlin with tangential history)"
+                 if p.tangential_model == abi.RP_TANGENTIAL_MINDLIN else ", physics-fix mode")
     n = x.shape[0]
     ntri = sum(v9.shape[0] for v9, _ in meshes)
     log(f"workload {args.workload}: {n} particles, {ntri} triangles, built in {time.perf_counter() - t0:.1f}s")
@@ -553,7 +561,7 @@ def run_single(args) -> dict:
     ms = native.probe_copy(nbytes)
     copy_gbs = 2 * nbytes / ms / 1e6
     log(f"copy probe: k_copy_probe read {nbytes} B + wrote {nbytes} B in {ms:.3f} ms ({copy_gbs:.0f} GB/s)")
-    traffic, traffic_src = traffic_of(args.workload)
+    traffic, traffic_src = traffic_of(wname)
 
     cpu = None
     if not args.no_cpu:
@@ -616,7 +624,7 @@ def run_single(args) -> dict:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} spheres, {ntri}-triangle STL, {desc}, dt={p.dt:g}, FP64",
+        "config": {"workload": f"{wname}: {n} spheres, {ntri}-triangle STL, {desc}, dt={p.dt:g}, FP64",
                    "particles": n, "triangles": ntri, "form": main_form},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src},
@@ -668,8 +676,9 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
     ap.add_argument("--physics-fix", action="store_true",
-                    help="batch workloads: the physics-fix mode with Mindlin springs and their tangential "
-                         "history, friction drawn per instance (the autotuner is off: it refuses a history)")
+                    help="the physics-fix mode (real shear modulus, Mindlin springs with their tangential "
+                         "history); batch workloads switch to Mindlin and draw a friction per instance; the "
+                         "autotuner is off (it refuses a history)")
     ap.add_argument("--autotune", type=int, default=None,
                     help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
                          "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
