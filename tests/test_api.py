@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import rusty_particles_amd as rp
-from rusty_particles_amd import native, simulation
+from rusty_particles_amd import abi, native, simulation
 
 
 def test_hello_and_threads():
@@ -180,6 +180,7 @@ def test_every_mutator_bumps_the_version():
         lambda: sim.set_jkr_solver("shortcut"),
         lambda: sim.add_mesh_arrays(*scenes.box_mesh((-1, -1, -1), (1, 1, 1))),
         lambda: sim.set_mesh_motion(0, (0, 0, 0), (0, 1, 0)),
+        lambda: sim.set_physics_fix(True),
     ]
     for f in calls:
         v = sim._version
@@ -205,3 +206,26 @@ def test_overlap_warning_once_per_counted_step(capsys):
     err = capsys.readouterr().err.strip().splitlines()
     assert err == [simulation.OVERLAP_WARNING] * 4
     assert err[0] == "Warning: Excessive particle overlap detected (>20% of radius). Simulation may be unstable."
+
+
+def test_history_bookkeeping_without_an_engine():
+    """The physics-fix history read with the host state: kept only in that
+    mode with a spring law, and particles added later join without entries
+    (their offsets repeat the last one)."""
+    sim = rp.Simulation.create(1e-3, -1, -1, -1, 1, 1, 1, 0)
+    sim.add_particles([[0, 0, 0], [0.01, 0, 0], [0.02, 0, 0]], 0.006, 1.0)
+    sim.set_contact_models("hertz", "coulomb")
+    assert not sim._keeps_history()
+    sim.set_physics_fix(True)
+    assert not sim._keeps_history()  # Coulomb friction has no history
+    sim.set_contact_models("hertz", "mindlin")
+    assert sim._keeps_history()
+    assert sim._params.mode == abi.RP_MODE_PHYSICS_FIX
+    sim._hist = (np.array([0, 1, 3, 4]), np.array([1, 0, 2, 1]), np.zeros((4, 3)))
+    sim.add_particle(0.5, 0, 0, 0.006, 1.0)
+    sim.add_particles([[0.6, 0, 0], [0.7, 0, 0]], 0.006, 1.0)
+    offs, part, xi = sim._hist
+    assert offs.tolist() == [0, 1, 3, 4, 4, 4, 4]
+    assert part.tolist() == [1, 0, 2, 1] and xi.shape == (4, 3)
+    sim.set_physics_fix(False)
+    assert sim._params.mode == abi.RP_MODE_PARITY and not sim._keeps_history()
