@@ -11,6 +11,8 @@
 //                   still arbitrary)
 //     k_rank_permute rank inside the cell by stable id (deterministic), move
 //                   the particle state to its new slot, drop removed ones
+//     k_cell_rmax   largest radius per cell (cells well below the largest
+//                   cutoff only: k_build then skips cells out of reach)
 //     k_build       Verlet neighbour list per particle (cutoff + skin), the
 //                   last workgroup to finish swaps the state buffers
 //   k_force         gather over the neighbour list: exact pair predicate,
@@ -228,6 +230,7 @@ struct KParams {
   // (c >> gsh[k]) * ghi[k] + (c & (2^gsh[k] - 1)) * glo[k]  (see cell_linear)
   int gsh[3], ghi[3], glo[3];
   unsigned int ncells;      // fine cells of one instance
+  int cell_reach;           // list build tests each cell's reach (cells below 3/4 of the largest cutoff)
   unsigned int ncells_all;  // ncells * ninst: sort buckets (instance-major)
   int ninst;
   const InstConsts* itab;   // [ninst] per-instance materials (batched engines)
@@ -696,6 +699,38 @@ __device__ inline double zero_threshold(const KParams& P, const Stat& U, int i, 
   return zero_threshold_r(P, ic ? ic->mc.pp : P.mc.pp, r, P.normal);
 }
 
+// Largest radius among each cell's particles (0 for an empty cell), rounded
+// up to float, for the list build's per-cell reach test.
+__global__ void __launch_bounds__(kBlock) k_cell_rmax(const Ctl* ctl, int par, KParams P, Dyn D,
+                                                      const uint32_t* cell_start, float* cell_rmax) {
+  if (!ctl->rebuild[par]) return;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < P.ncells_all; c += gridDim.x * blockDim.x) {
+    double m = 0.0;
+    for (uint32_t j = cell_start[c], j1 = cell_start[c + 1]; j < j1; ++j)
+      m = fmax(m, reinterpret_cast<const double4*>(D.xr)[j].w);
+    float f = (float)m;
+    if ((double)f < m) f = nextafterf(f, HUGE_VALF);
+    cell_rmax[c] = f;
+  }
+}
+
+// Where slot i sits in its cell along each axis, for the distance from it to
+// the cells of its search box: f = fraction of the cell below it, w = cell
+// width; use = false on an axis where that distance is not bounded this way
+// (the box covers the whole periodic axis, or the particle lies outside the
+// grid and was clamped into an edge cell).
+struct CellReach {
+  double f[3], w[3];
+  bool use[3];
+};
+
+// Lower bound, in cells, on the distance along one axis from the particle to
+// any point of the cell `o` cells away (its fraction f), less a rounding margin.
+__device__ inline double cell_gap(int o, double f) {
+  const double g = o > 0 ? (double)o - f : (o < 0 ? (double)(-o - 1) + f : 0.0);
+  return fmax(g - 1e-9, 0.0);
+}
+
 // Candidates of slot i in the slots [s0, s1) of the re-ordered scratch
 // buffer D (one or more cells with consecutive numbers): appends every j with
 // |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin to i's list (the radius
@@ -751,18 +786,37 @@ __device__ inline int box_wrap(int c, const KParams& P, const bool* all) {
 // numbers along a row are merged into one slot range (two cell_start loads
 // per range instead of per cell).  The set of candidates is the same in any
 // walk order; the list is sorted by stable id afterwards.
-template <int A0, int A1, int A2>
-__device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t* cell_start, uint32_t ibase, int i,
-                                 dv3 xi, double ri, double zthr, const int* lo, const int* hi, const bool* all,
-                                 int* nbr) {
+// A cell is skipped when its nearest point is farther from the particle than
+// margin_factor*(r_i + the cell's largest radius) + skin, with rounding to
+// spare: no particle in it can pass build_range's test.  With a wide radius
+// distribution (cells sized for the largest pairs) most of the box is out of
+// reach of a small particle; empty cells cost no range loads either.
+template <int A0, int A1, int A2, bool REACH>
+__device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t* cell_start, const float* cell_rmax,
+                                 uint32_t ibase, int i, dv3 xi, double ri, double zthr, const int* ci, const int* lo,
+                                 const int* hi, const bool* all, const CellReach& G, double lim2_max, int* nbr) {
   int cnt = 0;
   for (int c2 = lo[A2]; c2 <= hi[A2]; ++c2) {
     const uint32_t t2 = ibase + cell_term<A2>(box_wrap<A2>(c2, P, all), P);
+    const double g2 = REACH && G.use[A2] ? cell_gap(c2 - ci[A2], G.f[A2]) * G.w[A2] : 0.0;
     for (int c1 = lo[A1]; c1 <= hi[A1]; ++c1) {
+      const double g1 = REACH && G.use[A1] ? cell_gap(c1 - ci[A1], G.f[A1]) * G.w[A1] : 0.0;
+      const double g12 = g2 * g2 + g1 * g1;
+      if (REACH && g12 > lim2_max) continue;  // the whole row is out of reach
       const uint32_t t21 = t2 + cell_term<A1>(box_wrap<A1>(c1, P, all), P);
       uint32_t r0 = 0, r1 = 0;  // current run of cells [r0, r1)
       for (int c0 = lo[A0]; c0 <= hi[A0]; ++c0) {
         const uint32_t c = t21 + cell_term<A0>(box_wrap<A0>(c0, P, all), P);
+        if constexpr (REACH) {
+          const double g0 = G.use[A0] ? cell_gap(c0 - ci[A0], G.f[A0]) * G.w[A0] : 0.0;
+          const double d2 = g12 + g0 * g0;
+          bool take = d2 <= lim2_max;
+          if (take && d2 > 0.0) {
+            const double l = (P.margin_factor * (ri + (double)cell_rmax[c]) + P.skin) * (1.0 + 1e-6);
+            take = d2 <= l * l;
+          }
+          if (!take) continue;
+        }
         if (r1 > r0 && c == r1) {
           r1 = c + 1u;
           continue;
@@ -771,7 +825,7 @@ __device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t*
         r0 = c;
         r1 = c + 1u;
       }
-      cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, zthr, nbr, cnt);
+      if (r1 > r0) cnt = build_range(P, D, cell_start[r0], cell_start[r1], i, xi, ri, zthr, nbr, cnt);
     }
   }
   return cnt;
@@ -780,9 +834,13 @@ __device__ inline int build_walk(const KParams& P, const Dyn& D, const uint32_t*
 // Verlet list of slot i (from the re-ordered scratch buffers D, U): every j
 // with |x_i - x_j|_MIC < margin_factor*(r_i + r_j) + skin; then copies slot
 // i back into the working buffers (W, V) and records the new slot count.
-__global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, Dyn D, Stat U, Dyn W, Stat V,
-                                                  const uint32_t* cell_start, int* nbr_count, int* nbr,
-                                                  float* xb, Hist HD, Hist HW) {
+// Four waves per SIMD: the per-cell reach test takes the kernel to 131 VGPRs
+// (three waves); capped at 128 it spills 12 B and builds the 2M JKR packing's
+// lists in 5.8 ms instead of 6.9 ms.
+template <bool REACH>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_build(Ctl* ctl, int par, KParams P, Dyn D, Stat U, Dyn W, Stat V,
+                                                  const uint32_t* cell_start, const float* cell_rmax, int* nbr_count,
+                                                  int* nbr, float* xb, Hist HD, Hist HW) {
   if (!ctl->rebuild[par]) return;
   const int n = (int)cell_start[P.ncells_all];  // alive particles after compaction
   const int cap = P.cap;
@@ -810,13 +868,32 @@ __global__ void __launch_bounds__(kBlock) k_build(Ctl* ctl, int par, KParams P, 
         hi[k] = min(P.gn[k] - 1, ci[k] + kk);
       }
     }
+    CellReach G{};
+    if constexpr (REACH) {
+      const double xk[3] = {xi.x, xi.y, xi.z};
+      for (int k = 0; k < 3; ++k) {
+        const double q = (xk[k] - P.go[k]) * P.ginv[k];  // fine_coord's scaled coordinate
+        const double fl = floor(q);
+        G.f[k] = q - fl;
+        G.w[k] = 1.0 / P.ginv[k];
+        G.use[k] = !all[k] && fabs(q) < 1.0e9 && (P.periodic[k] || (fl >= 0.0 && fl < (double)P.gn[k]));
+      }
+    }
+    const double lm = lim_max * (1.0 + 1e-6);
+    const double lim2_max = lm * lm;
     int cnt = 0;
     const double zthr = zero_threshold(P, U, i, ri);
     // the grid axis whose neighbouring cells have consecutive numbers walks
     // innermost, so each row of the search box is one or a few slot ranges
-    if (P.gstride[0] == 1) cnt = build_walk<0, 1, 2>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
-    else if (P.gstride[1] == 1) cnt = build_walk<1, 0, 2>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
-    else cnt = build_walk<2, 0, 1>(P, D, cell_start, ibase, i, xi, ri, zthr, lo, hi, all, nbr);
+    if (P.gstride[0] == 1)
+      cnt = build_walk<0, 1, 2, REACH>(P, D, cell_start, cell_rmax, ibase, i, xi, ri, zthr, ci, lo, hi, all, G, lim2_max,
+                                nbr);
+    else if (P.gstride[1] == 1)
+      cnt = build_walk<1, 0, 2, REACH>(P, D, cell_start, cell_rmax, ibase, i, xi, ri, zthr, ci, lo, hi, all, G, lim2_max,
+                                nbr);
+    else
+      cnt = build_walk<2, 0, 1, REACH>(P, D, cell_start, cell_rmax, ibase, i, xi, ri, zthr, ci, lo, hi, all, G, lim2_max,
+                                nbr);
     const int kept = min(cnt, P.nbr_cap);
     // Near entries first, each part by ascending stable id = ascending API
     // index: the reference's summation order inside one reference cell
@@ -2003,6 +2080,7 @@ struct rp_engine {
   uint32_t* d_cell_count = nullptr;
   uint32_t* d_cell_start = nullptr;
   uint32_t* d_block_sums = nullptr;
+  float* d_cell_rmax = nullptr;  // largest radius in each cell (k_cell_rmax)
   size_t cells_alloc = 0;
   int* d_nbr_count = nullptr;
   int* d_nbr = nullptr;
@@ -2231,6 +2309,11 @@ rp_status compute_geometry(rp_engine* e) {
     if (total * (double)std::max(e->ninst, 1) <= kMaxCells) break;
     cs *= 1.25;
   }
+  // Cells well below the largest cutoff (a wide radius distribution) put most
+  // of a small particle's search box out of its reach: the list build then
+  // skips cells by their largest radius.  With cells of a full cutoff the
+  // 27-cell box is nearly all in reach and the test costs more than it saves.
+  K.cell_reach = cs < 0.75 * cut_max ? 1 : 0;
   // slowest-varying axis = the one with most cells
   int order[3] = {0, 1, 2};
   std::sort(order, order + 3, [&](int a, int b) { return K.gn[a] < K.gn[b]; });
@@ -2524,9 +2607,12 @@ rp_status alloc_cells(rp_engine* e) {
   free_ptr(e->d_cell_count);
   free_ptr(e->d_cell_start);
   free_ptr(e->d_block_sums);
+  free_ptr(e->d_cell_rmax);
   e->d_cell_count = e->d_cell_start = e->d_block_sums = nullptr;
+  e->d_cell_rmax = nullptr;
   HIP_TRY(hipMalloc(&e->d_cell_count, need * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&e->d_cell_start, need * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&e->d_cell_rmax, need * sizeof(float)));
   size_t nb = (need + kScanChunk - 1) / kScanChunk + 1;
   HIP_TRY(hipMalloc(&e->d_block_sums, nb * sizeof(uint32_t)));
   HIP_TRY(hipMemsetAsync(e->d_cell_count, 0, need * sizeof(uint32_t), e->stream));
@@ -2626,8 +2712,16 @@ rp_status launch_rebuild(rp_engine* e) {
   const Hist HS = e->hist_on ? e->hist[e->cur] : Hist{}, HD = e->hist_on ? e->hist[e->cur ^ 1] : Hist{};
   hipLaunchKernelGGL(k_rank_permute, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, W, V, D, U, e->d_key,
                      e->d_cell_start, e->d_tmp, HS, HD);
-  hipLaunchKernelGGL(k_build, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D, U, W, V, e->d_cell_start,
-                     e->d_nbr_count, e->d_nbr, e->d_xb, HD, HS);
+  if (K.cell_reach) {
+    const int gc = std::min((int)((K.ncells_all + kBlock - 1) / kBlock), kMaxGrid);
+    hipLaunchKernelGGL(k_cell_rmax, dim3(std::max(gc, 1)), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D,
+                       e->d_cell_start, e->d_cell_rmax);
+    hipLaunchKernelGGL(k_build<true>, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D, U, W, V,
+                       e->d_cell_start, e->d_cell_rmax, e->d_nbr_count, e->d_nbr, e->d_xb, HD, HS);
+  } else {
+    hipLaunchKernelGGL(k_build<false>, dim3(g), dim3(kBlock), 0, e->stream, e->d_ctl, par, K, D, U, W, V,
+                       e->d_cell_start, e->d_cell_rmax, e->d_nbr_count, e->d_nbr, e->d_xb, HD, HS);
+  }
   HIP_TRY(hipGetLastError());
   return RP_OK;
 }
@@ -4073,6 +4167,7 @@ void rp_engine_destroy(rp_engine* e) {
   free_ptr(e->d_cell_count);
   free_ptr(e->d_cell_start);
   free_ptr(e->d_block_sums);
+  free_ptr(e->d_cell_rmax);
   free_ptr(e->d_tris);
   free_ptr(e->d_tri_start);
   free_ptr(e->d_tri_idx);
