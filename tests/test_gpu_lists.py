@@ -1,0 +1,71 @@
+"""The list build's per-cell reach test on the GPU against the oracle.
+
+With a wide radius distribution the fine cells are sized for the largest
+pairs (half the largest cutoff), and the list build skips every cell whose
+nearest point lies beyond margin*(r_i + the cell's largest radius) + skin.
+The lists must stay supersets of the contact pairs: pair lists, states and
+counters bit-identical to the oracle through several rebuilds, with walls on
+four sides (particles near the grid's edge cells) and one periodic axis
+(gaps measured across the wrap).
+"""
+import numpy as np
+import pytest
+
+from helpers import copy_params, params
+from oracle import OracleSimulation
+from rusty_particles_amd import abi, native, scenes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "w", "a", "tau", "rx", "rv", "rw")
+
+
+def _bimodal(n, seed):
+    rng = np.random.default_rng(seed)
+    big = np.arange(n) < n // 10  # placed first
+    radii = np.where(big, 2.5e-3, 0.5e-3) * rng.uniform(0.97, 1.03, n)
+    lo, hi = (-0.028, -0.028, -0.03), (0.028, 0.028, 0.03)
+    x, r = scenes.rsa_radii(radii, lo, hi, (False, False, True), seed=seed)
+    r = r * 1.05  # grown after placement: ~1,300 pairs in contact from the start
+    v = rng.normal(0.0, 0.5, (n, 3))
+    return x, r, scenes.sphere_mass(r, 2500.0), v
+
+
+@pytest.mark.parametrize("normal", [abi.RP_NORMAL_HERTZ, abi.RP_NORMAL_SJKR], ids=["hertz", "sjkr"])
+def test_wide_radius_lists_match_the_oracle(normal):
+    x, r, m, v = _bimodal(8000, seed=71 + normal)
+    p = params(1e-5, (-0.03, -0.03, -0.03), (0.03, 0.03, 0.03), normal=normal, gamma=0.01,
+               periodic=(0, 0, 1))
+    p.skin = 1e-4  # thin: the lists are rebuilt several times
+    meshes = [scenes.box_mesh((-0.029, -0.029, -0.04), (0.029, 0.029, 0.04))]
+    print(f"case wide radii normal={normal}", flush=True)
+    e = native.Engine(copy_params(p))
+    e.upload(x, r, m, v=v)
+    for v9, n3 in meshes:
+        e.add_mesh(v9, n3)
+    e.set_record(True)
+    o = OracleSimulation(p.dt, None, None, params=copy_params(p))
+    o.set_state(x, r, m, v=v)
+    for v9, n3 in meshes:
+        o.add_mesh_arrays(v9, n3)
+    o.set_fast(True)
+    o.set_record(True)
+    # cells of half the largest cutoff: the reach test is on
+    margin = 1.5 if normal == abi.RP_NORMAL_SJKR else 1.0
+    cut_max = margin * 2.0 * r.max() + p.skin
+    g = e.cells()["grid"]
+    assert 1.0 / g["inv_cell"][0] < 0.75 * cut_max
+    npairs = 0
+    for chunk in (1, 100, 99):
+        o.step(chunk)
+        e.step(chunk)
+        oo, oj = o.pairs()
+        eo, ej = e.pairs()
+        assert np.array_equal(oo, eo) and np.array_equal(oj, ej), chunk
+        npairs = max(npairs, oj.size)
+        so, se = o.state(), e.download()
+        for k in FIELDS:
+            assert np.array_equal(so[k], se[k]), (chunk, k)
+    st = e.stats()
+    assert npairs > 500 and st["rebuilds"] >= 3 and st["wall_contacts"] > 0
+    assert st["excessive_overlap"] == o.overlap_flag()
