@@ -677,8 +677,7 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=None, help="ensemble size (256 batch, 64 jkr-batch)")
     ap.add_argument("--physics-fix", action="store_true",
                     help="the physics-fix mode (real shear modulus, Mindlin springs with their tangential "
-                         "history); batch workloads switch to Mindlin and draw a friction per instance; the "
-                         "autotuner is off (it refuses a history)")
+                         "history); batch workloads switch to Mindlin and draw a friction per instance")
     ap.add_argument("--autotune", type=int, default=None,
                     help="steps per chunk of the engine's skin / cell-size autotuner before timing (0 = off; "
                          "default 20 for the steady jkr and batch workloads, off for the hopper, whose particles "
@@ -690,7 +689,7 @@ def main() -> int:
                          "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.autotune is None:
-        args.autotune = 0 if args.workload in ("hopper", "box", "drum", "drum-fix") or args.physics_fix else 20
+        args.autotune = 0 if args.workload in ("hopper", "box", "drum", "drum-fix") else 20
     if args.steps is None:
         args.steps = 1000 if args.workload in ("drum", "drum-fix") else 5000
     if args.jkr_forms is None:

@@ -343,8 +343,9 @@ rp_status rp_engine_cells(rp_engine* e, int32_t rebuild, int64_t* ref_cell, uint
  * pass (JKR with the shortcut or surrogate, sJKR; Coulomb or spring
  * tangential law) it then times the other tiling of that pass
  * (rp_engine_set_tiling) with the chosen skin and cell and keeps the faster.
- * The particle state is restored, so the simulation continues as if the call
- * had not been made (skin, cell size and tiling never change a result).  A
+ * The particle state is restored, the physics-fix mode's tangential history
+ * with it, so the simulation continues as if the call had not been made
+ * (skin, cell size and tiling never change a result).  A
  * tuned skin is widened later, up to the automatic skin, when more than a
  * quarter of the rebuild checks of 200 steps rebuilt the lists. */
 rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, double* cell_out);

@@ -3937,8 +3937,20 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   if (steps == 0) return fail(RP_ERR_INVALID, "steps must be positive");
   rp_status st = prepare(e);
   if (st) return st;
-  if (e->hist_on)  // the candidates would advance the history tables the snapshot does not hold
-    return fail(RP_ERR_STATE, "the autotuner does not run with a tangential history (physics-fix mode)");
+  // the tangential history (physics-fix mode) in API order, so that it can
+  // be loaded into tables of any capacity after each candidate's rebuild
+  std::vector<int64_t> h_off, h_part;
+  std::vector<double> h_xi;
+  size_t h_tot = 0;
+  if (e->hist_on) {
+    size_t n = 0;
+    if ((st = rp_engine_count(e, &n))) return st;
+    if ((st = rp_engine_history(e, nullptr, nullptr, nullptr, 0, &h_tot))) return st;
+    h_off.resize(n + 1);
+    h_part.resize(std::max<size_t>(h_tot, 1));
+    h_xi.resize(3 * std::max<size_t>(h_tot, 1));
+    if ((st = rp_engine_history(e, h_off.data(), h_part.data(), h_xi.data(), h_part.size(), &h_tot))) return st;
+  }
   e->skin_tuned = false;  // no guard while the candidates run
   HIP_TRY(hipStreamSynchronize(e->stream));
   // snapshot of everything a step reads: state buffers, control block, host
@@ -3956,6 +3968,12 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
                  label0 = e->label_host, chain0 = e->chain_no;
   const bool at_rest0 = e->at_rest, timing0 = e->timing;
   const double skin0 = e->prm.skin, cell0 = e->prm.cell_size;
+  // after restore() and the candidate's skin and cell: the lists are built
+  // with them and the history goes back into the tables
+  auto reload_hist = [&]() -> rp_status {
+    if (!e->hist_on) return RP_OK;
+    return rp_engine_set_history(e, h_off.data(), h_part.data(), h_xi.data(), h_tot);
+  };
   auto restore = [&]() -> rp_status {
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->cur = cur0;
@@ -4002,6 +4020,7 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
     if ((st = restore())) break;
     e->prm.skin = cand[q].first;
     e->prm.cell_size = cand[q].second;
+    if ((st = reload_hist())) break;
     if ((st = rp_engine_step(e, 2))) break;
     if ((st = read_ctl(e))) break;
     const unsigned long long r0 = e->host_ctl.rebuilds;
@@ -4039,6 +4058,7 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
       e->prm.skin = best_skin;
       e->prm.cell_size = best_cell;
       (void)rp_engine_set_tiling(e, pass == 0 ? tiling0 : other);
+      if ((st = reload_hist())) break;
       if ((st = rp_engine_step(e, 2))) break;
       if ((st = read_ctl(e))) break;
       const unsigned long long r0 = e->host_ctl.rebuilds;
@@ -4063,6 +4083,7 @@ rp_status rp_engine_autotune(rp_engine* e, uint32_t steps, double* skin_out, dou
   rp_status rs = restore();
   e->prm.skin = st ? skin0 : best_skin;
   e->prm.cell_size = st ? cell0 : best_cell;
+  if (!rs) rs = reload_hist();
   if (!st && !rs && best_skin != skin0) {
     e->skin_tuned = true;
     e->guard_steps = 0;

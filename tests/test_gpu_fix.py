@@ -17,7 +17,9 @@
   (rp_engine_set_history): the run continues bit for bit as if never
   interrupted -- through the engine, through Simulation (an edit between
   runs) and through an Ensemble formed from members that have run.
-* What the mode refuses: the autotuner with a history; malformed histories.
+* The autotuner with a history: the run goes on exactly as if it had not
+  been tuned, history included.
+* What the mode refuses: malformed histories.
 """
 import math
 
@@ -436,13 +438,28 @@ def test_set_history_refuses_malformed_tables():
     assert np.isfinite(e.download()["x"]).all()
 
 
-def test_refusals():
-    p = params(1e-5, (-0.05,) * 3, (0.05,) * 3, tangential=abi.RP_TANGENTIAL_MINDLIN)
-    p.mode = abi.RP_MODE_PHYSICS_FIX
-    sc = dense_scene(counts=(4, 4, 4), seed=2)
-    e = native.Engine(copy_params(p))
-    e.upload(sc["x"], sc["radius"], sc["mass"])
-    with pytest.raises(native.EngineError):
-        e.autotune(5)
-    e.step(3)  # still usable
-    assert np.isfinite(e.download()["x"]).all()
+def test_autotune_keeps_the_history():
+    """Tuning skin, cell size and tiling mid-run (its candidates step the
+    state and the history, then restore both) changes nothing: the tuned
+    engine ends where an untuned one does, history included."""
+    sc = dense_scene(counts=(8, 8, 8), spacing=0.0095, seed=43, vel=0.3, omega=30.0)
+    p = _jkr_mindlin_fix()
+    meshes = [scenes.box_mesh((-0.045, -0.045, -0.045), (0.045, 0.045, 0.045))]
+    runs = []
+    for tune in (False, True):
+        e = native.Engine(copy_params(p))
+        e.upload(sc["x"], sc["radius"], sc["mass"], sc["fixed"], v=sc["v"], w=sc["w"])
+        for v9, n3 in meshes:
+            e.add_mesh(v9, n3)
+        e.set_jkr_solver("shortcut")
+        e.step(60)
+        assert e.history()[1].size > 50
+        if tune:
+            e.autotune(20)
+        e.step(140)
+        runs.append((e.download(), e.history()))
+    (sa, ha), (sb, hb) = runs
+    for k in FIELDS:
+        assert np.array_equal(sa[k], sb[k]), k
+    for a, b in zip(ha, hb):
+        assert np.array_equal(a, b)
