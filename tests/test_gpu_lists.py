@@ -6,7 +6,8 @@ nearest point lies beyond margin*(r_i + the cell's largest radius) + skin.
 The lists must stay supersets of the contact pairs: pair lists, states and
 counters bit-identical to the oracle through several rebuilds, with walls on
 four sides (particles near the grid's edge cells) and one periodic axis
-(gaps measured across the wrap).
+(gaps measured across the wrap); and batched JKR packings, whose cells are
+numbered instance-major, against single runs.
 """
 import numpy as np
 import pytest
@@ -69,3 +70,34 @@ def test_wide_radius_lists_match_the_oracle(normal):
     st = e.stats()
     assert npairs > 500 and st["rebuilds"] >= 3 and st["wall_contacts"] > 0
     assert st["excessive_overlap"] == o.overlap_flag()
+
+
+def test_batched_jkr_packings_with_wide_radii_match_single_runs():
+    """Five periodic JKR packings (r lognormal, sigma 0.3) in one batched
+    engine with a thin skin, so the per-cell reach test runs in every
+    rebuild over the instance-major cells: each instance ends bit-identical
+    to a single engine given that instance alone."""
+    p, counts, mats, x, r, m, v = scenes.jkr_ensemble_workload(5, nmin=1500, nmax=4000)
+    p.skin = 5e-9  # ~10 steps of motion: a rebuild every few steps
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    print(f"case wide radii batched counts={counts}", flush=True)
+    e = native.Engine(copy_params(p))
+    e.upload_batch(counts, mats, x, r, m, v=v)
+    e.set_jkr_solver("shortcut")
+    cut_max = 1.5 * 2.0 * r.max() + p.skin
+    assert 1.0 / e.cells()["grid"]["inv_cell"][0] < 0.75 * cut_max
+    for c in (1, 59):
+        e.step(c)
+    got = e.download()
+    assert e.stats()["rebuilds"] >= 3 and e.stats()["contacts"] > 1000
+    for k in range(len(counts)):
+        q = copy_params(p)
+        q.particle = mats[k]
+        s1 = native.Engine(q)
+        sl = slice(int(offs[k]), int(offs[k + 1]))
+        s1.upload(x[sl], r[sl], m[sl], v=v[sl])
+        s1.set_jkr_solver("shortcut")
+        s1.step(60)
+        ref = s1.download()
+        for key in FIELDS:
+            assert np.array_equal(got[key][sl], ref[key]), (k, key)
