@@ -75,12 +75,17 @@ constexpr int kBlock = 256;
 #define RP_FORCE_BLOCK 128
 #endif
 constexpr int kForceBlock = RP_FORCE_BLOCK;
-// Four waves per SIMD for every force kernel: the JKR and sJKR variants need
-// 140-150 VGPRs (three waves) unconstrained; capped at 128 they spill a few
-// registers and still run 5-12 % faster (batch 592 -> 532 us, JKR shortcut
-// 406 -> 361 us per launch); the hopper kernel fits in 128 anyway.
+// Four waves per SIMD for the force kernels without a tangential history:
+// the JKR and sJKR variants need 140-150 VGPRs (three waves) unconstrained;
+// capped at 128 they spill a few registers and still run 5-12 % faster (batch
+// 592 -> 532 us, JKR shortcut 406 -> 361 us per launch); the hopper kernel
+// fits in 128 anyway.  The physics-fix history kernels spill 60-200 B at 128
+// and run faster at three waves with none (16M drum shortcut 2,849 -> 2,684
+// us, surrogate 3,626 -> 3,003 us; JKR ensemble 206 -> 193 and 256 -> 209 us).
+// The expression is evaluated per instantiation (HIST is the kernel's
+// template argument).
 #ifndef RP_FORCE_ATTR
-#define RP_FORCE_ATTR __attribute__((amdgpu_waves_per_eu(4)))
+#define RP_FORCE_ATTR __attribute__((amdgpu_waves_per_eu(HIST ? 3 : 4)))
 #endif
 constexpr uint8_t FLAG_FIXED = 1;
 constexpr uint8_t FLAG_DEAD = 2;
